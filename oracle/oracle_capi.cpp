@@ -1,0 +1,326 @@
+// C entry points of the CPU oracle (test infrastructure only; see oracle.hpp).
+//
+// Two groups:
+//   * or_*_seq: drive one functor through a sequence of pair states, the same
+//     way oracle/ref_harness.cpp drives the reference functor headers, so the
+//     two can be compared output for output.
+//   * or_sys_*: the whole system (functor fold + acceleration pass +
+//     integrator + neighbour list), used as the parity reference for the HIP
+//     path on identical inputs.
+#include "oracle.hpp"
+
+#include <cstring>
+
+using namespace oracle;
+
+namespace {
+v3 ld(const double *p) { return {p[0], p[1], p[2]}; }
+void st(double *p, v3 a) {
+    p[0] = a.x;
+    p[1] = a.y;
+    p[2] = a.z;
+}
+void st_spring(double *p, const spring3 &s) {
+    for (int k = 0; k < 3; k++) st(p + 3 * k, s.s[k]);
+}
+contact_params cp_from(const double *q) {
+    contact_params c;
+    std::memcpy(&c, q, sizeof(c));
+    return c;
+}
+sinter_params sp_from(const double *q) {
+    sinter_params s;
+    std::memcpy(&s, q, sizeof(s));
+    return s;
+}
+hamaker_params hp_from(const double *q) {
+    hamaker_params h;
+    std::memcpy(&h, q, sizeof(h));
+    return h;
+}
+}  // namespace
+
+extern "C" {
+
+// Pair state per step: xi, xj, vi, vj, wi, wj (18 doubles). Output per step:
+// (i=0,j=1) lin, ang then (i=1,j=0) lin, ang (12 doubles), and the two spring
+// triples after the step (18 doubles) when springs_out is not null.
+void or_contact_seq(const double *params18, size_t nsteps, const double *states, double *out,
+                    double *springs_out) {
+    contact_params p = cp_from(params18);
+    spring3 s01{{vzero, vzero, vzero}}, s10{{vzero, vzero, vzero}};
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 18 * k;
+        v3 x0 = ld(s), x1 = ld(s + 3), v0 = ld(s + 6), v1 = ld(s + 9), w0 = ld(s + 12), w1 = ld(s + 15);
+        bool r;
+        acc2 e01 = contact_eval(p, x0, x1, v0, v1, w0, w1, s01, r);
+        acc2 e10 = contact_eval(p, x1, x0, v1, v0, w1, w0, s10, r);
+        st(out + 12 * k, e01.lin);
+        st(out + 12 * k + 3, e01.ang);
+        st(out + 12 * k + 6, e10.lin);
+        st(out + 12 * k + 9, e10.ang);
+        if (springs_out) {
+            st_spring(springs_out + 18 * k, s01);
+            st_spring(springs_out + 18 * k + 9, s10);
+        }
+    }
+}
+
+// alt_sinter with its embedded contact functor; bonding decided from the first
+// state's positions with the reference predicate. bonded_out gets 1/0.
+void or_sinter_seq(const double *sinter13, const double *contact18, size_t nsteps, const double *states,
+                   double *out, double *springs_out, int *bonded_out) {
+    sinter_params sp = sp_from(sinter13);
+    contact_params cp = cp_from(contact18);
+    std::vector<v3> x0 = {ld(states), ld(states + 3)};
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped;
+    sinter_topology(2, x0, sp.r_part, sp.d_crit, bonds, skipped);
+    bool bonded = !bonds.empty();
+    *bonded_out = bonded ? 1 : 0;
+    spring3 s01{{vzero, vzero, vzero}}, s10{{vzero, vzero, vzero}};  // sinter springs
+    spring3 c01{{vzero, vzero, vzero}}, c10{{vzero, vzero, vzero}};  // contact springs
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 18 * k;
+        v3 x0_ = ld(s), x1 = ld(s + 3), v0 = ld(s + 6), v1 = ld(s + 9), w0 = ld(s + 12), w1 = ld(s + 15);
+        acc2 e01, e10;
+        bool r;
+        if (bonded) {
+            e01 = sinter_bonded_eval(sp, x0_, x1, v0, v1, w0, w1, s01);
+            e10 = sinter_bonded_eval(sp, x1, x0_, v1, v0, w1, w0, s10);
+        } else {
+            e01 = contact_eval(cp, x0_, x1, v0, v1, w0, w1, c01, r);
+            e10 = contact_eval(cp, x1, x0_, v1, v0, w1, w0, c10, r);
+        }
+        st(out + 12 * k, e01.lin);
+        st(out + 12 * k + 3, e01.ang);
+        st(out + 12 * k + 6, e10.lin);
+        st(out + 12 * k + 9, e10.ang);
+        if (springs_out) {
+            st_spring(springs_out + 18 * k, s01);
+            st_spring(springs_out + 18 * k + 9, s10);
+        }
+    }
+}
+
+// mode 1: hamaker_functor, mode 2: limited_range_hamaker_functor.
+void or_hamaker_seq(int mode, const double *ham5, size_t nsteps, const double *states, double *out) {
+    hamaker_params hp = hp_from(ham5);
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 18 * k;
+        v3 x0 = ld(s), x1 = ld(s + 3);
+        acc2 e01 = mode == 1 ? hamaker_eval(hp, x0, x1) : lr_hamaker_eval(hp, x0, x1);
+        acc2 e10 = mode == 1 ? hamaker_eval(hp, x1, x0) : lr_hamaker_eval(hp, x1, x0);
+        st(out + 12 * k, e01.lin);
+        st(out + 12 * k + 3, e01.ang);
+        st(out + 12 * k + 6, e10.lin);
+        st(out + 12 * k + 9, e10.ang);
+    }
+}
+
+// Container fold of {alt_sinter (with contact), limited_range_hamaker}.
+void or_fold_seq(const double *sinter13, const double *contact18, const double *ham5, size_t nsteps,
+                 const double *states, double *out) {
+    sinter_params sp = sp_from(sinter13);
+    contact_params cp = cp_from(contact18);
+    hamaker_params hp = hp_from(ham5);
+    std::vector<v3> x0 = {ld(states), ld(states + 3)};
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped;
+    sinter_topology(2, x0, sp.r_part, sp.d_crit, bonds, skipped);
+    bool bonded = !bonds.empty();
+    spring3 s01{{vzero, vzero, vzero}}, s10{{vzero, vzero, vzero}};
+    spring3 c01{{vzero, vzero, vzero}}, c10{{vzero, vzero, vzero}};
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 18 * k;
+        v3 x0_ = ld(s), x1 = ld(s + 3), v0 = ld(s + 6), v1 = ld(s + 9), w0 = ld(s + 12), w1 = ld(s + 15);
+        bool r;
+        acc2 b01 = bonded ? sinter_bonded_eval(sp, x0_, x1, v0, v1, w0, w1, s01)
+                          : contact_eval(cp, x0_, x1, v0, v1, w0, w1, c01, r);
+        acc2 b10 = bonded ? sinter_bonded_eval(sp, x1, x0_, v1, v0, w1, w0, s10)
+                          : contact_eval(cp, x1, x0_, v1, v0, w1, w0, c10, r);
+        acc2 e01 = b01 + lr_hamaker_eval(hp, x0_, x1);
+        acc2 e10 = b10 + lr_hamaker_eval(hp, x1, x0_);
+        st(out + 12 * k, e01.lin);
+        st(out + 12 * k + 3, e01.ang);
+        st(out + 12 * k + 6, e10.lin);
+        st(out + 12 * k + 9, e10.ang);
+    }
+}
+
+// One particle against one triangular facet carrying {surface contact,
+// surface hamaker}. Facet: 3 vertices (9) + velocity (3). Per step state:
+// x, v, w (9). Output per step: lin, ang (6). The facet is advanced by
+// dt_advance after every step (triangular_facet.h:62-66).
+void or_facet_seq(const double *verts9, const double *vfacet3, const double *contact18, const double *ham5,
+                  double dt_advance, size_t nsteps, const double *states, double *out) {
+    contact_params cp = cp_from(contact18);
+    hamaker_params hp = hp_from(ham5);
+    facet f = make_facet(ld(vfacet3), ld(verts9), ld(verts9 + 3), ld(verts9 + 6));
+    spring3 sp{{vzero, vzero, vzero}};
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 9 * k;
+        v3 x = ld(s), v = ld(s + 3), w = ld(s + 6);
+        v3 xf = closest_point_on_triangle(f, x);
+        acc2 e = surface_contact_eval(cp, xf, f.v_facet, x, v, w, sp) + surface_hamaker_eval(hp, xf, x);
+        st(out + 6 * k, e.lin);
+        st(out + 6 * k + 3, e.ang);
+        for (int q = 0; q < 3; q++) f.vert[q] = f.vert[q] + f.v_facet * dt_advance;
+    }
+}
+
+// Closest point on a facet for a batch of query points (3 per point).
+void or_closest_point(const double *verts9, size_t npts, const double *pts, double *out) {
+    facet f = make_facet(vzero, ld(verts9), ld(verts9 + 3), ld(verts9 + 6));
+    for (size_t k = 0; k < npts; k++) st(out + 3 * k, closest_point_on_triangle(f, ld(pts + 3 * k)));
+}
+
+// alt_sinter topology; returns the number of bonds (pairs i<j in insertion
+// order written to pairs_out when it has room), skipped cycles to *skipped.
+size_t or_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs_out,
+                          size_t cap, size_t *skipped) {
+    std::vector<v3> x(n);
+    for (size_t i = 0; i < n; i++) x[i] = ld(x0 + 3 * i);
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    sinter_topology(n, x, r_part, d_crit, bonds, *skipped);
+    for (size_t b = 0; b < bonds.size() && b < cap; b++) {
+        pairs_out[2 * b] = bonds[b].first;
+        pairs_out[2 * b + 1] = bonds[b].second;
+    }
+    return bonds.size();
+}
+
+// ------------------------------------------------------------ system API ----
+void *or_sys_create(const system_params *p, size_t n, const double *x0, const double *v0, const double *th0,
+                    const double *w0) {
+    return new dem_system(*p, n, x0, v0, th0, w0);
+}
+void or_sys_destroy(void *s) { delete static_cast<dem_system *>(s); }
+
+void or_sys_set_facets(void *s, size_t nf, const double *verts, const double *vel) {
+    auto *S = static_cast<dem_system *>(s);
+    S->facets.clear();
+    S->surface_springs.assign(nf, {});
+    for (size_t f = 0; f < nf; f++) {
+        S->facets.push_back(make_facet(ld(vel + 3 * f), ld(verts + 9 * f), ld(verts + 9 * f + 3), ld(verts + 9 * f + 6)));
+        S->surface_springs[f].resize(S->n);
+    }
+}
+
+void or_sys_rebuild_neighbors(void *s, int brute) { static_cast<dem_system *>(s)->rebuild_neighbors(brute != 0); }
+void or_sys_compute_accelerations(void *s) { static_cast<dem_system *>(s)->compute_accelerations(); }
+
+// nsteps of: [rebuild when step_index % rebuild_every == 0] -> step -> advance facets.
+void or_sys_step(void *s, double dt, uint64_t nsteps, uint32_t rebuild_every, uint64_t *step_index) {
+    auto *S = static_cast<dem_system *>(s);
+    for (uint64_t k = 0; k < nsteps; k++) {
+        if (rebuild_every && S->p.neighbor_mode == 1 && (*step_index % rebuild_every) == 0)
+            S->rebuild_neighbors(false);
+        S->step(dt);
+        if (!S->facets.empty()) S->advance_facets(dt);
+        (*step_index)++;
+    }
+}
+
+void or_sys_get_state(void *s, double *x, double *v, double *th, double *w, double *a, double *al) {
+    auto *S = static_cast<dem_system *>(s);
+    for (size_t i = 0; i < S->n; i++) {
+        if (x) st(x + 3 * i, S->x[i]);
+        if (v) st(v + 3 * i, S->v[i]);
+        if (th) st(th + 3 * i, S->th[i]);
+        if (w) st(w + 3 * i, S->w[i]);
+        if (a) st(a + 3 * i, S->a[i]);
+        if (al) st(al + 3 * i, S->al[i]);
+    }
+}
+
+void or_sys_set_state(void *s, const double *x, const double *v, const double *th, const double *w) {
+    auto *S = static_cast<dem_system *>(s);
+    for (size_t i = 0; i < S->n; i++) {
+        if (x) S->x[i] = ld(x + 3 * i);
+        if (v) S->v[i] = ld(v + 3 * i);
+        if (th) S->th[i] = ld(th + 3 * i);
+        if (w) S->w[i] = ld(w + 3 * i);
+    }
+}
+
+double or_sys_time(void *s) { return static_cast<dem_system *>(s)->t; }
+
+// CSR neighbour list; returns the total entry count (idx written when it fits).
+size_t or_sys_get_neighbors(void *s, uint64_t *offsets, uint32_t *idx, size_t cap) {
+    auto *S = static_cast<dem_system *>(s);
+    size_t tot = 0;
+    for (size_t i = 0; i < S->n; i++) {
+        offsets[i] = tot;
+        for (uint32_t j : S->nbr[i]) {
+            if (tot < cap) idx[tot] = j;
+            tot++;
+        }
+    }
+    offsets[S->n] = tot;
+    return tot;
+}
+
+size_t or_sys_get_bonds(void *s, uint32_t *pairs, size_t cap, size_t *skipped) {
+    auto *S = static_cast<dem_system *>(s);
+    *skipped = S->bonds_skipped;
+    for (size_t b = 0; b < S->bonds.size() && b < cap; b++) {
+        pairs[2 * b] = S->bonds[b].first;
+        pairs[2 * b + 1] = S->bonds[b].second;
+    }
+    return S->bonds.size();
+}
+
+// which = 0: contact springs, 1: sinter (bond) springs. Entries row-major by
+// (i, j ascending). Returns the entry count.
+size_t or_sys_get_springs(void *s, int which, uint32_t *ij, double *springs, size_t cap) {
+    auto *S = static_cast<dem_system *>(s);
+    auto &rows = which == 0 ? S->contact_springs.rows : S->sinter_springs.rows;
+    size_t tot = 0;
+    for (size_t i = 0; i < S->n; i++)
+        for (auto &e : rows[i]) {
+            if (tot < cap) {
+                ij[2 * tot] = uint32_t(i);
+                ij[2 * tot + 1] = e.first;
+                st_spring(springs + 9 * tot, e.second);
+            }
+            tot++;
+        }
+    return tot;
+}
+
+// test/compute_energy.cpp:7-58 and test/mass_distribution.cpp:7-27.
+// out: ke_total, ke_trans, ke_rot, |P|, |L|, rg, cm[3]
+void or_sys_diagnostics(void *s, double mass, double inertia, double *out) {
+    auto *S = static_cast<dem_system *>(s);
+    double ke = 0.0;
+    for (size_t i = 0; i < S->n; i++) {
+        ke += mass * dot(S->v[i], S->v[i]) / 2.0;
+        ke += inertia * dot(S->w[i], S->w[i]) / 2.0;
+    }
+    double ket = 0.0, ker = 0.0;
+    for (size_t i = 0; i < S->n; i++) ket += mass * dot(S->v[i], S->v[i]);
+    for (size_t i = 0; i < S->n; i++) ker += inertia * dot(S->w[i], S->w[i]);
+    v3 P = vzero;
+    for (size_t i = 0; i < S->n; i++) P = P + S->v[i];
+    v3 L = vzero;
+    for (size_t i = 0; i < S->n; i++) L = L + inertia * S->w[i];
+    for (size_t i = 0; i < S->n; i++) L = L + cross(S->x[i], mass * S->v[i]);
+    v3 cm = vzero;
+    for (size_t i = 0; i < S->n; i++) cm = cm + S->x[i];
+    cm = cm / double(S->n);
+    double rg = 0.0;
+    for (size_t i = 0; i < S->n; i++) {
+        v3 d = S->x[i] - cm;
+        rg += dot(d, d);
+    }
+    out[0] = ke;
+    out[1] = ket / 2.0;
+    out[2] = ker / 2.0;
+    out[3] = mass * norm(P);
+    out[4] = norm(L);
+    out[5] = std::sqrt(rg / double(S->n));
+    st(out + 6, cm);
+}
+
+}  // extern "C"
