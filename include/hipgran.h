@@ -1,0 +1,184 @@
+/*
+ * hipgran — C ABI of the MI355X (gfx950) DEM per-timestep path.
+ *
+ * The reference (egor-demidov/libgran) is a header-only C++ template library
+ * with no binary interface; its drop-in surface is the granular_system family
+ * of templates. Each entry point below replaces one operation that the
+ * reference performs inside those templates or inside its time-stepping
+ * dependency; the citation says which. The C++ drop-in headers under
+ * libgran_amd/include/libgran/ call these functions, and so can any FFI
+ * (ctypes, cgo, JNI): plain pointers and sizes only, no exceptions.
+ *
+ * Conventions
+ *   - Vectors cross the ABI as AoS triples: double[3*n] (x0,y0,z0,x1,...).
+ *   - Every function returns hg_status; on failure hg_last_error() returns a
+ *     thread-local message for the last failing call on this thread.
+ *   - One hg_system is not internally synchronised: call it from one thread at
+ *     a time. Distinct systems may be used concurrently; each owns its own
+ *     HIP stream on the device that was current at hg_create.
+ */
+#ifndef HIPGRAN_H
+#define HIPGRAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hg_status {
+    HG_OK = 0,
+    HG_ERR_INVALID = 1,     /* bad argument or unsupported combination */
+    HG_ERR_OOM = 2,         /* device or host allocation failed */
+    HG_ERR_HIP = 3,         /* a HIP runtime call failed */
+    HG_ERR_NUMERIC = 4,     /* NaN/Inf reached the particle state */
+    HG_ERR_CAPACITY = 5,    /* a neighbour or pair-state table overflowed */
+    HG_ERR_NO_DEVICE = 6    /* no gfx950 device visible */
+} hg_status;
+
+/* contact_force_functor ctor arguments, same order and meaning
+ * (include/libgran/contact_force/contact_force.h:10-30, n_part omitted).
+ * Also used for surface_contact_force_functor (surface_contact_force.h:10-30). */
+typedef struct hg_contact_params {
+    double k, gamma_n, k_t, gamma_t, mu_s, phi_d, k_r, gamma_r, mu_r, phi_r,
+           k_o, gamma_o, mu_o, phi_o, r_part, mass, inertia, dt;
+} hg_contact_params;
+
+/* alt_sinter_functor ctor arguments (alt_sinter_bridge.h:19-36): bonded
+ * stiffness/damping, then critical_separation as d_crit. */
+typedef struct hg_sinter_params {
+    double k, gamma_n, k_t, gamma_t, k_r, gamma_r, k_o, gamma_o,
+           r_part, mass, inertia, dt, d_crit;
+} hg_sinter_params;
+
+/* hamaker_functor (hamaker_force.h:20-26) / limited_range_hamaker_functor
+ * (limited_range_hamaker_force.h:20-27) / surface_hamaker_functor
+ * (surface_hamaker_force.h:12-18). r_cutoff is read only in LIMITED mode. */
+typedef struct hg_hamaker_params {
+    double A, h0, r_cutoff, r_part, mass;
+} hg_hamaker_params;
+
+enum { HG_BINARY_NONE = 0, HG_BINARY_CONTACT = 1, HG_BINARY_SINTER = 2 };
+enum { HG_HAMAKER_NONE = 0, HG_HAMAKER_FULL = 1, HG_HAMAKER_LIMITED = 2 };
+/* HG_PAIRS_ALL: every ordered pair j != i, as granular_system
+ *   (granular_system.h:25-86).
+ * HG_PAIRS_NEIGHBOR_LIST: listed pairs only, as granular_system_neighbor_list
+ *   (granular_system_neighbor_list.h:26-87) with radius r_verlet. */
+enum { HG_PAIRS_ALL = 0, HG_PAIRS_NEIGHBOR_LIST = 1 };
+/* FP64: every operation in double, in the reference's arithmetic order.
+ * MIXED: distances, overlaps and gaps in double; force/torque arithmetic in
+ * float; positions, orientations, velocities and spring vectors kept and
+ * integrated in double. */
+enum { HG_PRECISION_FP64 = 0, HG_PRECISION_MIXED = 1 };
+
+typedef struct hg_params {
+    int32_t binary_model;        /* HG_BINARY_*: first functor of the binary container */
+    int32_t hamaker_model;       /* HG_HAMAKER_*: second functor of the binary container */
+    int32_t pair_mode;           /* HG_PAIRS_* */
+    int32_t precision;           /* HG_PRECISION_* */
+    int32_t facet_contact;       /* facets carry a surface_contact_force_functor */
+    int32_t facet_hamaker;       /* facets carry a surface_hamaker_functor */
+    uint32_t neighbor_capacity;  /* max listed neighbours per particle; 0 = derived from r_verlet */
+    uint32_t pair_capacity;      /* max bonded + touching pairs per particle; 0 = default */
+    double r_verlet;             /* neighbour-list radius (granular_system_neighbor_list.h:54) */
+    hg_contact_params contact;           /* contact functor (also the one alt_sinter embeds) */
+    hg_sinter_params sinter;             /* alt_sinter bonded model + d_crit */
+    hg_hamaker_params hamaker;           /* binary hamaker functor */
+    hg_contact_params surface_contact;   /* facet surface contact */
+    hg_hamaker_params surface_hamaker;   /* facet surface hamaker */
+} hg_params;
+
+typedef struct hg_system hg_system;
+
+/* Thread-local message of the last failing call on this thread. */
+const char *hg_last_error(void);
+
+/* Creates a system on the current HIP device and uploads the initial state.
+ * Replaces: the granular_system / granular_system_neighbor_list ctors
+ * (granular_system.h:53-63, granular_system_neighbor_list.h:54-64) together
+ * with the alt_sinter_functor ctor's bond search (alt_sinter_bridge.h:19-90),
+ * which runs here on x0 when binary_model == HG_BINARY_SINTER. In
+ * neighbour-list mode the list is built once here. theta0/omega0/v0 may be
+ * NULL (zeros). */
+hg_status hg_create(const hg_params *params, size_t n, const double *x0, const double *v0,
+                    const double *theta0, const double *omega0, hg_system **out);
+
+/* Frees all device and host memory of the system. */
+hg_status hg_destroy(hg_system *sys);
+
+/* Installs nf triangular facets (unary container of triangular_facet,
+ * triangular_facet.h:18-30): verts = nf*9 doubles (3 vertices each),
+ * v_facet = nf*3 doubles. Replaces earlier facets and clears their springs. */
+hg_status hg_set_facets(hg_system *sys, size_t nf, const double *verts, const double *v_facet);
+/* Reads the facets' current vertices (nf*9) and velocities (nf*3). */
+hg_status hg_get_facets(hg_system *sys, double *verts, double *v_facet);
+
+/* Rebuilds the neighbour list from the current positions.
+ * Replaces: update_neighbor_list() of rotational_binary_system_neighbors_omp,
+ * called by the driver (test/hamaker_neighbor.cpp:121-122). */
+hg_status hg_rebuild_neighbors(hg_system *sys);
+
+/* One acceleration pass at the current state: every pair through the functor
+ * fold, then the facets. Mutates spring state exactly as the reference's pass
+ * does. Replaces: the acceleration pass of rotational_binary_system_omp
+ * calling granular_system::compute_accelerations (granular_system.h:65-81). */
+hg_status hg_compute_accelerations(hg_system *sys);
+
+/* Advances nsteps velocity-Verlet steps of size dt. When rebuild_every > 0
+ * and the system is in neighbour-list mode, the list is rebuilt before every
+ * step whose global index is a multiple of rebuild_every. Facets advance by
+ * v_facet*dt after each step (triangular_facet.h:62-66).
+ * Replaces: do_step(dt) of the libtimestep system with
+ * rotational_velocity_verlet_half + rotational_step_handler
+ * (test/contact.cpp:98-106). */
+hg_status hg_step(hg_system *sys, double dt, uint64_t nsteps, uint32_t rebuild_every);
+
+/* Copies the state out (any pointer may be NULL).
+ * Replaces: get_x/get_v/get_theta/get_omega (Granular-system.md:65-226). */
+hg_status hg_get_state(hg_system *sys, double *x, double *v, double *theta, double *omega);
+/* Overwrites the state (any pointer may be NULL); marks accelerations stale. */
+hg_status hg_set_state(hg_system *sys, const double *x, const double *v, const double *theta,
+                       const double *omega);
+/* Last computed accelerations a (linear) and alpha (angular). */
+hg_status hg_get_accelerations(hg_system *sys, double *a, double *alpha);
+/* Simulation time and number of steps taken. */
+hg_status hg_get_time(hg_system *sys, double *t, uint64_t *step_index);
+
+/* Neighbour list as CSR: offsets[n+1]; idx written when cap suffices;
+ * *total = number of entries. Each row is ascending. */
+hg_status hg_get_neighbors(hg_system *sys, uint64_t *offsets, uint32_t *idx, size_t cap, size_t *total);
+
+/* Bonds (i<j, in the lexicographic order the reference inserts them) and the
+ * number of cycle-closing candidates skipped. Mirrors alt_sinter_functor's
+ * public bonded_contacts table (alt_sinter_bridge.h:163). */
+hg_status hg_get_bonds(hg_system *sys, uint32_t *pairs, size_t cap, size_t *nbonds, size_t *nskipped);
+
+/* Active spring entries, ordered by (i, j): which = 0 contact springs
+ * (contact_force.h:53-54,156), 1 bonded springs (alt_sinter_bridge.h:158-161).
+ * ij = 2 per entry, springs = 9 per entry (tangential, rolling, torsion). */
+hg_status hg_get_springs(hg_system *sys, int which, uint32_t *ij, double *springs, size_t cap,
+                         size_t *total);
+
+/* Device reductions of the reference diagnostics (test/compute_energy.cpp:7-58,
+ * test/mass_distribution.cpp:7-27). out[9] = KE_total, KE_trans, KE_rot, |P|,
+ * |L|, Rg, centre of mass (3). */
+hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *out);
+
+/* The alt_sinter bond search on its own (host only, no device needed): the
+ * same bonds and skip count hg_create derives for HG_BINARY_SINTER.
+ * Replaces: the topology part of the alt_sinter_functor ctor
+ * (alt_sinter_bridge.h:56-90). pairs = 2*cap uint32. */
+hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs, size_t cap,
+                             size_t *nbonds, size_t *nskipped);
+
+/* Seeded tunable-dimension cluster-cluster aggregate of n touching spheres of
+ * radius r_part (fractal dimension df, prefactor kf). x_out = 3*n doubles.
+ * Not in the reference: the input generator of the aggregate workloads. */
+hg_status hg_generate_aggregate(size_t n, double r_part, double df, double kf, uint64_t seed, double *x_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPGRAN_H */

@@ -1,0 +1,144 @@
+"""libgran_amd: the libgran DEM per-timestep path (bonded/sintered contact,
+Luding friction with persistent springs, Hamaker attraction, neighbour-list
+rebuild, rotational velocity Verlet) on AMD MI355X (gfx950).
+
+The product is the C ABI in include/hipgran.h, implemented by hand-written HIP
+kernels in libgran_amd/csrc and built into libgran_amd/libhipgran.so. The C++
+drop-in headers (libgran_amd/include/libgran/...) and this Python module are
+thin hosts over that ABI. Importing this module loads the library and fails
+loudly when it is missing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_NONE,
+                   PAIRS_ALL, PAIRS_NEIGHBOR_LIST, PRECISION_FP64, PRECISION_MIXED, HipgranError, Params)
+
+_abi.lib()  # fail at import time when the HIP library is absent
+
+__all__ = ["System", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
+           "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
+           "PAIRS_ALL", "PAIRS_NEIGHBOR_LIST", "PRECISION_FP64", "PRECISION_MIXED"]
+
+
+def make_params(binary_model, hamaker_model, pair_mode=PAIRS_ALL, r_verlet=0.0, contact=None, sinter=None,
+                hamaker=None, facet_contact=0, facet_hamaker=0, surface_contact=None, surface_hamaker=None,
+                precision=PRECISION_FP64, neighbor_capacity=0, pair_capacity=0):
+    """Builds hg_params; functor parameter sequences are in the reference ctor
+    order (contact_force.h:10-30, alt_sinter_bridge.h:19-36 + d_crit,
+    limited_range_hamaker_force.h:20-27 as A, h0, r_cutoff, r_part, mass)."""
+    p = Params()
+    p.binary_model, p.hamaker_model, p.pair_mode, p.precision = binary_model, hamaker_model, pair_mode, precision
+    p.facet_contact, p.facet_hamaker = facet_contact, facet_hamaker
+    p.neighbor_capacity, p.pair_capacity, p.r_verlet = neighbor_capacity, pair_capacity, r_verlet
+    for name, vals in (("contact", contact), ("sinter", sinter), ("hamaker", hamaker),
+                       ("surface_contact", surface_contact), ("surface_hamaker", surface_hamaker)):
+        if vals is not None:
+            _abi.fill(getattr(p, name), vals)
+    return p
+
+
+def generate_aggregate(n, r_part, df, kf, seed):
+    """Seeded TD-CCA aggregate, (n, 3) positions in metres (host only)."""
+    x = np.zeros((n, 3))
+    _abi.check(_abi.lib().hg_generate_aggregate(n, r_part, df, kf, seed, _abi.dptr(x)))
+    return x
+
+
+def sinter_topology(x0, r_part, d_crit):
+    """Bonds (i<j, reference insertion order) and skipped-cycle count (host only)."""
+    x0 = _abi.c64(x0)
+    n = x0.shape[0]
+    nb, sk = C.c_size_t(0), C.c_size_t(0)
+    _abi.check(_abi.lib().hg_sinter_topology(n, _abi.dptr(x0), r_part, d_crit, None, 0, C.byref(nb), C.byref(sk)))
+    pairs = np.zeros((max(nb.value, 1), 2), dtype=np.uint32)
+    _abi.check(_abi.lib().hg_sinter_topology(n, _abi.dptr(x0), r_part, d_crit, _abi.u32ptr(pairs), nb.value,
+                                             C.byref(nb), C.byref(sk)))
+    return pairs[:nb.value], sk.value
+
+
+class System:
+    """One DEM system on the current GPU (hg_system)."""
+
+    def __init__(self, params, x0, v0=None, theta0=None, omega0=None):
+        L = _abi.lib()
+        x0 = _abi.c64(x0)
+        self.n = x0.shape[0]
+        self.params = params
+        self._h = C.c_void_p()
+        _abi.check(L.hg_create(C.byref(params), self.n, _abi.dptr(x0), _abi.dptr(_abi.c64(v0)),
+                               _abi.dptr(_abi.c64(theta0)), _abi.dptr(_abi.c64(omega0)), C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _abi.lib().hg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def set_facets(self, verts, v_facet):
+        verts, v_facet = _abi.c64(verts), _abi.c64(v_facet)
+        _abi.check(_abi.lib().hg_set_facets(self._h, verts.shape[0], _abi.dptr(verts), _abi.dptr(v_facet)))
+
+    def facets(self, nf):
+        verts, vel = np.zeros((nf, 3, 3)), np.zeros((nf, 3))
+        _abi.check(_abi.lib().hg_get_facets(self._h, _abi.dptr(verts), _abi.dptr(vel)))
+        return verts, vel
+
+    def rebuild_neighbors(self):
+        _abi.check(_abi.lib().hg_rebuild_neighbors(self._h))
+
+    def compute_accelerations(self):
+        _abi.check(_abi.lib().hg_compute_accelerations(self._h))
+
+    def step(self, dt, nsteps=1, rebuild_every=0):
+        _abi.check(_abi.lib().hg_step(self._h, dt, nsteps, rebuild_every))
+
+    def state(self):
+        arrs = [np.zeros((self.n, 3)) for _ in range(4)]
+        _abi.check(_abi.lib().hg_get_state(self._h, *[_abi.dptr(a) for a in arrs]))
+        out = dict(zip(("x", "v", "theta", "omega"), arrs))
+        a, al = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        _abi.check(_abi.lib().hg_get_accelerations(self._h, _abi.dptr(a), _abi.dptr(al)))
+        out["a"], out["alpha"] = a, al
+        return out
+
+    def set_state(self, x=None, v=None, theta=None, omega=None):
+        arrs = [_abi.c64(q) for q in (x, v, theta, omega)]
+        _abi.check(_abi.lib().hg_set_state(self._h, *[_abi.dptr(q) for q in arrs]))
+
+    def time(self):
+        t, k = C.c_double(0), C.c_uint64(0)
+        _abi.check(_abi.lib().hg_get_time(self._h, C.byref(t), C.byref(k)))
+        return t.value, k.value
+
+    def neighbors(self):
+        off = np.zeros(self.n + 1, dtype=np.uint64)
+        tot = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_get_neighbors(self._h, off.ctypes.data_as(_abi._u64p), None, 0, C.byref(tot)))
+        idx = np.zeros(max(tot.value, 1), dtype=np.uint32)
+        _abi.check(_abi.lib().hg_get_neighbors(self._h, off.ctypes.data_as(_abi._u64p), _abi.u32ptr(idx), tot.value,
+                                               C.byref(tot)))
+        return off, idx[:tot.value]
+
+    def bonds(self):
+        nb, sk = C.c_size_t(0), C.c_size_t(0)
+        _abi.check(_abi.lib().hg_get_bonds(self._h, None, 0, C.byref(nb), C.byref(sk)))
+        pairs = np.zeros((max(nb.value, 1), 2), dtype=np.uint32)
+        _abi.check(_abi.lib().hg_get_bonds(self._h, _abi.u32ptr(pairs), nb.value, C.byref(nb), C.byref(sk)))
+        return pairs[:nb.value], sk.value
+
+    def springs(self, which):
+        tot = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_get_springs(self._h, which, None, None, 0, C.byref(tot)))
+        ij = np.zeros((max(tot.value, 1), 2), dtype=np.uint32)
+        s = np.zeros((max(tot.value, 1), 9))
+        _abi.check(_abi.lib().hg_get_springs(self._h, which, _abi.u32ptr(ij), _abi.dptr(s), tot.value, C.byref(tot)))
+        return ij[:tot.value], s[:tot.value]
+
+    def diagnostics(self, mass, inertia):
+        out = np.zeros(9)
+        _abi.check(_abi.lib().hg_diagnostics(self._h, mass, inertia, _abi.dptr(out)))
+        return dict(ke=out[0], ke_trans=out[1], ke_rot=out[2], P=out[3], L=out[4], rg=out[5], cm=out[6:9].copy())

@@ -1,0 +1,248 @@
+// Vector algebra and pair/facet force models of the DEM step, for gfx950
+// device code and the host-side set-up code.
+//
+// The FP64 functions follow the reference functors' expression order
+// term by term (cited per function) so that, compiled with
+// -ffp-contract=off, they reproduce the reference's double-precision results
+// bit for bit. The vector type carries Eigen::Vector3d's semantics as the
+// functors use them: dot = (x*x' + y*y') + z*z', norm = sqrt(dot),
+// normalized() of a zero vector returns it unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hipgran.h"
+
+#define HGD __host__ __device__ __forceinline__
+
+namespace hg {
+
+struct d3 {
+    double x, y, z;
+};
+
+HGD d3 mk(double x, double y, double z) { return d3{x, y, z}; }
+HGD d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+HGD d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+HGD d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+HGD d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+HGD d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+HGD d3 operator/(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+HGD double dot(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+HGD double norm(d3 a) { return sqrt(dot(a, a)); }
+HGD d3 normalized(d3 a) {
+    double z = dot(a, a);
+    if (z > 0.0) return a / sqrt(z);
+    return a;
+}
+HGD d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+HGD d3 zero3() { return {0.0, 0.0, 0.0}; }
+// std::min(a, b) semantics: returns a unless b < a.
+HGD double min_ref(double a, double b) { return (b < a) ? b : a; }
+
+struct acc6 {
+    d3 lin, ang;
+};
+HGD acc6 operator+(acc6 a, acc6 b) { return {a.lin + b.lin, a.ang + b.ang}; }
+
+// Tangential, rolling and torsion spring of one ordered pair.
+struct spring9 {
+    d3 s[3];
+};
+HGD spring9 zero_spring() { return spring9{{zero3(), zero3(), zero3()}}; }
+
+template <typename T> HGD T sq(T x) { return x * x; }
+template <typename T> HGD T cb(T x) { return x * x * x; }
+
+// ---- contact_force.h:105-147 (compute_shear_contribution) ------------------
+HGD d3 shear_contact(d3 &xi, d3 n, double stiffness, double damping, double normal_force, double mu_static,
+                     double phi_dynamic, d3 rel_v, double dt) {
+    double mu_dynamic = mu_static * phi_dynamic;
+    d3 xi_new = xi - dot(xi, n) * n;
+    if (norm(xi_new) > 0.0) {
+        xi_new = xi_new / sqrt(dot(xi_new, xi_new));
+        xi_new = xi_new * norm(xi);
+    }
+    xi = xi_new;
+    d3 f0 = (-stiffness) * xi - damping * rel_v;
+    double static_friction = mu_static * normal_force;
+    double dynamic_friction = mu_dynamic * normal_force;
+    double f0n = norm(f0);
+    d3 t = f0n > 0.0 ? normalized(f0) : zero3();
+    double f_selected;
+    if (f0n <= static_friction) {
+        xi = xi + rel_v * dt;
+        f_selected = static_friction;
+    } else {
+        xi = (-1.0 / stiffness) * (dynamic_friction * t + damping * rel_v);
+        f_selected = dynamic_friction;
+    }
+    return min_ref(f_selected, f0n) * t;
+}
+
+// ---- contact_force.h:57-96 --------------------------------------------------
+// `active` is false where the reference resets the pair's springs
+// (overlap <= 0, contact_force.h:68-71); the spring is then zero.
+HGD acc6 contact_eval(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp,
+                      bool &active) {
+    d3 d = xi - xj;
+    d3 n = normalized(d);
+    double overlap = 2.0 * p.r_part - dot(d, n);
+    if (overlap <= 0) {
+        active = false;
+        sp = zero_spring();
+        return {zero3(), zero3()};
+    }
+    active = true;
+    double rpp = p.r_part - overlap / 2.0;
+    double v_n = -dot(vi - vj, n);
+    double f_n = p.k * overlap + p.gamma_n * v_n;
+    d3 cwi = cross(n, wi), cwj = cross(n, wj);
+    d3 v_ij = ((vi - vj) + rpp * cwi) + rpp * cwj;
+    d3 v_t = v_ij - dot(v_ij, n) * n;
+    d3 v_r = (-rpp) * (cwi - cwj);
+    d3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    d3 f_t = shear_contact(sp.s[0], n, p.k_t, p.gamma_t, f_n, p.mu_s, p.phi_d, v_t, p.dt);
+    d3 f_r = shear_contact(sp.s[1], n, p.k_r, p.gamma_r, f_n, p.mu_r, p.phi_r, v_r, p.dt);
+    d3 f_o = shear_contact(sp.s[2], n, p.k_o, p.gamma_o, f_n, p.mu_o, p.phi_o, v_o, p.dt);
+    d3 tau_t = rpp * cross(n, f_t);
+    d3 tau_r = p.r_part * cross(n, f_r);
+    d3 tau_o = p.r_part * f_o;
+    return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+}
+
+// ---- alt_sinter_bridge.h:137-156 -------------------------------------------
+HGD d3 shear_sinter(d3 &xi, d3 n, double stiffness, double damping, d3 rel_v, double dt) {
+    d3 xi_new = xi - dot(xi, n) * n;
+    if (norm(xi_new) > 0.0) {
+        xi_new = xi_new / sqrt(dot(xi_new, xi_new));
+        xi_new = xi_new * norm(xi);
+    }
+    xi = xi_new;
+    d3 f0 = (-stiffness) * xi - damping * rel_v;
+    xi = xi + rel_v * dt;
+    return f0;
+}
+
+// ---- alt_sinter_bridge.h:92-129 (bonded branch; no overlap gate) ----------
+HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp) {
+    d3 d = xi - xj;
+    d3 n = normalized(d);
+    double overlap = 2.0 * p.r_part - dot(d, n);
+    double rpp = p.r_part - overlap / 2.0;
+    double v_n = -dot(vi - vj, n);
+    double f_n = p.k * overlap + p.gamma_n * v_n;
+    d3 cwi = cross(n, wi), cwj = cross(n, wj);
+    d3 v_ij = ((vi - vj) + rpp * cwi) + rpp * cwj;
+    d3 v_t = v_ij - dot(v_ij, n) * n;
+    d3 v_r = (-rpp) * (cwi - cwj);
+    d3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    d3 f_t = shear_sinter(sp.s[0], n, p.k_t, p.gamma_t, v_t, p.dt);
+    d3 f_r = shear_sinter(sp.s[1], n, p.k_r, p.gamma_r, v_r, p.dt);
+    d3 f_o = shear_sinter(sp.s[2], n, p.k_o, p.gamma_o, v_o, p.dt);
+    d3 tau_t = rpp * cross(n, f_t);
+    d3 tau_r = p.r_part * cross(n, f_r);
+    d3 tau_o = p.r_part * f_o;
+    return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+}
+
+// ---- hamaker_force.h:36-39: the scalar factor of -dU/dh --------------------
+HGD double hamaker_scalar(double A, double rp, double h) {
+    return (-A) / 6.0 *
+           ((((4.0 * rp + 2.0 * h) / (4.0 * rp + h) / h - 2.0 / (2.0 * rp + h)) - 4.0 * sq(rp) / cb(2.0 * rp + h)) -
+            2.0 * sq(rp) * (4.0 * rp + 2.0 * h) / sq(4.0 * rp + h) / sq(h));
+}
+
+// ---- hamaker_force.h:28-43 -------------------------------------------------
+HGD acc6 hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
+    d3 r = xj - xi;
+    double h = norm(r) - 2.0 * p.r_part;
+    if (h < p.h0) h = p.h0;
+    d3 f = hamaker_scalar(p.A, p.r_part, h) * normalized(r);
+    return {f / p.mass, zero3()};
+}
+
+// ---- limited_range_hamaker_force.h:29-47 (strict '>' at the cutoff) -------
+HGD acc6 lr_hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
+    if (norm(xi - xj) > p.r_cutoff) return {zero3(), zero3()};
+    return hamaker_eval(p, xi, xj);
+}
+
+// ---- triangular facet (triangular_facet.h:18-137) --------------------------
+struct facet_dev {
+    d3 vert[3];
+    d3 v_facet;
+    d3 normal;
+    double area;
+    double pad[2];
+};
+
+HGD d3 facet_edge_point(const facet_dev &f, int a, int b, d3 p) {
+    d3 lv = f.vert[b] - f.vert[a];
+    d3 ulv = normalized(lv);
+    d3 tpv = p - f.vert[a];
+    double len = norm(lv);
+    double pr = dot(tpv, ulv);
+    if (pr < 0.0) return f.vert[a];
+    if (pr > len) return f.vert[b];
+    return f.vert[a] + pr * ulv;
+}
+
+HGD d3 closest_point_on_triangle(const facet_dev &f, d3 p) {
+    d3 offset = dot(f.vert[0], f.normal) * f.normal;
+    d3 proj = p - dot(p, f.normal) * f.normal;
+    d3 cp = proj + offset;
+    double area = 0.0;
+    d3 pa = f.vert[0] - cp, pb = f.vert[1] - cp, pc = f.vert[2] - cp;
+    area += 1.0 / 2.0 * norm(cross(pa, pb));
+    area += 1.0 / 2.0 * norm(cross(pb, pc));
+    area += 1.0 / 2.0 * norm(cross(pc, pa));
+    if (!(area > f.area)) return cp;
+    d3 e1 = facet_edge_point(f, 0, 1, p), e2 = facet_edge_point(f, 1, 2, p), e3 = facet_edge_point(f, 2, 0, p);
+    d3 q1 = p - e1, q2 = p - e2, q3 = p - e3;
+    double s1 = dot(q1, q1), s2 = dot(q2, q2), s3 = dot(q3, q3);
+    if (s1 < s2) {
+        if (s1 < s3) return e1;
+    } else {
+        if (s2 < s3) return e2;
+    }
+    return e3;
+}
+
+// ---- surface_contact_force.h:57-96 ------------------------------------------
+HGD acc6 surface_contact_eval(const hg_contact_params &p, d3 xf, d3 vf, d3 xi, d3 vi, d3 wi, spring9 &sp) {
+    d3 d = xi - xf;
+    d3 n = normalized(d);
+    double overlap = p.r_part - dot(d, n);
+    if (overlap <= 0) {
+        sp = zero_spring();
+        return {zero3(), zero3()};
+    }
+    double rpp = p.r_part - overlap;
+    double v_n = -dot(vi - vf, n);
+    double f_n = p.k * overlap + p.gamma_n * v_n;
+    d3 cw = cross(n, wi);
+    d3 v_ij = (vi - vf) + rpp * cw;
+    d3 v_t = v_ij - dot(v_ij, n) * n;
+    d3 v_r = (-rpp) * cw;
+    d3 v_o = (p.r_part * dot(n, wi)) * n;
+    d3 f_t = shear_contact(sp.s[0], n, p.k_t, p.gamma_t, f_n, p.mu_s, p.phi_d, v_t, p.dt);
+    d3 f_r = shear_contact(sp.s[1], n, p.k_r, p.gamma_r, f_n, p.mu_r, p.phi_r, v_r, p.dt);
+    d3 f_o = shear_contact(sp.s[2], n, p.k_o, p.gamma_o, f_n, p.mu_o, p.phi_o, v_o, p.dt);
+    d3 tau_t = rpp * cross(n, f_t);
+    d3 tau_r = p.r_part * cross(n, f_r);
+    d3 tau_o = p.r_part * f_o;
+    return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+}
+
+// ---- surface_hamaker_force.h:18-30 ------------------------------------------
+HGD acc6 surface_hamaker_eval(const hg_hamaker_params &p, d3 xf, d3 xi) {
+    d3 r = xf - xi;
+    double h = norm(r) - p.r_part;
+    if (h < p.h0) h = p.h0;
+    d3 f = 2.0 * p.A / 3.0 * (cb(p.r_part) / sq(h) / sq(h + 2.0 * p.r_part)) * normalized(r);
+    return {f / p.mass, zero3()};
+}
+
+}  // namespace hg

@@ -1,0 +1,119 @@
+// Internal declarations of the hipgran library: the device-side system
+// object, kernel launchers and error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hipgran.h"
+#include "dem_math.h"
+
+namespace hg {
+
+// Pair-state table entry tag: bit 31 marks a bonded (alt_sinter) pair; the low
+// 31 bits are the partner index j.
+constexpr uint32_t kBondBit = 0x80000000u;
+constexpr uint32_t kIdxMask = 0x7fffffffu;
+constexpr uint32_t kNoEntry = 0xffffffffu;  // (kNoEntry & kIdxMask) exceeds every valid j
+
+// Device flag words (int32 each) written by kernels, read back by the host.
+enum : int { FLAG_NL_OVERFLOW = 0, FLAG_PAIR_OVERFLOW = 1, FLAG_NL_MAXCOUNT = 2, FLAG_PAIR_MAXCOUNT = 3, FLAG_COUNT = 4 };
+
+constexpr int kMaxFacets = 16;
+
+// Everything a force/integrate kernel needs, passed by value.
+struct sys_view {
+    uint32_t n;
+    const double *x, *v, *w;  // AoS-by-field: 4 doubles per particle (w unused lane)
+    double *a, *al;
+    // neighbour list (transposed: slot k of particle i at k*n + i)
+    const uint32_t *nb;
+    const uint32_t *ncnt;
+    // pair-state table, old (read) and new (written) halves
+    const uint32_t *pj_old;
+    const double *ps_old;
+    const uint32_t *pcnt_old;
+    uint32_t *pj_new;
+    double *ps_new;
+    uint32_t *pcnt_new;
+    uint32_t cap_s;
+    // facets
+    const facet_dev *facets;
+    double *ss;  // surface springs: (f*9 + c)*n + i
+    int nf;
+    int *flags;
+};
+
+struct hg_system_impl {
+    hg_params p{};
+    size_t n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // particle state, 4 doubles per particle per field
+    double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
+    bool acc_valid = false;
+    double t = 0.0;
+    uint64_t step_index = 0;
+    // pair-state table (double buffered)
+    uint32_t cap_s = 0;
+    uint32_t *pj[2] = {nullptr, nullptr};
+    double *ps[2] = {nullptr, nullptr};
+    uint32_t *pcnt[2] = {nullptr, nullptr};
+    int cur = 0;
+    // neighbour list
+    uint32_t cap_n = 0;
+    uint32_t *nb = nullptr, *ncnt = nullptr;
+    // cell-list scratch
+    uint32_t hash_size = 0;
+    uint32_t *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr, *bucket_fill = nullptr,
+             *cell_items = nullptr;
+    double *bbox = nullptr;  // lo[3], hi[3]
+    // facets
+    int nf = 0;
+    facet_dev *facets = nullptr;
+    double *ss = nullptr;
+    // flags
+    int *flags = nullptr;
+    int *flags_host = nullptr;  // pinned
+    // reductions
+    double *red = nullptr;
+    // host copies of the topology
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t bonds_skipped = 0;
+    std::vector<void *> allocations;
+};
+
+// error helpers (thread-local message)
+void set_error(const std::string &msg);
+hg_status fail(hg_status st, const std::string &msg);
+
+#define HG_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::hg::fail(HG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// kernel launchers (kernels.hip)
+hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s);
+hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const double *a, const double *al,
+                             uint32_t n, bool pending, double half_prev, double half, double dt, facet_dev *facets,
+                             int nf, double facet_dt, hipStream_t s);
+hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
+                             facet_dev *facets, int nf, double facet_dt, hipStream_t s);
+hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t *p, size_t count, hipStream_t s);
+hipError_t launch_zero_f64(double *p, size_t count, hipStream_t s);
+hipError_t launch_diagnostics(const double *x, const double *v, const double *w, uint32_t n, double mass,
+                              double inertia, double *out, hipStream_t s);
+
+// host-side set-up (topology.cpp)
+void sinter_topology_host(size_t n, const double *x0, double r_part, double d_crit,
+                          std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped);
+
+}  // namespace hg
+
+struct hg_system : hg::hg_system_impl {};

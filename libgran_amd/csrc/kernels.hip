@@ -1,0 +1,559 @@
+// gfx950 kernels of the FP64 DEM step: acceleration pass (pair fold + facets),
+// velocity-Verlet kick/drift, cell-list neighbour rebuild and diagnostics.
+//
+// Data layout in HBM
+//   particle fields x, v, theta, omega, a, alpha: one array per field, 4
+//     doubles (32 B) per particle, so a particle's vector is one aligned
+//     32-byte load and a wavefront's loads are contiguous.
+//   neighbour list: transposed fixed-capacity slots, nb[k*n + i], ascending j
+//     per particle; consecutive lanes (particles) read consecutive words.
+//   pair-state table: per particle a j-sorted list of (tag, spring9) where
+//     tag = j | bonded-bit, transposed as pj[s*n + i] and ps[(s*9+c)*n + i].
+//     It holds every bonded pair and every pair whose contact spring the
+//     reference would hold non-zero. Each acceleration pass reads the old
+//     table and writes the new one while walking the partners in ascending j,
+//     so spring state survives neighbour rebuilds with no separate merge.
+#include <hip/hip_runtime.h>
+
+#include "hg_internal.h"
+
+namespace hg {
+
+namespace {
+
+__device__ __forceinline__ d3 ld3(const double *base, uint32_t i) {
+    const double4 q = reinterpret_cast<const double4 *>(base)[i];
+    return d3{q.x, q.y, q.z};
+}
+__device__ __forceinline__ void st3(double *base, uint32_t i, d3 a) {
+    reinterpret_cast<double4 *>(base)[i] = double4{a.x, a.y, a.z, 0.0};
+}
+
+// Walks one particle's old pair-state table in ascending j alongside the
+// partner loop, writing the new table. Entries not visited this pass (pairs
+// that are not listed) are copied unchanged: the reference leaves the dense
+// spring table untouched for pairs it does not evaluate.
+struct walker {
+    const uint32_t *pj_old;
+    const double *ps_old;
+    uint32_t *pj_new;
+    double *ps_new;
+    size_t n;
+    uint32_t i, cnt_old, p, q, cap, ej;
+
+    __device__ void init(const sys_view &S, uint32_t i_) {
+        pj_old = S.pj_old;
+        ps_old = S.ps_old;
+        pj_new = S.pj_new;
+        ps_new = S.ps_new;
+        n = S.n;
+        i = i_;
+        cap = S.cap_s;
+        cnt_old = S.pcnt_old[i];
+        p = 0;
+        q = 0;
+        ej = cnt_old > 0 ? pj_old[i] : kNoEntry;
+    }
+    __device__ void next() {
+        p++;
+        ej = p < cnt_old ? pj_old[size_t(p) * n + i] : kNoEntry;
+    }
+    __device__ spring9 load() const {
+        spring9 s;
+        const double *b = ps_old + size_t(p) * 9 * n + i;
+        s.s[0] = d3{b[0], b[n], b[2 * n]};
+        s.s[1] = d3{b[3 * n], b[4 * n], b[5 * n]};
+        s.s[2] = d3{b[6 * n], b[7 * n], b[8 * n]};
+        return s;
+    }
+    __device__ void store(uint32_t tag, const spring9 &s) {
+        if (q < cap) {
+            pj_new[size_t(q) * n + i] = tag;
+            double *b = ps_new + size_t(q) * 9 * n + i;
+            b[0] = s.s[0].x; b[n] = s.s[0].y; b[2 * n] = s.s[0].z;
+            b[3 * n] = s.s[1].x; b[4 * n] = s.s[1].y; b[5 * n] = s.s[1].z;
+            b[6 * n] = s.s[2].x; b[7 * n] = s.s[2].y; b[8 * n] = s.s[2].z;
+        }
+        q++;
+    }
+    __device__ void advance_to(uint32_t j) {
+        while ((ej & kIdxMask) < j) {
+            store(ej, load());
+            next();
+        }
+    }
+    __device__ bool take(uint32_t j, spring9 &s, bool &bonded) {
+        if ((ej & kIdxMask) == j) {
+            s = load();
+            bonded = (ej & kBondBit) != 0;
+            next();
+            return true;
+        }
+        return false;
+    }
+    __device__ void finish() {
+        while (p < cnt_old) {
+            store(ej, load());
+            next();
+        }
+    }
+};
+
+// One ordered pair through the binary container fold
+// (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
+template <int BIN, int HAM>
+__device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
+                                          walker &W, acc6 &acc) {
+    acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
+    if constexpr (BIN != HG_BINARY_NONE) {
+        W.advance_to(j);
+        spring9 sp = zero_spring();
+        bool bonded = false;
+        W.take(j, sp, bonded);
+        bool keep;
+        if (BIN == HG_BINARY_SINTER && bonded) {
+            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp);
+            keep = true;
+        } else {
+            b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
+        }
+        if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
+    }
+    if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
+    if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
+    acc6 e;
+    if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) e = b + h;
+    else if constexpr (BIN != HG_BINARY_NONE) e = b;
+    else e = h;
+    acc.lin = acc.lin + e.lin;
+    acc.ang = acc.ang + e.ang;
+}
+
+// Unary container over facets (unary_force_container.h:20-32, a right fold),
+// each facet folding its surface functors (triangular_facet.h:32-51).
+__device__ __forceinline__ acc6 facet_fold(const hg_params &P, const sys_view &S, uint32_t i, d3 xi, d3 vi, d3 wi) {
+    acc6 s{zero3(), zero3()};
+    for (int f = S.nf - 1; f >= 0; f--) {
+        const facet_dev F = S.facets[f];
+        d3 xf = closest_point_on_triangle(F, xi);
+        acc6 c{zero3(), zero3()}, h{zero3(), zero3()};
+        if (P.facet_contact) {
+            double *b = S.ss + size_t(f) * 9 * S.n + i;
+            const size_t n = S.n;
+            spring9 sp{{d3{b[0], b[n], b[2 * n]}, d3{b[3 * n], b[4 * n], b[5 * n]}, d3{b[6 * n], b[7 * n], b[8 * n]}}};
+            c = surface_contact_eval(P.surface_contact, xf, F.v_facet, xi, vi, wi, sp);
+            b[0] = sp.s[0].x; b[n] = sp.s[0].y; b[2 * n] = sp.s[0].z;
+            b[3 * n] = sp.s[1].x; b[4 * n] = sp.s[1].y; b[5 * n] = sp.s[1].z;
+            b[6 * n] = sp.s[2].x; b[7 * n] = sp.s[2].y; b[8 * n] = sp.s[2].z;
+        }
+        if (P.facet_hamaker) h = surface_hamaker_eval(P.surface_hamaker, xf, xi);
+        acc6 e = (P.facet_contact && P.facet_hamaker) ? c + h : (P.facet_contact ? c : h);
+        s = (f == S.nf - 1) ? e : e + s;
+    }
+    return s;
+}
+
+template <int BIN>
+__device__ __forceinline__ void close_table(const sys_view &S, uint32_t i, walker &W) {
+    if constexpr (BIN != HG_BINARY_NONE) {
+        W.finish();
+        S.pcnt_new[i] = W.q < W.cap ? W.q : W.cap;
+        if (W.q > W.cap) atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
+        atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(W.q));
+    }
+}
+
+// Neighbour-list acceleration pass: one lane per particle i, partners in the
+// list's ascending order, so the per-particle sum has the reference's order.
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
+    acc6 acc{zero3(), zero3()};
+    walker W;
+    if constexpr (BIN != HG_BINARY_NONE) W.init(S, i);
+    const uint32_t nn = S.ncnt[i];
+    const uint32_t *nbi = S.nb + i;
+    for (uint32_t k = 0; k < nn; k++) {
+        const uint32_t j = nbi[size_t(k) * S.n];
+        pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(S.x, j), ld3(S.v, j), ld3(S.w, j), W, acc);
+    }
+    close_table<BIN>(S, i, W);
+    if (S.nf > 0) acc = acc + facet_fold(P, S, i, xi, vi, wi);
+    st3(S.a, i, acc.lin);
+    st3(S.al, i, acc.ang);
+}
+
+// All-pairs acceleration pass (granular_system): every j != i in ascending
+// order. Partner state is staged through LDS one tile per block-wide step;
+// all lanes read the same LDS word, a broadcast with no bank conflict.
+constexpr int kTile = 256;
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
+    __shared__ double sx[3][kTile], sv[3][kTile], sw[3][kTile];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < S.n;
+    d3 xi = zero3(), vi = zero3(), wi = zero3();
+    if (live) {
+        xi = ld3(S.x, i);
+        vi = ld3(S.v, i);
+        wi = ld3(S.w, i);
+    }
+    acc6 acc{zero3(), zero3()};
+    walker W;
+    if constexpr (BIN != HG_BINARY_NONE) {
+        if (live) W.init(S, i);
+    }
+    for (uint32_t base = 0; base < S.n; base += kTile) {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < kTile && base + t < S.n; t += blockDim.x) {
+            d3 q = ld3(S.x, base + t), u = ld3(S.v, base + t), r = ld3(S.w, base + t);
+            sx[0][t] = q.x; sx[1][t] = q.y; sx[2][t] = q.z;
+            sv[0][t] = u.x; sv[1][t] = u.y; sv[2][t] = u.z;
+            sw[0][t] = r.x; sw[1][t] = r.y; sw[2][t] = r.z;
+        }
+        __syncthreads();
+        if (live) {
+            const uint32_t m = (S.n - base) < uint32_t(kTile) ? (S.n - base) : uint32_t(kTile);
+            for (uint32_t jj = 0; jj < m; jj++) {
+                const uint32_t j = base + jj;
+                if (j == i) continue;
+                pair_fold<BIN, HAM>(P, j, xi, vi, wi, d3{sx[0][jj], sx[1][jj], sx[2][jj]},
+                                    d3{sv[0][jj], sv[1][jj], sv[2][jj]}, d3{sw[0][jj], sw[1][jj], sw[2][jj]}, W,
+                                    acc);
+            }
+        }
+    }
+    if (!live) return;
+    close_table<BIN>(S, i, W);
+    if (S.nf > 0) acc = acc + facet_fold(P, S, i, xi, vi, wi);
+    st3(S.a, i, acc.lin);
+    st3(S.al, i, acc.ang);
+}
+
+// Kick (completing the previous step when `pending`), kick, drift:
+//   v += a*(dt/2); w += alpha*(dt/2); x += v*dt; theta += w*dt.
+// Lane 0 of block 0 advances the facets when a previous step completes here.
+__global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double *th, double *w, const double *a,
+                                                    const double *al, uint32_t n, int pending, double half_prev,
+                                                    double half, double dt, facet_dev *facets, int nf,
+                                                    double facet_dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pending && i == 0) {
+        for (int f = 0; f < nf; f++)
+            for (int k = 0; k < 3; k++) facets[f].vert[k] = facets[f].vert[k] + facets[f].v_facet * facet_dt;
+    }
+    if (i >= n) return;
+    d3 vi = ld3(v, i), wi = ld3(w, i);
+    const d3 ai = ld3(a, i), ali = ld3(al, i);
+    if (pending) {
+        vi = vi + ai * half_prev;
+        wi = wi + ali * half_prev;
+    }
+    vi = vi + ai * half;
+    wi = wi + ali * half;
+    st3(x, i, ld3(x, i) + vi * dt);
+    st3(th, i, ld3(th, i) + wi * dt);
+    st3(v, i, vi);
+    st3(w, i, wi);
+}
+
+__global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const double *a, const double *al,
+                                                    uint32_t n, double half, facet_dev *facets, int nf,
+                                                    double facet_dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        for (int f = 0; f < nf; f++)
+            for (int k = 0; k < 3; k++) facets[f].vert[k] = facets[f].vert[k] + facets[f].v_facet * facet_dt;
+    }
+    if (i >= n) return;
+    st3(v, i, ld3(v, i) + ld3(a, i) * half);
+    st3(w, i, ld3(w, i) + ld3(al, i) * half);
+}
+
+__global__ void k_zero_u32(uint32_t *p, size_t count) {
+    for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < count; k += size_t(gridDim.x) * blockDim.x)
+        p[k] = 0;
+}
+__global__ void k_zero_f64(double *p, size_t count) {
+    for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < count; k += size_t(gridDim.x) * blockDim.x)
+        p[k] = 0.0;
+}
+
+// ---------------------------------------------------- neighbour rebuild -----
+// Single-block bounding box of the positions: bbox = lo[3], hi[3].
+__global__ void __launch_bounds__(1024) k_bbox(const double *x, uint32_t n, double *bbox) {
+    __shared__ double red[6][1024];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        d3 q = ld3(x, i);
+        lo[0] = fmin(lo[0], q.x); lo[1] = fmin(lo[1], q.y); lo[2] = fmin(lo[2], q.z);
+        hi[0] = fmax(hi[0], q.x); hi[1] = fmax(hi[1], q.y); hi[2] = fmax(hi[2], q.z);
+    }
+    for (int c = 0; c < 3; c++) {
+        red[c][threadIdx.x] = lo[c];
+        red[3 + c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) {
+                red[c][threadIdx.x] = fmin(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+                red[3 + c][threadIdx.x] = fmax(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) bbox[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__device__ __forceinline__ uint32_t cell_hash(long long cx, long long cy, long long cz, uint32_t mask) {
+    uint64_t h = uint64_t(cx) * 73856093ull ^ uint64_t(cy) * 19349663ull ^ uint64_t(cz) * 83492791ull;
+    h ^= h >> 29;
+    h *= 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 32;
+    return uint32_t(h) & mask;
+}
+
+__device__ __forceinline__ void cell_coords(d3 q, const double *bbox, double cs, long long &cx, long long &cy,
+                                            long long &cz) {
+    cx = (long long)((q.x - bbox[0]) / cs);
+    cy = (long long)((q.y - bbox[1]) / cs);
+    cz = (long long)((q.z - bbox[2]) / cs);
+}
+
+__global__ void __launch_bounds__(256) k_cell_count(const double *x, uint32_t n, const double *bbox, double cs,
+                                                    uint32_t mask, uint32_t *cell_of, uint32_t *bucket_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long cx, cy, cz;
+    cell_coords(ld3(x, i), bbox, cs, cx, cy, cz);
+    const uint32_t h = cell_hash(cx, cy, cz, mask);
+    cell_of[i] = h;
+    atomicAdd(&bucket_count[h], 1u);
+}
+
+// Single-block exclusive scan of `count` words.
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t *in, uint32_t *out, uint32_t count) {
+    __shared__ uint32_t sums[1024];
+    const uint32_t per = (count + blockDim.x - 1) / blockDim.x;
+    const uint32_t b = threadIdx.x * per, e = min(count, b + per);
+    uint32_t s = 0;
+    for (uint32_t k = b; k < e; k++) s += in[k];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        uint32_t add = threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = sums[threadIdx.x] - s;
+    for (uint32_t k = b; k < e; k++) {
+        out[k] = run;
+        run += in[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t *bucket_start,
+                                                   uint32_t *bucket_fill, uint32_t *cell_items) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = cell_of[i];
+    cell_items[bucket_start[h] + atomicAdd(&bucket_fill[h], 1u)] = i;
+}
+
+// Per particle: the 27 surrounding cells' buckets (deduplicated, since two
+// cells may share a bucket), candidates filtered by the reference predicate
+// norm(x_i - x_j) < r_verlet, inserted in ascending j.
+__global__ void __launch_bounds__(256) k_nl_fill(const double *x, uint32_t n, const double *bbox, double cs,
+                                                 uint32_t mask, double r_verlet, const uint32_t *bucket_count,
+                                                 const uint32_t *bucket_start, const uint32_t *cell_items,
+                                                 uint32_t *nb, uint32_t *ncnt, uint32_t cap, int *flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const d3 xi = ld3(x, i);
+    long long cx, cy, cz;
+    cell_coords(xi, bbox, cs, cx, cy, cz);
+    uint32_t seen[27];
+    int nseen = 0;
+    uint32_t cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const uint32_t h = cell_hash(cx + dx, cy + dy, cz + dz, mask);
+                bool dup = false;
+                for (int s = 0; s < nseen; s++) dup |= seen[s] == h;
+                if (dup) continue;
+                seen[nseen++] = h;
+                const uint32_t b = bucket_start[h], e = b + bucket_count[h];
+                for (uint32_t q = b; q < e; q++) {
+                    const uint32_t j = cell_items[q];
+                    if (j == i) continue;
+                    if (norm(xi - ld3(x, j)) < r_verlet) {
+                        if (cnt < cap) {
+                            // insertion into the ascending slot list
+                            uint32_t pos = cnt;
+                            while (pos > 0) {
+                                const uint32_t prev = nb[size_t(pos - 1) * n + i];
+                                if (prev < j) break;
+                                nb[size_t(pos) * n + i] = prev;
+                                pos--;
+                            }
+                            nb[size_t(pos) * n + i] = j;
+                        }
+                        cnt++;
+                    }
+                }
+            }
+    ncnt[i] = cnt < cap ? cnt : cap;
+    if (cnt > cap) atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
+    atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+}
+
+// ---------------------------------------------------------- diagnostics -----
+// Single block: out = KE_total, KE_trans, KE_rot, |P|, |L|, Rg, cm(3).
+__global__ void __launch_bounds__(1024) k_diag(const double *x, const double *v, const double *w, uint32_t n,
+                                               double mass, double inertia, double *out) {
+    __shared__ double red[8][1024];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // 2*KE_trans, 2*KE_rot, sum v (3), L (3)
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        d3 vi = ld3(v, i), wi = ld3(w, i), xi = ld3(x, i);
+        acc[0] += mass * dot(vi, vi);
+        acc[1] += inertia * dot(wi, wi);
+        acc[2] += vi.x; acc[3] += vi.y; acc[4] += vi.z;
+        d3 L = inertia * wi + cross(xi, mass * vi);
+        acc[5] += L.x; acc[6] += L.y; acc[7] += L.z;
+    }
+    for (int c = 0; c < 8; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 8; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    d3 cmsum = zero3();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) cmsum = cmsum + ld3(x, i);
+    __syncthreads();
+    double k0 = red[0][0], k1 = red[1][0];
+    d3 P{red[2][0], red[3][0], red[4][0]}, L{red[5][0], red[6][0], red[7][0]};
+    __syncthreads();
+    red[0][threadIdx.x] = cmsum.x; red[1][threadIdx.x] = cmsum.y; red[2][threadIdx.x] = cmsum.z;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    const d3 cm = d3{red[0][0], red[1][0], red[2][0]} / double(n);
+    __syncthreads();
+    double rg = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        d3 dd = ld3(x, i) - cm;
+        rg += dot(dd, dd);
+    }
+    red[0][threadIdx.x] = rg;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[0][threadIdx.x] += red[0][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = k0 / 2.0 + k1 / 2.0;
+        out[1] = k0 / 2.0;
+        out[2] = k1 / 2.0;
+        out[3] = mass * norm(P);
+        out[4] = norm(L);
+        out[5] = sqrt(red[0][0] / double(n));
+        out[6] = cm.x;
+        out[7] = cm.y;
+        out[8] = cm.z;
+    }
+}
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
+
+template <int BIN, int HAM>
+hipError_t launch_force_t(const hg_params &p, const sys_view &sv, hipStream_t s) {
+    if (sv.n == 0) return hipSuccess;
+    if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
+        // 64-lane blocks spread a modest particle count over more CUs.
+        k_force_nl<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
+    } else {
+        k_force_all<BIN, HAM><<<blocks_for(sv.n, kTile), kTile, 0, s>>>(p, sv);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
+#define HG_CASE(B, H)                                                       \
+    if (p.binary_model == B && p.hamaker_model == H) return launch_force_t<B, H>(p, sv, s);
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+#undef HG_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const double *a, const double *al,
+                             uint32_t n, bool pending, double half_prev, double half, double dt, facet_dev *facets,
+                             int nf, double facet_dt, hipStream_t s) {
+    unsigned nb = blocks_for(n > 0 ? n : 1, 256);
+    k_kick_drift<<<nb, 256, 0, s>>>(x, v, th, w, a, al, n, pending ? 1 : 0, half_prev, half, dt, facets, nf,
+                                    facet_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
+                             facet_dev *facets, int nf, double facet_dt, hipStream_t s) {
+    unsigned nb = blocks_for(n > 0 ? n : 1, 256);
+    k_final_kick<<<nb, 256, 0, s>>>(v, w, a, al, n, half, facets, nf, facet_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned nb = unsigned(std::min<size_t>(1024, (count + 255) / 256));
+    k_zero_u32<<<nb, 256, 0, s>>>(p, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f64(double *p, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    unsigned nb = unsigned(std::min<size_t>(1024, (count + 255) / 256));
+    k_zero_f64<<<nb, 256, 0, s>>>(p, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s) {
+    const uint32_t n = uint32_t(S.n);
+    if (n == 0) return hipSuccess;
+    const double cs = S.p.r_verlet * (1.0 + 1e-9);
+    const uint32_t mask = S.hash_size - 1;
+    hipError_t e;
+    if ((e = launch_zero_u32(S.bucket_count, S.hash_size, s)) != hipSuccess) return e;
+    if ((e = launch_zero_u32(S.bucket_fill, S.hash_size, s)) != hipSuccess) return e;
+    k_bbox<<<1, 1024, 0, s>>>(S.x, n, S.bbox);
+    k_cell_count<<<blocks_for(n, 256), 256, 0, s>>>(S.x, n, S.bbox, cs, mask, S.cell_of, S.bucket_count);
+    k_scan<<<1, 1024, 0, s>>>(S.bucket_count, S.bucket_start, S.hash_size);
+    k_cell_fill<<<blocks_for(n, 256), 256, 0, s>>>(n, S.cell_of, S.bucket_start, S.bucket_fill, S.cell_items);
+    k_nl_fill<<<blocks_for(n, 64), 64, 0, s>>>(S.x, n, S.bbox, cs, mask, S.p.r_verlet, S.bucket_count,
+                                               S.bucket_start, S.cell_items, S.nb, S.ncnt, S.cap_n, S.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_diagnostics(const double *x, const double *v, const double *w, uint32_t n, double mass,
+                              double inertia, double *out, hipStream_t s) {
+    k_diag<<<1, 1024, 0, s>>>(x, v, w, n, mass, inertia, out);
+    return hipGetLastError();
+}
+
+}  // namespace hg

@@ -1,0 +1,498 @@
+// C ABI of the single-system DEM path (include/hipgran.h): lifecycle, state
+// transfer, stepping and inspection. Device work is queued on the system's own
+// stream; entry points that return data synchronise that stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hg_internal.h"
+
+namespace hg {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string &msg) { g_err = msg; }
+hg_status fail(hg_status st, const std::string &msg) {
+    g_err = msg;
+    return st;
+}
+
+namespace {
+
+template <typename T>
+hg_status dalloc(hg_system *S, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HG_ERR_OOM : HG_ERR_HIP,
+                    std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " +
+                        hipGetErrorString(e));
+    S->allocations.push_back(*p);
+    return HG_OK;
+}
+
+#define HG_TRY(expr)                   \
+    do {                               \
+        hg_status st_ = (expr);        \
+        if (st_ != HG_OK) return st_;  \
+    } while (0)
+
+hg_status check_device() {
+    int dev = 0;
+    HG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HG_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(HG_ERR_NO_DEVICE, std::string("hipgran is built for gfx950; current device is ") +
+                                          prop.gcnArchName);
+    return HG_OK;
+}
+
+// AoS triples -> 4-double records (and back).
+std::vector<double> pack4(const double *src, size_t n) {
+    std::vector<double> out(4 * n, 0.0);
+    if (src)
+        for (size_t i = 0; i < n; i++) {
+            out[4 * i] = src[3 * i];
+            out[4 * i + 1] = src[3 * i + 1];
+            out[4 * i + 2] = src[3 * i + 2];
+        }
+    return out;
+}
+
+hg_status download3(hg_system *S, const double *dev, double *host3) {
+    if (!host3) return HG_OK;
+    std::vector<double> tmp(4 * S->n);
+    HG_HIP(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    for (size_t i = 0; i < S->n; i++) {
+        host3[3 * i] = tmp[4 * i];
+        host3[3 * i + 1] = tmp[4 * i + 1];
+        host3[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return HG_OK;
+}
+
+hg_status upload3(hg_system *S, double *dev, const double *host3) {
+    std::vector<double> tmp = pack4(host3, S->n);
+    HG_HIP(hipMemcpyAsync(dev, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    return HG_OK;
+}
+
+uint32_t next_pow2(uint64_t v) {
+    uint32_t p = 1;
+    while (p < v && p < (1u << 30)) p <<= 1;
+    return p;
+}
+
+// Device flags -> status. Clears them for the next call.
+hg_status check_flags(hg_system *S, const char *where) {
+    HG_HIP(hipMemcpyAsync(S->flags_host, S->flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    const int nl = S->flags_host[FLAG_NL_OVERFLOW], pr = S->flags_host[FLAG_PAIR_OVERFLOW];
+    const int nlmax = S->flags_host[FLAG_NL_MAXCOUNT], prmax = S->flags_host[FLAG_PAIR_MAXCOUNT];
+    if (nl || pr) {
+        HG_HIP(hipMemsetAsync(S->flags, 0, FLAG_COUNT * sizeof(int), S->stream));
+        HG_HIP(hipStreamSynchronize(S->stream));
+        if (nl)
+            return fail(HG_ERR_CAPACITY, std::string(where) + ": neighbour list needs " + std::to_string(nlmax) +
+                                             " slots per particle, capacity is " + std::to_string(S->cap_n));
+        return fail(HG_ERR_CAPACITY, std::string(where) + ": pair-state table needs " + std::to_string(prmax) +
+                                         " slots per particle, capacity is " + std::to_string(S->cap_s));
+    }
+    return HG_OK;
+}
+
+sys_view make_view(hg_system *S) {
+    sys_view v;
+    v.n = uint32_t(S->n);
+    v.x = S->x;
+    v.v = S->v;
+    v.w = S->w;
+    v.a = S->a;
+    v.al = S->al;
+    v.nb = S->nb;
+    v.ncnt = S->ncnt;
+    v.pj_old = S->pj[S->cur];
+    v.ps_old = S->ps[S->cur];
+    v.pcnt_old = S->pcnt[S->cur];
+    v.pj_new = S->pj[S->cur ^ 1];
+    v.ps_new = S->ps[S->cur ^ 1];
+    v.pcnt_new = S->pcnt[S->cur ^ 1];
+    v.cap_s = S->cap_s;
+    v.facets = S->facets;
+    v.ss = S->ss;
+    v.nf = S->nf;
+    v.flags = S->flags;
+    return v;
+}
+
+// One acceleration pass (queued; no synchronisation).
+hg_status force_pass(hg_system *S) {
+    sys_view v = make_view(S);
+    HG_HIP(launch_force(S->p, v, S->stream));
+    if (S->p.binary_model != HG_BINARY_NONE) S->cur ^= 1;
+    return HG_OK;
+}
+
+hg_status validate(const hg_params *p, size_t n) {
+    if (!p) return fail(HG_ERR_INVALID, "params is null");
+    if (n == 0 || n >= (size_t(1) << 31)) return fail(HG_ERR_INVALID, "n must be in [1, 2^31)");
+    if (p->binary_model < HG_BINARY_NONE || p->binary_model > HG_BINARY_SINTER)
+        return fail(HG_ERR_INVALID, "unknown binary_model");
+    if (p->hamaker_model < HG_HAMAKER_NONE || p->hamaker_model > HG_HAMAKER_LIMITED)
+        return fail(HG_ERR_INVALID, "unknown hamaker_model");
+    if (p->pair_mode != HG_PAIRS_ALL && p->pair_mode != HG_PAIRS_NEIGHBOR_LIST)
+        return fail(HG_ERR_INVALID, "unknown pair_mode");
+    if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(p->r_verlet > 0.0))
+        return fail(HG_ERR_INVALID, "neighbour-list mode needs r_verlet > 0");
+    if (p->precision != HG_PRECISION_FP64)
+        return fail(HG_ERR_INVALID, "single-system path supports HG_PRECISION_FP64 only; mixed precision runs "
+                                    "through the batched path");
+    return HG_OK;
+}
+
+}  // namespace
+}  // namespace hg
+
+using namespace hg;
+
+extern "C" {
+
+const char *hg_last_error(void) { return g_err.c_str(); }
+
+hg_status hg_create(const hg_params *params, size_t n, const double *x0, const double *v0, const double *theta0,
+                    const double *omega0, hg_system **out) {
+    if (!out) return fail(HG_ERR_INVALID, "out is null");
+    *out = nullptr;
+    HG_TRY(validate(params, n));
+    if (!x0) return fail(HG_ERR_INVALID, "x0 is null");
+    HG_TRY(check_device());
+    hg_system *S = new hg_system();
+    S->p = *params;
+    S->n = n;
+    HG_HIP(hipGetDevice(&S->device));
+    auto cleanup_on = [&](hg_status st) {
+        if (st != HG_OK) {
+            std::string msg = g_err;
+            hg_destroy(S);
+            g_err = msg;
+        }
+        return st;
+    };
+#define HG_TRYC(expr)                              \
+    do {                                           \
+        hg_status st_ = (expr);                    \
+        if (st_ != HG_OK) return cleanup_on(st_);  \
+    } while (0)
+#define HG_HIPC(call)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return cleanup_on(fail(HG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)));      \
+    } while (0)
+
+    HG_HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    double **fields[6] = {&S->x, &S->v, &S->th, &S->w, &S->a, &S->al};
+    for (auto f : fields) HG_TRYC(dalloc(S, f, 4 * n));
+    HG_TRYC(upload3(S, S->x, x0));
+    HG_TRYC(upload3(S, S->v, v0));
+    HG_TRYC(upload3(S, S->th, theta0));
+    HG_TRYC(upload3(S, S->w, omega0));
+    HG_HIPC(launch_zero_f64(S->a, 4 * n, S->stream));
+    HG_HIPC(launch_zero_f64(S->al, 4 * n, S->stream));
+    HG_TRYC(dalloc(S, &S->flags, FLAG_COUNT));
+    HG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&S->flags_host), FLAG_COUNT * sizeof(int)));
+    HG_HIPC(hipMemsetAsync(S->flags, 0, FLAG_COUNT * sizeof(int), S->stream));
+    HG_TRYC(dalloc(S, &S->red, 16));
+
+    // Bond topology (alt_sinter ctor) and the initial pair-state table.
+    std::vector<std::vector<uint32_t>> rows(n);
+    if (S->p.binary_model == HG_BINARY_SINTER) {
+        sinter_topology_host(n, x0, S->p.sinter.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped);
+        for (auto &b : S->bonds) {
+            rows[b.first].push_back(b.second | kBondBit);
+            rows[b.second].push_back(b.first | kBondBit);
+        }
+        for (auto &r : rows)
+            std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return (a & kIdxMask) < (b & kIdxMask); });
+    }
+    size_t max_bonds = 0;
+    for (auto &r : rows) max_bonds = std::max(max_bonds, r.size());
+    if (S->p.binary_model != HG_BINARY_NONE) {
+        S->cap_s = S->p.pair_capacity ? S->p.pair_capacity : uint32_t((max_bonds + 24 + 3) / 4 * 4);
+        if (S->cap_s < max_bonds)
+            return cleanup_on(fail(HG_ERR_CAPACITY, "pair_capacity is below the largest bond count " +
+                                                        std::to_string(max_bonds)));
+        for (int b = 0; b < 2; b++) {
+            HG_TRYC(dalloc(S, &S->pj[b], size_t(S->cap_s) * n));
+            HG_TRYC(dalloc(S, &S->ps[b], size_t(S->cap_s) * 9 * n));
+            HG_TRYC(dalloc(S, &S->pcnt[b], n));
+        }
+        std::vector<uint32_t> pj(size_t(S->cap_s) * n, kNoEntry), cnt(n);
+        for (size_t i = 0; i < n; i++) {
+            cnt[i] = uint32_t(rows[i].size());
+            for (size_t s = 0; s < rows[i].size(); s++) pj[s * n + i] = rows[i][s];
+        }
+        HG_HIPC(hipMemcpyAsync(S->pj[0], pj.data(), pj.size() * 4, hipMemcpyHostToDevice, S->stream));
+        HG_HIPC(hipMemcpyAsync(S->pcnt[0], cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, S->stream));
+        HG_HIPC(launch_zero_f64(S->ps[0], size_t(S->cap_s) * 9 * n, S->stream));
+        HG_HIPC(hipStreamSynchronize(S->stream));
+    }
+
+    if (S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
+        // Default capacity: close-packing bound of spheres of radius r whose
+        // centres lie within r_verlet, with 30% headroom, rounded up to 8.
+        const double r = S->p.contact.r_part > 0 ? S->p.contact.r_part
+                                                 : (S->p.sinter.r_part > 0 ? S->p.sinter.r_part : S->p.hamaker.r_part);
+        uint64_t cap = S->p.neighbor_capacity;
+        if (cap == 0) {
+            double ratio = r > 0 ? S->p.r_verlet / r : 64.0;
+            double bound = 0.7405 * (std::pow(ratio + 1.0, 3.0) - 1.0) * 1.3;
+            cap = uint64_t(std::ceil(std::max(bound, 8.0) / 8.0) * 8.0);
+        }
+        cap = std::min<uint64_t>(cap, n > 1 ? n - 1 : 1);
+        S->cap_n = uint32_t(std::max<uint64_t>(cap, 1));
+        HG_TRYC(dalloc(S, &S->nb, size_t(S->cap_n) * n));
+        HG_TRYC(dalloc(S, &S->ncnt, n));
+        S->hash_size = next_pow2(std::max<uint64_t>(2 * n, 1024));
+        HG_TRYC(dalloc(S, &S->cell_of, n));
+        HG_TRYC(dalloc(S, &S->cell_items, n));
+        HG_TRYC(dalloc(S, &S->bucket_count, S->hash_size));
+        HG_TRYC(dalloc(S, &S->bucket_start, S->hash_size));
+        HG_TRYC(dalloc(S, &S->bucket_fill, S->hash_size));
+        HG_TRYC(dalloc(S, &S->bbox, 8));
+        HG_HIPC(launch_neighbor_build(*S, S->stream));
+        HG_TRYC(check_flags(S, "hg_create"));
+    }
+    HG_HIPC(hipStreamSynchronize(S->stream));
+    *out = S;
+    return HG_OK;
+#undef HG_TRYC
+#undef HG_HIPC
+}
+
+hg_status hg_destroy(hg_system *S) {
+    if (!S) return HG_OK;
+    if (S->stream) (void)hipStreamSynchronize(S->stream);
+    for (void *p : S->allocations) (void)hipFree(p);
+    if (S->flags_host) (void)hipHostFree(S->flags_host);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+    delete S;
+    return HG_OK;
+}
+
+hg_status hg_set_facets(hg_system *S, size_t nf, const double *verts, const double *v_facet) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (nf > size_t(kMaxFacets)) return fail(HG_ERR_INVALID, "at most 16 facets are supported");
+    if (nf > 0 && (!verts || !v_facet)) return fail(HG_ERR_INVALID, "verts and v_facet are required");
+    std::vector<facet_dev> F(nf);
+    for (size_t f = 0; f < nf; f++) {
+        const double *q = verts + 9 * f;
+        for (int k = 0; k < 3; k++) F[f].vert[k] = d3{q[3 * k], q[3 * k + 1], q[3 * k + 2]};
+        F[f].v_facet = d3{v_facet[3 * f], v_facet[3 * f + 1], v_facet[3 * f + 2]};
+        // triangular_facet.h:23-29
+        d3 u = F[f].vert[1] - F[f].vert[0], w = F[f].vert[2] - F[f].vert[0];
+        F[f].area = 1.0 / 2.0 * norm(cross(u, w));
+        F[f].normal = normalized(cross(u, w));
+        F[f].pad[0] = F[f].pad[1] = 0.0;
+    }
+    if (!S->facets) HG_TRY(dalloc(S, &S->facets, kMaxFacets));
+    if (!S->ss && nf > 0) HG_TRY(dalloc(S, &S->ss, size_t(kMaxFacets) * 9 * S->n));
+    if (nf > 0) {
+        HG_HIP(hipMemcpyAsync(S->facets, F.data(), nf * sizeof(facet_dev), hipMemcpyHostToDevice, S->stream));
+        HG_HIP(launch_zero_f64(S->ss, nf * 9 * S->n, S->stream));
+    }
+    HG_HIP(hipStreamSynchronize(S->stream));
+    S->nf = int(nf);
+    S->acc_valid = false;
+    return HG_OK;
+}
+
+hg_status hg_get_facets(hg_system *S, double *verts, double *v_facet) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (S->nf == 0) return HG_OK;
+    std::vector<facet_dev> F(S->nf);
+    HG_HIP(hipMemcpyAsync(F.data(), S->facets, F.size() * sizeof(facet_dev), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    for (int f = 0; f < S->nf; f++) {
+        for (int k = 0; k < 3; k++) {
+            if (verts) {
+                verts[9 * f + 3 * k] = F[f].vert[k].x;
+                verts[9 * f + 3 * k + 1] = F[f].vert[k].y;
+                verts[9 * f + 3 * k + 2] = F[f].vert[k].z;
+            }
+        }
+        if (v_facet) {
+            v_facet[3 * f] = F[f].v_facet.x;
+            v_facet[3 * f + 1] = F[f].v_facet.y;
+            v_facet[3 * f + 2] = F[f].v_facet.z;
+        }
+    }
+    return HG_OK;
+}
+
+hg_status hg_rebuild_neighbors(hg_system *S) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (S->p.pair_mode != HG_PAIRS_NEIGHBOR_LIST) return fail(HG_ERR_INVALID, "system has no neighbour list");
+    HG_HIP(launch_neighbor_build(*S, S->stream));
+    return check_flags(S, "hg_rebuild_neighbors");
+}
+
+hg_status hg_compute_accelerations(hg_system *S) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(force_pass(S));
+    S->acc_valid = true;
+    return check_flags(S, "hg_compute_accelerations");
+}
+
+hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_every) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (nsteps == 0) return HG_OK;
+    if (!S->acc_valid) {
+        HG_TRY(force_pass(S));
+        S->acc_valid = true;
+    }
+    const double half = dt / 2.0;
+    const uint32_t n = uint32_t(S->n);
+    const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
+    for (uint64_t s = 0; s < nsteps; s++) {
+        if (nl && rebuild_every && (S->step_index % rebuild_every) == 0) HG_HIP(launch_neighbor_build(*S, S->stream));
+        HG_HIP(launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt, S->facets, S->nf,
+                                 dt, S->stream));
+        HG_TRY(force_pass(S));
+        S->t = S->t + dt;
+        S->step_index++;
+    }
+    HG_HIP(launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, S->nf, dt, S->stream));
+    return check_flags(S, "hg_step");
+}
+
+hg_status hg_get_state(hg_system *S, double *x, double *v, double *theta, double *omega) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(download3(S, S->x, x));
+    HG_TRY(download3(S, S->v, v));
+    HG_TRY(download3(S, S->th, theta));
+    HG_TRY(download3(S, S->w, omega));
+    return HG_OK;
+}
+
+hg_status hg_set_state(hg_system *S, const double *x, const double *v, const double *theta, const double *omega) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (x) HG_TRY(upload3(S, S->x, x));
+    if (v) HG_TRY(upload3(S, S->v, v));
+    if (theta) HG_TRY(upload3(S, S->th, theta));
+    if (omega) HG_TRY(upload3(S, S->w, omega));
+    S->acc_valid = false;
+    return HG_OK;
+}
+
+hg_status hg_get_accelerations(hg_system *S, double *a, double *alpha) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(download3(S, S->a, a));
+    HG_TRY(download3(S, S->al, alpha));
+    return HG_OK;
+}
+
+hg_status hg_get_time(hg_system *S, double *t, uint64_t *step_index) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (t) *t = S->t;
+    if (step_index) *step_index = S->step_index;
+    return HG_OK;
+}
+
+hg_status hg_get_neighbors(hg_system *S, uint64_t *offsets, uint32_t *idx, size_t cap, size_t *total) {
+    if (!S || !offsets) return fail(HG_ERR_INVALID, "system and offsets are required");
+    if (S->p.pair_mode != HG_PAIRS_NEIGHBOR_LIST) return fail(HG_ERR_INVALID, "system has no neighbour list");
+    const size_t n = S->n;
+    std::vector<uint32_t> nb(size_t(S->cap_n) * n), cnt(n);
+    HG_HIP(hipMemcpyAsync(nb.data(), S->nb, nb.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(cnt.data(), S->ncnt, cnt.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    size_t tot = 0;
+    for (size_t i = 0; i < n; i++) {
+        offsets[i] = tot;
+        for (uint32_t k = 0; k < cnt[i]; k++) {
+            if (idx && tot < cap) idx[tot] = nb[size_t(k) * n + i];
+            tot++;
+        }
+    }
+    offsets[n] = tot;
+    if (total) *total = tot;
+    return HG_OK;
+}
+
+hg_status hg_get_bonds(hg_system *S, uint32_t *pairs, size_t cap, size_t *nbonds, size_t *nskipped) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    for (size_t b = 0; pairs && b < S->bonds.size() && b < cap; b++) {
+        pairs[2 * b] = S->bonds[b].first;
+        pairs[2 * b + 1] = S->bonds[b].second;
+    }
+    if (nbonds) *nbonds = S->bonds.size();
+    if (nskipped) *nskipped = S->bonds_skipped;
+    return HG_OK;
+}
+
+hg_status hg_get_springs(hg_system *S, int which, uint32_t *ij, double *springs, size_t cap, size_t *total) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (total) *total = 0;
+    if (S->p.binary_model == HG_BINARY_NONE) return HG_OK;
+    const size_t n = S->n, cs = S->cap_s;
+    std::vector<uint32_t> pj(cs * n), cnt(n);
+    std::vector<double> ps(cs * 9 * n);
+    HG_HIP(hipMemcpyAsync(pj.data(), S->pj[S->cur], pj.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(ps.data(), S->ps[S->cur], ps.size() * 8, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(cnt.data(), S->pcnt[S->cur], cnt.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    size_t tot = 0;
+    for (size_t i = 0; i < n; i++)
+        for (uint32_t s = 0; s < cnt[i]; s++) {
+            const uint32_t tag = pj[size_t(s) * n + i];
+            const bool bonded = (tag & kBondBit) != 0;
+            if (bonded != (which == 1)) continue;
+            if (tot < cap) {
+                if (ij) {
+                    ij[2 * tot] = uint32_t(i);
+                    ij[2 * tot + 1] = tag & kIdxMask;
+                }
+                if (springs)
+                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * 9 + c) * n + i];
+            }
+            tot++;
+        }
+    if (total) *total = tot;
+    return HG_OK;
+}
+
+hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs, size_t cap,
+                             size_t *nbonds, size_t *nskipped) {
+    if (n > 0 && !x0) return fail(HG_ERR_INVALID, "x0 is null");
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped = 0;
+    sinter_topology_host(n, x0, r_part, d_crit, bonds, skipped);
+    for (size_t b = 0; pairs && b < bonds.size() && b < cap; b++) {
+        pairs[2 * b] = bonds[b].first;
+        pairs[2 * b + 1] = bonds[b].second;
+    }
+    if (nbonds) *nbonds = bonds.size();
+    if (nskipped) *nskipped = skipped;
+    return HG_OK;
+}
+
+hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out) {
+    if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    HG_HIP(launch_diagnostics(S->x, S->v, S->w, uint32_t(S->n), mass, inertia, S->red, S->stream));
+    HG_HIP(hipMemcpyAsync(out, S->red, 9 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    return HG_OK;
+}
+
+}  // extern "C"

@@ -12,14 +12,8 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    gpu_ok = None
+    # A ROCm GPU is visible when the kernel driver node exists.
+    gpu_ok = os.path.exists("/dev/kfd")
     for it in items:
-        if "gpu" in it.keywords:
-            if gpu_ok is None:
-                try:
-                    import torch  # noqa: F401
-                    gpu_ok = torch.cuda.is_available()
-                except Exception:
-                    gpu_ok = False
-            if not gpu_ok:
-                it.add_marker(pytest.mark.skip(reason="no GPU visible"))
+        if "gpu" in it.keywords and not gpu_ok:
+            it.add_marker(pytest.mark.skip(reason="no GPU visible"))

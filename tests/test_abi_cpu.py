@@ -1,0 +1,74 @@
+"""Host-side checks of the hipgran library that need no GPU: the shared
+object loads, exports every function include/hipgran.h declares, and its host
+logic (bond topology, aggregate generator) agrees with the oracle."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+libgran_amd = pytest.importorskip("libgran_amd")
+from libgran_amd import _abi  # noqa: E402
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "functors.npz"))
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _abi.header_symbols()
+    assert len(declared) >= 18
+    L = C.CDLL(_abi.LIB_PATH)
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(declared) == set(_abi.SIGNATURES), "binding and header disagree"
+
+
+def test_params_struct_layout_matches_header():
+    # 8 int32 then a double: r_verlet at byte 32; 59 doubles of functor parameters.
+    assert _abi.Params.r_verlet.offset == 32
+    assert C.sizeof(_abi.Params) == 40 + 8 * (18 + 13 + 5 + 18 + 5)
+
+
+def test_topology_matches_reference_fixtures():
+    pairs, sk = libgran_amd.sinter_topology(GOLD["agg150_x"], O.R_PART, 1e-3 * O.R_PART)
+    np.testing.assert_array_equal(pairs, GOLD["agg150_bonds"])
+    assert sk == 0
+    pairs, sk = libgran_amd.sinter_topology(GOLD["lattice_x"], O.R_PART, 1.0e-9)
+    np.testing.assert_array_equal(pairs, GOLD["lattice_bonds"])
+    assert sk == 28
+
+
+@pytest.mark.parametrize("n,seed,df", [(200, 0, 1.8), (1000, 3, 2.2), (3000, 1, 1.6)])
+def test_topology_matches_oracle_on_generated_aggregates(n, seed, df):
+    x = libgran_amd.generate_aggregate(n, O.R_PART, df, 1.3, seed)
+    for d_crit in (1e-3 * O.R_PART, 1e-9, 0.2 * O.R_PART):
+        p1, s1 = libgran_amd.sinter_topology(x, O.R_PART, d_crit)
+        p2, s2 = O.oracle_topology(x, O.R_PART, d_crit)
+        np.testing.assert_array_equal(p1, p2)
+        assert s1 == s2
+
+
+def test_generator_is_deterministic_touching_and_connected():
+    n = 400
+    x = libgran_amd.generate_aggregate(n, O.R_PART, 1.8, 1.3, 7)
+    x2 = libgran_amd.generate_aggregate(n, O.R_PART, 1.8, 1.3, 7)
+    np.testing.assert_array_equal(x, x2)
+    assert not np.array_equal(x, libgran_amd.generate_aggregate(n, O.R_PART, 1.8, 1.3, 8))
+    d = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=2)
+    np.fill_diagonal(d, np.inf)
+    assert d.min() > 2 * O.R_PART * (1 - 1e-9), "spheres overlap"
+    pairs, sk = libgran_amd.sinter_topology(x, O.R_PART, 1e-3 * O.R_PART)
+    assert len(pairs) == n - 1 and sk >= 0  # the touching graph spans the aggregate
+    # centred on its centre of mass
+    assert np.linalg.norm(x.mean(axis=0)) < 1e-9 * O.R_PART * n
+
+
+def test_generator_fractal_scaling():
+    """N = kf (Rg/r)^Df holds approximately for the generated aggregates."""
+    for df in (1.6, 1.8, 2.2):
+        n = 2000
+        x = libgran_amd.generate_aggregate(n, O.R_PART, df, 1.3, 11)
+        rg = np.sqrt(((x - x.mean(axis=0)) ** 2).sum(axis=1).mean())
+        n_pred = 1.3 * (rg / O.R_PART) ** df
+        assert 0.5 < n_pred / n < 2.0, (df, n_pred)
