@@ -172,6 +172,16 @@ hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *ou
 hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs, size_t cap,
                              size_t *nbonds, size_t *nskipped);
 
+/* Device-time profile of a system's launches, measured with HIP events
+ * recorded on the system's stream around each launch (not in the reference,
+ * which only times whole runs with std::chrono, main.cpp:146-164).
+ * hg_profile_enable(sys, 1) resets and starts; 0 stops. hg_profile_read
+ * synchronises the stream and returns, per phase, the launch count and the
+ * summed device milliseconds: [0] acceleration pass, [1] kick/drift and
+ * final kick, [2] neighbour rebuild (all of its kernels), [3] reserved. */
+hg_status hg_profile_enable(hg_system *sys, int enable);
+hg_status hg_profile_read(hg_system *sys, uint64_t *launches, double *device_ms);
+
 /* Seeded tunable-dimension cluster-cluster aggregate of n touching spheres of
  * radius r_part (fractal dimension df, prefactor kf). x_out = 3*n doubles.
  * Not in the reference: the input generator of the aggregate workloads. */

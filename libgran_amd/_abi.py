@@ -80,6 +80,8 @@ SIGNATURES = {
     "hg_get_springs": (C.c_int, [_sysp, C.c_int, _u32p, _dp, C.c_size_t, _szp]),
     "hg_diagnostics": (C.c_int, [_sysp, C.c_double, C.c_double, _dp]),
     "hg_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp, _szp]),
+    "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
+    "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
 }
 

@@ -84,7 +84,19 @@ struct hg_system_impl {
     std::vector<std::pair<uint32_t, uint32_t>> bonds;
     size_t bonds_skipped = 0;
     std::vector<void *> allocations;
+    // launch profiling (hg_profile_enable)
+    bool profiling = false;
+    struct timed_range {
+        int phase;
+        hipEvent_t begin, end;
+    };
+    std::vector<hipEvent_t> event_pool;
+    std::vector<timed_range> pending;
+    uint64_t prof_launches[4] = {0, 0, 0, 0};
+    double prof_ms[4] = {0, 0, 0, 0};
 };
+
+enum : int { PHASE_FORCE = 0, PHASE_KICK = 1, PHASE_REBUILD = 2 };
 
 // error helpers (thread-local message)
 void set_error(const std::string &msg);

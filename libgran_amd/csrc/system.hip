@@ -93,10 +93,52 @@ uint32_t next_pow2(uint64_t v) {
     return p;
 }
 
+hipError_t get_event(hg_system *S, hipEvent_t *e) {
+    if (!S->event_pool.empty()) {
+        *e = S->event_pool.back();
+        S->event_pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(e);
+}
+
+// Queues `launch` on the system stream, bracketed by events when profiling.
+template <typename F>
+hipError_t timed(hg_system *S, int phase, F &&launch) {
+    if (!S->profiling) return launch();
+    hipEvent_t a, b;
+    hipError_t e;
+    if ((e = get_event(S, &a)) != hipSuccess) return e;
+    if ((e = get_event(S, &b)) != hipSuccess) return e;
+    if ((e = hipEventRecord(a, S->stream)) != hipSuccess) return e;
+    if ((e = launch()) != hipSuccess) return e;
+    if ((e = hipEventRecord(b, S->stream)) != hipSuccess) return e;
+    S->pending.push_back({phase, a, b});
+    return hipSuccess;
+}
+
+// After a stream synchronisation: fold finished ranges into the totals.
+hg_status collect_profile(hg_system *S) {
+    for (auto &r : S->pending) {
+        float ms = 0.f;
+        HG_HIP(hipEventElapsedTime(&ms, r.begin, r.end));
+        S->prof_ms[r.phase] += double(ms);
+        S->prof_launches[r.phase]++;
+        S->event_pool.push_back(r.begin);
+        S->event_pool.push_back(r.end);
+    }
+    S->pending.clear();
+    return HG_OK;
+}
+
 // Device flags -> status. Clears them for the next call.
 hg_status check_flags(hg_system *S, const char *where) {
     HG_HIP(hipMemcpyAsync(S->flags_host, S->flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipStreamSynchronize(S->stream));
+    if (!S->pending.empty()) {
+        hg_status st = collect_profile(S);
+        if (st != HG_OK) return st;
+    }
     const int nl = S->flags_host[FLAG_NL_OVERFLOW], pr = S->flags_host[FLAG_PAIR_OVERFLOW];
     const int nlmax = S->flags_host[FLAG_NL_MAXCOUNT], prmax = S->flags_host[FLAG_PAIR_MAXCOUNT];
     if (nl || pr) {
@@ -138,7 +180,7 @@ sys_view make_view(hg_system *S) {
 // One acceleration pass (queued; no synchronisation).
 hg_status force_pass(hg_system *S) {
     sys_view v = make_view(S);
-    HG_HIP(launch_force(S->p, v, S->stream));
+    HG_HIP(timed(S, PHASE_FORCE, [&] { return launch_force(S->p, v, S->stream); }));
     if (S->p.binary_model != HG_BINARY_NONE) S->cur ^= 1;
     return HG_OK;
 }
@@ -284,6 +326,11 @@ hg_status hg_destroy(hg_system *S) {
     if (!S) return HG_OK;
     if (S->stream) (void)hipStreamSynchronize(S->stream);
     for (void *p : S->allocations) (void)hipFree(p);
+    for (auto &r : S->pending) {
+        (void)hipEventDestroy(r.begin);
+        (void)hipEventDestroy(r.end);
+    }
+    for (auto e : S->event_pool) (void)hipEventDestroy(e);
     if (S->flags_host) (void)hipHostFree(S->flags_host);
     if (S->stream) (void)hipStreamDestroy(S->stream);
     delete S;
@@ -365,14 +412,19 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
     const uint32_t n = uint32_t(S->n);
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     for (uint64_t s = 0; s < nsteps; s++) {
-        if (nl && rebuild_every && (S->step_index % rebuild_every) == 0) HG_HIP(launch_neighbor_build(*S, S->stream));
-        HG_HIP(launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt, S->facets, S->nf,
-                                 dt, S->stream));
+        if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
+            HG_HIP(timed(S, PHASE_REBUILD, [&] { return launch_neighbor_build(*S, S->stream); }));
+        HG_HIP(timed(S, PHASE_KICK, [&] {
+            return launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt, S->facets,
+                                     S->nf, dt, S->stream);
+        }));
         HG_TRY(force_pass(S));
         S->t = S->t + dt;
         S->step_index++;
     }
-    HG_HIP(launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, S->nf, dt, S->stream));
+    HG_HIP(timed(S, PHASE_KICK, [&] {
+        return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, S->nf, dt, S->stream);
+    }));
     return check_flags(S, "hg_step");
 }
 
@@ -469,6 +521,29 @@ hg_status hg_get_springs(hg_system *S, int which, uint32_t *ij, double *springs,
             tot++;
         }
     if (total) *total = tot;
+    return HG_OK;
+}
+
+hg_status hg_profile_enable(hg_system *S, int enable) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_HIP(hipStreamSynchronize(S->stream));
+    HG_TRY(collect_profile(S));
+    for (int k = 0; k < 4; k++) {
+        S->prof_launches[k] = 0;
+        S->prof_ms[k] = 0.0;
+    }
+    S->profiling = enable != 0;
+    return HG_OK;
+}
+
+hg_status hg_profile_read(hg_system *S, uint64_t *launches, double *device_ms) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_HIP(hipStreamSynchronize(S->stream));
+    HG_TRY(collect_profile(S));
+    for (int k = 0; k < 4; k++) {
+        if (launches) launches[k] = S->prof_launches[k];
+        if (device_ms) device_ms[k] = S->prof_ms[k];
+    }
     return HG_OK;
 }
 
