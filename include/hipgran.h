@@ -172,6 +172,59 @@ hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *ou
 hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs, size_t cap,
                              size_t *nbonds, size_t *nskipped);
 
+/* ------------------------------------------------------------------------
+ * Pools: many independent systems stepped together.
+ *
+ * Not in the reference, which simulates one granular_system per process. A
+ * pool holds up to max_systems systems that share one hg_params (functor set,
+ * pair mode, precision, dt) in one device arena of max_particles particles.
+ * hg_pool_step advances every active system by the same number of steps with
+ * one persistent kernel launch per rebuild interval: each workgroup takes one
+ * whole system at a time (largest first) and runs all of its steps with
+ * workgroup barriers between the kick/drift and force phases. A system's
+ * results do not depend on which other systems share the pool. Systems can be
+ * added and removed between steps (continuous batching); a system's
+ * per-step behaviour is that of an hg_system with the same params.
+ * ------------------------------------------------------------------------ */
+typedef struct hg_pool hg_pool;
+
+/* max_facets: facets per system (0..16). */
+hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t max_systems, size_t max_facets,
+                         hg_pool **out);
+hg_status hg_pool_destroy(hg_pool *pool);
+/* Adds a system; its particles must fit in one free range of the arena,
+ * otherwise HG_ERR_CAPACITY (nothing is changed). Bonds are found on x0 as in
+ * hg_create. *slot receives the system's id. */
+hg_status hg_pool_add(hg_pool *pool, size_t n, const double *x0, const double *v0, const double *theta0,
+                      const double *omega0, int32_t *slot);
+/* Removes a system and frees its particle range. */
+hg_status hg_pool_remove(hg_pool *pool, int32_t slot);
+/* Facets of one system (nf <= max_facets), as hg_set_facets. */
+hg_status hg_pool_set_facets(hg_pool *pool, int32_t slot, size_t nf, const double *verts, const double *v_facet);
+/* Advances every active system by nsteps of size dt. In neighbour-list mode
+ * with rebuild_every > 0, nsteps must be a multiple of rebuild_every and each
+ * system's list is rebuilt before every step whose index (the system's own
+ * step count) is a multiple of rebuild_every. */
+hg_status hg_pool_step(hg_pool *pool, double dt, uint64_t nsteps, uint32_t rebuild_every);
+/* Per-system copies, as the hg_system getters. */
+hg_status hg_pool_get_state(hg_pool *pool, int32_t slot, double *x, double *v, double *theta, double *omega);
+hg_status hg_pool_get_accelerations(hg_pool *pool, int32_t slot, double *a, double *alpha);
+hg_status hg_pool_get_neighbors(hg_pool *pool, int32_t slot, uint64_t *offsets, uint32_t *idx, size_t cap,
+                                size_t *total);
+hg_status hg_pool_get_springs(hg_pool *pool, int32_t slot, int which, uint32_t *ij, double *springs, size_t cap,
+                              size_t *total);
+hg_status hg_pool_get_bonds(hg_pool *pool, int32_t slot, uint32_t *pairs, size_t cap, size_t *nbonds,
+                            size_t *nskipped);
+/* Particle count, simulation time and step count of one system. */
+hg_status hg_pool_info(hg_pool *pool, int32_t slot, size_t *n, double *t, uint64_t *step_index);
+/* Bytes of device memory one system of n particles occupies in a pool with
+ * these params (state, neighbour list, pair-state table, facet springs):
+ * the admission unit of an HBM budget. */
+size_t hg_pool_bytes_per_particle(const hg_params *params, size_t max_facets);
+/* Device time of the last hg_pool_step's chunk launches (milliseconds, HIP
+ * events on the pool's stream) and the number of launches. */
+hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *rebuild_ms, uint64_t *chunks);
+
 /* Device-time profile of a system's launches, measured with HIP events
  * recorded on the system's stream around each launch (not in the reference,
  * which only times whole runs with std::chrono, main.cpp:146-164).

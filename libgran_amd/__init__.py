@@ -18,7 +18,7 @@ from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, HAMAKER_FULL, HAM
 
 _abi.lib()  # fail at import time when the HIP library is absent
 
-__all__ = ["System", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
+__all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
            "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
            "PAIRS_ALL", "PAIRS_NEIGHBOR_LIST", "PRECISION_FP64", "PRECISION_MIXED"]
 
@@ -142,3 +142,104 @@ class System:
         out = np.zeros(9)
         _abi.check(_abi.lib().hg_diagnostics(self._h, mass, inertia, _abi.dptr(out)))
         return dict(ke=out[0], ke_trans=out[1], ke_rot=out[2], P=out[3], L=out[4], rg=out[5], cm=out[6:9].copy())
+
+    def profile(self, enable=True):
+        _abi.check(_abi.lib().hg_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        launches = np.zeros(4, dtype=np.uint64)
+        ms = np.zeros(4)
+        _abi.check(_abi.lib().hg_profile_read(self._h, launches.ctypes.data_as(_abi._u64p), _abi.dptr(ms)))
+        return launches, ms
+
+
+class Pool:
+    """Many independent systems stepped together (hg_pool)."""
+
+    def __init__(self, params, max_particles, max_systems, max_facets=0):
+        self.params = params
+        self._h = C.c_void_p()
+        _abi.check(_abi.lib().hg_pool_create(C.byref(params), max_particles, max_systems, max_facets,
+                                             C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _abi.lib().hg_pool_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    @staticmethod
+    def bytes_per_particle(params, max_facets=0):
+        return int(_abi.lib().hg_pool_bytes_per_particle(C.byref(params), max_facets))
+
+    def add(self, x0, v0=None, theta0=None, omega0=None):
+        x0 = _abi.c64(x0)
+        slot = C.c_int32(-1)
+        _abi.check(_abi.lib().hg_pool_add(self._h, x0.shape[0], _abi.dptr(x0), _abi.dptr(_abi.c64(v0)),
+                                          _abi.dptr(_abi.c64(theta0)), _abi.dptr(_abi.c64(omega0)), C.byref(slot)))
+        return slot.value
+
+    def try_add(self, x0, v0=None, theta0=None, omega0=None):
+        """add(), or None when the arena or the slot table is full."""
+        try:
+            return self.add(x0, v0, theta0, omega0)
+        except HipgranError as e:
+            if e.status == _abi.HG_ERR_CAPACITY:
+                return None
+            raise
+
+    def remove(self, slot):
+        _abi.check(_abi.lib().hg_pool_remove(self._h, slot))
+
+    def set_facets(self, slot, verts, v_facet):
+        verts, v_facet = _abi.c64(verts), _abi.c64(v_facet)
+        _abi.check(_abi.lib().hg_pool_set_facets(self._h, slot, verts.shape[0], _abi.dptr(verts), _abi.dptr(v_facet)))
+
+    def step(self, dt, nsteps, rebuild_every=0):
+        _abi.check(_abi.lib().hg_pool_step(self._h, dt, nsteps, rebuild_every))
+
+    def info(self, slot):
+        n, t, k = C.c_size_t(0), C.c_double(0), C.c_uint64(0)
+        _abi.check(_abi.lib().hg_pool_info(self._h, slot, C.byref(n), C.byref(t), C.byref(k)))
+        return n.value, t.value, k.value
+
+    def state(self, slot):
+        n = self.info(slot)[0]
+        arrs = [np.zeros((n, 3)) for _ in range(6)]
+        _abi.check(_abi.lib().hg_pool_get_state(self._h, slot, *[_abi.dptr(a) for a in arrs[:4]]))
+        _abi.check(_abi.lib().hg_pool_get_accelerations(self._h, slot, _abi.dptr(arrs[4]), _abi.dptr(arrs[5])))
+        return dict(zip(("x", "v", "theta", "omega", "a", "alpha"), arrs))
+
+    def neighbors(self, slot):
+        n = self.info(slot)[0]
+        off = np.zeros(n + 1, dtype=np.uint64)
+        tot = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_pool_get_neighbors(self._h, slot, off.ctypes.data_as(_abi._u64p), None, 0,
+                                                    C.byref(tot)))
+        idx = np.zeros(max(tot.value, 1), dtype=np.uint32)
+        _abi.check(_abi.lib().hg_pool_get_neighbors(self._h, slot, off.ctypes.data_as(_abi._u64p), _abi.u32ptr(idx),
+                                                    tot.value, C.byref(tot)))
+        return off, idx[:tot.value]
+
+    def springs(self, slot, which):
+        tot = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_pool_get_springs(self._h, slot, which, None, None, 0, C.byref(tot)))
+        ij = np.zeros((max(tot.value, 1), 2), dtype=np.uint32)
+        s = np.zeros((max(tot.value, 1), 9))
+        _abi.check(_abi.lib().hg_pool_get_springs(self._h, slot, which, _abi.u32ptr(ij), _abi.dptr(s), tot.value,
+                                                  C.byref(tot)))
+        return ij[:tot.value], s[:tot.value]
+
+    def bonds(self, slot):
+        nb, sk = C.c_size_t(0), C.c_size_t(0)
+        _abi.check(_abi.lib().hg_pool_get_bonds(self._h, slot, None, 0, C.byref(nb), C.byref(sk)))
+        pairs = np.zeros((max(nb.value, 1), 2), dtype=np.uint32)
+        _abi.check(_abi.lib().hg_pool_get_bonds(self._h, slot, _abi.u32ptr(pairs), nb.value, C.byref(nb),
+                                                C.byref(sk)))
+        return pairs[:nb.value], sk.value
+
+    def last_step_profile(self):
+        c, r, k = C.c_double(0), C.c_double(0), C.c_uint64(0)
+        _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k)))
+        return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value)

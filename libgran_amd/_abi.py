@@ -83,6 +83,20 @@ SIGNATURES = {
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
+    "hg_pool_create": (C.c_int, [C.POINTER(Params), C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(_sysp)]),
+    "hg_pool_destroy": (C.c_int, [_sysp]),
+    "hg_pool_add": (C.c_int, [_sysp, C.c_size_t, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
+    "hg_pool_remove": (C.c_int, [_sysp, C.c_int32]),
+    "hg_pool_set_facets": (C.c_int, [_sysp, C.c_int32, C.c_size_t, _dp, _dp]),
+    "hg_pool_step": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
+    "hg_pool_get_state": (C.c_int, [_sysp, C.c_int32, _dp, _dp, _dp, _dp]),
+    "hg_pool_get_accelerations": (C.c_int, [_sysp, C.c_int32, _dp, _dp]),
+    "hg_pool_get_neighbors": (C.c_int, [_sysp, C.c_int32, _u64p, _u32p, C.c_size_t, _szp]),
+    "hg_pool_get_springs": (C.c_int, [_sysp, C.c_int32, C.c_int, _u32p, _dp, C.c_size_t, _szp]),
+    "hg_pool_get_bonds": (C.c_int, [_sysp, C.c_int32, _u32p, C.c_size_t, _szp, _szp]),
+    "hg_pool_info": (C.c_int, [_sysp, C.c_int32, _szp, _dp, _u64p]),
+    "hg_pool_bytes_per_particle": (C.c_size_t, [C.POINTER(Params), C.c_size_t]),
+    "hg_pool_last_step_profile": (C.c_int, [_sysp, _dp, _dp, _u64p]),
 }
 
 _lib = None

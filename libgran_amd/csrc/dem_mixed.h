@@ -1,0 +1,223 @@
+// Mixed-precision forms of the pair and facet force models (HG_PRECISION_MIXED).
+//
+// Kept in FP64: positions and the centre distance |x_i - x_j| with everything
+// derived from it by subtraction (contact overlap, Hamaker gap h, cutoff
+// test), the facet closest point, and the spring vectors: their rotation into
+// the contact plane, rescaling and the accumulation xi += v dt. In FP32 with
+// a*b+c contraction allowed: the normal, relative velocities, normal force,
+// Coulomb test and cap, torques and the Hamaker scalar. The gap h must stay
+// FP64: h = |d| - 2r near h0 = 0.1 nm loses eps_32 * 2r/h0 ~ 1.7e-5 of
+// relative precision in FP32. Accelerations are summed in FP32 per particle.
+//
+// The model equations are those of dem_math.h (contact_force.h:57-147,
+// alt_sinter_bridge.h:92-156, hamaker_force.h:28-43,
+// limited_range_hamaker_force.h:29-47, surface_contact_force.h:57-147,
+// surface_hamaker_force.h:18-30); only the arithmetic types differ.
+#pragma once
+
+#include "dem_math.h"
+
+namespace hg {
+
+struct f3 {
+    float x, y, z;
+};
+HGD f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+HGD f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+HGD f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+HGD f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+HGD f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+HGD float dotf(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+HGD f3 crossf(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+HGD f3 tof(d3 a) { return {float(a.x), float(a.y), float(a.z)}; }
+HGD d3 tod(f3 a) { return {double(a.x), double(a.y), double(a.z)}; }
+HGD f3 zerof() { return {0.f, 0.f, 0.f}; }
+
+struct accf {
+    f3 lin, ang;
+};
+HGD accf operator+(accf a, accf b) { return {a.lin + b.lin, a.ang + b.ang}; }
+
+// Inverse masses/inertias and FP32 model constants, prepared once per launch.
+struct mixed_consts {
+    float c_k, c_gn, c_kt, c_gt, c_mus, c_mud, c_kr, c_gr, c_mur, c_mudr, c_ko, c_go, c_muo, c_mudo, c_r, c_im, c_ii;
+    float s_k, s_gn, s_kt, s_gt, s_kr, s_gr, s_ko, s_go, s_r, s_im, s_ii;
+    float h_a6, h_r, h_im;   // -A/6, r, 1/m
+    double c_r2, s_r2, h_r2, h_h0, h_cut;
+    double c_dt, s_dt;
+    float sc_k, sc_gn, sc_kt, sc_gt, sc_mus, sc_mud, sc_kr, sc_gr, sc_mur, sc_mudr, sc_ko, sc_go, sc_muo, sc_mudo, sc_r,
+        sc_im, sc_ii;
+    double sc_rd, sc_dt;
+    float sh_a23, sh_im;
+    double sh_r, sh_h0;
+};
+
+inline mixed_consts make_mixed_consts(const hg_params &p) {
+    mixed_consts m;
+    const hg_contact_params &c = p.contact;
+    m.c_k = float(c.k); m.c_gn = float(c.gamma_n); m.c_kt = float(c.k_t); m.c_gt = float(c.gamma_t);
+    m.c_mus = float(c.mu_s); m.c_mud = float(c.mu_s * c.phi_d);
+    m.c_kr = float(c.k_r); m.c_gr = float(c.gamma_r); m.c_mur = float(c.mu_r); m.c_mudr = float(c.mu_r * c.phi_r);
+    m.c_ko = float(c.k_o); m.c_go = float(c.gamma_o); m.c_muo = float(c.mu_o); m.c_mudo = float(c.mu_o * c.phi_o);
+    m.c_r = float(c.r_part); m.c_im = float(1.0 / c.mass); m.c_ii = float(1.0 / c.inertia);
+    m.c_r2 = 2.0 * c.r_part; m.c_dt = c.dt;
+    const hg_sinter_params &s = p.sinter;
+    m.s_k = float(s.k); m.s_gn = float(s.gamma_n); m.s_kt = float(s.k_t); m.s_gt = float(s.gamma_t);
+    m.s_kr = float(s.k_r); m.s_gr = float(s.gamma_r); m.s_ko = float(s.k_o); m.s_go = float(s.gamma_o);
+    m.s_r = float(s.r_part); m.s_im = float(1.0 / s.mass); m.s_ii = float(1.0 / s.inertia);
+    m.s_r2 = 2.0 * s.r_part; m.s_dt = s.dt;
+    const hg_hamaker_params &h = p.hamaker;
+    m.h_a6 = float(-h.A / 6.0); m.h_r = float(h.r_part); m.h_im = float(1.0 / h.mass);
+    m.h_r2 = 2.0 * h.r_part; m.h_h0 = h.h0; m.h_cut = h.r_cutoff;
+    const hg_contact_params &sc = p.surface_contact;
+    m.sc_k = float(sc.k); m.sc_gn = float(sc.gamma_n); m.sc_kt = float(sc.k_t); m.sc_gt = float(sc.gamma_t);
+    m.sc_mus = float(sc.mu_s); m.sc_mud = float(sc.mu_s * sc.phi_d);
+    m.sc_kr = float(sc.k_r); m.sc_gr = float(sc.gamma_r); m.sc_mur = float(sc.mu_r); m.sc_mudr = float(sc.mu_r * sc.phi_r);
+    m.sc_ko = float(sc.k_o); m.sc_go = float(sc.gamma_o); m.sc_muo = float(sc.mu_o); m.sc_mudo = float(sc.mu_o * sc.phi_o);
+    m.sc_r = float(sc.r_part); m.sc_im = float(1.0 / sc.mass); m.sc_ii = float(1.0 / sc.inertia);
+    m.sc_rd = sc.r_part; m.sc_dt = sc.dt;
+    const hg_hamaker_params &sh = p.surface_hamaker;
+    m.sh_a23 = float(2.0 * sh.A / 3.0); m.sh_im = float(1.0 / sh.mass);
+    m.sh_r = sh.r_part; m.sh_h0 = sh.h0;
+    return m;
+}
+
+// Rotates a spring into the plane normal to n and restores its length (FP64).
+__device__ __forceinline__ d3 rotate_spring(d3 xi, f3 nf) {
+    const d3 n = tod(nf);
+    d3 xr = xi - dot(xi, n) * n;
+    const double l2 = dot(xr, xr);
+    if (l2 > 0.0) xr = xr * sqrt(dot(xi, xi) / l2);
+    return xr;
+}
+
+// contact_force.h:105-147 with FP32 forces and an FP64 spring.
+__device__ __forceinline__ f3 shear_contact_mx(d3 &xi, f3 n, float k, float g, float fn, float mus, float mud, f3 rv,
+                                               double dt) {
+#pragma clang fp contract(fast)
+    xi = rotate_spring(xi, n);
+    const f3 f0 = (-k) * tof(xi) - g * rv;
+    const float sf = mus * fn, df = mud * fn;
+    const float f0n = sqrtf(dotf(f0, f0));
+    const f3 t = f0n > 0.f ? f0 * (1.0f / f0n) : zerof();
+    float fsel;
+    if (f0n <= sf) {
+        xi = xi + tod(rv) * dt;
+        fsel = sf;
+    } else {
+        xi = tod((-1.0f / k) * (df * t + g * rv));
+        fsel = df;
+    }
+    return ((f0n < fsel) ? f0n : fsel) * t;
+}
+
+// alt_sinter_bridge.h:137-156 with FP32 forces and an FP64 spring.
+__device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, f3 n, float k, float g, f3 rv, double dt) {
+#pragma clang fp contract(fast)
+    xi = rotate_spring(xi, n);
+    const f3 f0 = (-k) * tof(xi) - g * rv;
+    xi = xi + tod(rv) * dt;
+    return f0;
+}
+
+// Shared kinematics of contact and bonded pairs: n from the FP64 distance.
+__device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, f3 &n) {
+    const d3 d = xi - xj;
+    dist = sqrt(dot(d, d));
+    const double inv = dist > 0.0 ? 1.0 / dist : 0.0;
+    n = tof(d * inv);
+}
+
+template <bool BONDED>
+__device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, double dist, f3 n, f3 vi, f3 vj, f3 wi,
+                                                     f3 wj, spring9 &sp, bool &active) {
+#pragma clang fp contract(fast)
+    const double ovd = (BONDED ? m.s_r2 : m.c_r2) - dist;
+    if (!BONDED && ovd <= 0.0) {
+        active = false;
+        sp = zero_spring();
+        return {zerof(), zerof()};
+    }
+    active = true;
+    const float ov = float(ovd);
+    const float r = BONDED ? m.s_r : m.c_r;
+    const float rpp = r - 0.5f * ov;
+    const f3 dv = vi - vj;
+    const float v_n = -dotf(dv, n);
+    const float f_n = (BONDED ? m.s_k : m.c_k) * ov + (BONDED ? m.s_gn : m.c_gn) * v_n;
+    const f3 cwi = crossf(n, wi), cwj = crossf(n, wj);
+    const f3 v_ij = dv + rpp * cwi + rpp * cwj;
+    const f3 v_t = v_ij - dotf(v_ij, n) * n;
+    const f3 v_r = (-rpp) * (cwi - cwj);
+    const f3 v_o = (r * (dotf(n, wi) - dotf(n, wj))) * n;
+    f3 f_t, f_r, f_o;
+    if (BONDED) {
+        f_t = shear_sinter_mx(sp.s[0], n, m.s_kt, m.s_gt, v_t, m.s_dt);
+        f_r = shear_sinter_mx(sp.s[1], n, m.s_kr, m.s_gr, v_r, m.s_dt);
+        f_o = shear_sinter_mx(sp.s[2], n, m.s_ko, m.s_go, v_o, m.s_dt);
+    } else {
+        f_t = shear_contact_mx(sp.s[0], n, m.c_kt, m.c_gt, f_n, m.c_mus, m.c_mud, v_t, m.c_dt);
+        f_r = shear_contact_mx(sp.s[1], n, m.c_kr, m.c_gr, f_n, m.c_mur, m.c_mudr, v_r, m.c_dt);
+        f_o = shear_contact_mx(sp.s[2], n, m.c_ko, m.c_go, f_n, m.c_muo, m.c_mudo, v_o, m.c_dt);
+    }
+    const f3 tau = (-rpp) * crossf(n, f_t) + r * crossf(n, f_r) + r * f_o;
+    return {(f_n * n + f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
+}
+
+// Hamaker (full or limited range) with an FP64 gap.
+template <int HAM>
+__device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 n) {
+#pragma clang fp contract(fast)
+    if (HAM == HG_HAMAKER_LIMITED && dist > m.h_cut) return zerof();
+    double hd = dist - m.h_r2;
+    if (hd < m.h_h0) hd = m.h_h0;
+    const float h = float(hd), rp = m.h_r;
+    const float a = 4.0f * rp + h, b = 2.0f * rp + h;
+    const float ia = 1.0f / a, ib = 1.0f / b, ih = 1.0f / h;
+    const float s = m.h_a6 * ((a + h) * ia * ih - 2.0f * ib - 4.0f * rp * rp * ib * ib * ib -
+                              2.0f * rp * rp * (a + h) * ia * ia * ih * ih);
+    // the reference's direction is normalized(x_j - x_i) = -n
+    return (-s * m.h_im) * n;
+}
+
+__device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf, f3 vf, d3 xi, f3 vi, f3 wi,
+                                                   spring9 &sp) {
+#pragma clang fp contract(fast)
+    const d3 d = xi - xf;
+    const double dist = sqrt(dot(d, d));
+    const double ovd = m.sc_rd - dist;
+    if (ovd <= 0.0) {
+        sp = zero_spring();
+        return {zerof(), zerof()};
+    }
+    const f3 n = tof(d * (1.0 / dist));
+    const float ov = float(ovd), r = m.sc_r;
+    const float rpp = r - ov;
+    const f3 dv = vi - vf;
+    const float v_n = -dotf(dv, n);
+    const float f_n = m.sc_k * ov + m.sc_gn * v_n;
+    const f3 cw = crossf(n, wi);
+    const f3 v_ij = dv + rpp * cw;
+    const f3 v_t = v_ij - dotf(v_ij, n) * n;
+    const f3 v_r = (-rpp) * cw;
+    const f3 v_o = (r * dotf(n, wi)) * n;
+    const f3 f_t = shear_contact_mx(sp.s[0], n, m.sc_kt, m.sc_gt, f_n, m.sc_mus, m.sc_mud, v_t, m.sc_dt);
+    const f3 f_r = shear_contact_mx(sp.s[1], n, m.sc_kr, m.sc_gr, f_n, m.sc_mur, m.sc_mudr, v_r, m.sc_dt);
+    const f3 f_o = shear_contact_mx(sp.s[2], n, m.sc_ko, m.sc_go, f_n, m.sc_muo, m.sc_mudo, v_o, m.sc_dt);
+    const f3 tau = (-rpp) * crossf(n, f_t) + r * crossf(n, f_r) + r * f_o;
+    return {(f_n * n + f_t) * m.sc_im, tau * m.sc_ii};
+}
+
+__device__ __forceinline__ f3 surface_hamaker_mx(const mixed_consts &m, d3 xf, d3 xi) {
+#pragma clang fp contract(fast)
+    const d3 r = xf - xi;
+    const double dist = sqrt(dot(r, r));
+    double hd = dist - m.sh_r;
+    if (hd < m.sh_h0) hd = m.sh_h0;
+    const float h = float(hd), rp = float(m.sh_r);
+    const float q = h + 2.0f * rp;
+    const float s = m.sh_a23 * (rp * rp * rp) / (h * h * q * q);
+    return (s * m.sh_im) * tof(r * (1.0 / dist));
+}
+
+}  // namespace hg

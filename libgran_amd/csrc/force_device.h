@@ -1,0 +1,208 @@
+// Device building blocks of the acceleration pass shared by the single-system
+// kernels (kernels.hip) and the pool kernels (pool.hip): particle loads, the
+// pair-state table walker, the binary functor fold and the facet fold.
+//
+// All arrays are "transposed" with an explicit stride: slot s of particle i
+// lives at s*stride + i, so consecutive lanes (consecutive i) touch
+// consecutive words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dem_mixed.h"
+#include "hg_internal.h"
+
+namespace hg {
+
+__device__ __forceinline__ d3 ld3(const double *base, uint32_t i) {
+    const double4 q = reinterpret_cast<const double4 *>(base)[i];
+    return d3{q.x, q.y, q.z};
+}
+__device__ __forceinline__ void st3(double *base, uint32_t i, d3 a) {
+    reinterpret_cast<double4 *>(base)[i] = double4{a.x, a.y, a.z, 0.0};
+}
+
+__device__ __forceinline__ spring9 ld_spring(const double *b, size_t stride) {
+    return spring9{{d3{b[0], b[stride], b[2 * stride]}, d3{b[3 * stride], b[4 * stride], b[5 * stride]},
+                    d3{b[6 * stride], b[7 * stride], b[8 * stride]}}};
+}
+__device__ __forceinline__ void st_spring(double *b, size_t stride, const spring9 &s) {
+    b[0] = s.s[0].x; b[stride] = s.s[0].y; b[2 * stride] = s.s[0].z;
+    b[3 * stride] = s.s[1].x; b[4 * stride] = s.s[1].y; b[5 * stride] = s.s[1].z;
+    b[6 * stride] = s.s[2].x; b[7 * stride] = s.s[2].y; b[8 * stride] = s.s[2].z;
+}
+
+// Walks one particle's old pair-state table in ascending j alongside the
+// partner loop, writing the new table. Entries not visited this pass (pairs
+// that are not listed) are copied unchanged: the reference leaves the dense
+// spring table untouched for pairs it does not evaluate.
+struct walker {
+    const uint32_t *pj_old;
+    const double *ps_old;
+    uint32_t *pj_new;
+    double *ps_new;
+    size_t stride;
+    uint32_t i, cnt_old, p, q, cap, ej;
+
+    __device__ void init(const pair_table &T, uint32_t i_) {
+        pj_old = T.pj_old;
+        ps_old = T.ps_old;
+        pj_new = T.pj_new;
+        ps_new = T.ps_new;
+        stride = T.stride;
+        i = i_;
+        cap = T.cap;
+        cnt_old = T.pcnt_old[i];
+        p = 0;
+        q = 0;
+        ej = cnt_old > 0 ? pj_old[i] : kNoEntry;
+    }
+    __device__ void next() {
+        p++;
+        ej = p < cnt_old ? pj_old[size_t(p) * stride + i] : kNoEntry;
+    }
+    __device__ spring9 load() const { return ld_spring(ps_old + size_t(p) * 9 * stride + i, stride); }
+    __device__ void store(uint32_t tag, const spring9 &s) {
+        if (q < cap) {
+            pj_new[size_t(q) * stride + i] = tag;
+            st_spring(ps_new + size_t(q) * 9 * stride + i, stride, s);
+        }
+        q++;
+    }
+    __device__ void advance_to(uint32_t j) {
+        while ((ej & kIdxMask) < j) {
+            store(ej, load());
+            next();
+        }
+    }
+    __device__ bool take(uint32_t j, spring9 &s, bool &bonded) {
+        if ((ej & kIdxMask) == j) {
+            s = load();
+            bonded = (ej & kBondBit) != 0;
+            next();
+            return true;
+        }
+        return false;
+    }
+    __device__ void finish() {
+        while (p < cnt_old) {
+            store(ej, load());
+            next();
+        }
+    }
+};
+
+// One ordered pair through the binary container fold
+// (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
+template <int BIN, int HAM>
+__device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
+                                          walker &W, acc6 &acc) {
+    acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
+    if constexpr (BIN != HG_BINARY_NONE) {
+        W.advance_to(j);
+        spring9 sp = zero_spring();
+        bool bonded = false;
+        W.take(j, sp, bonded);
+        bool keep;
+        if (BIN == HG_BINARY_SINTER && bonded) {
+            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp);
+            keep = true;
+        } else {
+            b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
+        }
+        if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
+    }
+    if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
+    if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
+    acc6 e;
+    if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) e = b + h;
+    else if constexpr (BIN != HG_BINARY_NONE) e = b;
+    else e = h;
+    acc.lin = acc.lin + e.lin;
+    acc.ang = acc.ang + e.ang;
+}
+
+// Unary container over facets (unary_force_container.h:20-32, a right fold),
+// each facet folding its surface functors (triangular_facet.h:32-51). Surface
+// springs of facet f for particle i at ss + f*9*stride + i.
+__device__ __forceinline__ acc6 facet_fold(const hg_params &P, const facet_dev *facets, int nf, double *ss,
+                                           size_t stride, uint32_t i, d3 xi, d3 vi, d3 wi) {
+    acc6 s{zero3(), zero3()};
+    for (int f = nf - 1; f >= 0; f--) {
+        const facet_dev F = facets[f];
+        d3 xf = closest_point_on_triangle(F, xi);
+        acc6 c{zero3(), zero3()}, h{zero3(), zero3()};
+        if (P.facet_contact) {
+            double *b = ss + size_t(f) * 9 * stride + i;
+            spring9 sp = ld_spring(b, stride);
+            c = surface_contact_eval(P.surface_contact, xf, F.v_facet, xi, vi, wi, sp);
+            st_spring(b, stride, sp);
+        }
+        if (P.facet_hamaker) h = surface_hamaker_eval(P.surface_hamaker, xf, xi);
+        acc6 e = (P.facet_contact && P.facet_hamaker) ? c + h : (P.facet_contact ? c : h);
+        s = (f == nf - 1) ? e : e + s;
+    }
+    return s;
+}
+
+// Mixed-precision fold of one ordered pair (dem_mixed.h); the spring walk is
+// the FP64 one.
+template <int BIN, int HAM>
+__device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, f3 vi, f3 wi, d3 xj, f3 vj,
+                                             f3 wj, walker &W, accf &acc) {
+    double dist;
+    f3 n;
+    pair_kinematics_mx(xi, xj, dist, n);
+    accf b{zerof(), zerof()};
+    f3 h = zerof();
+    if constexpr (BIN != HG_BINARY_NONE) {
+        W.advance_to(j);
+        spring9 sp = zero_spring();
+        bool bonded = false;
+        W.take(j, sp, bonded);
+        bool keep;
+        if (BIN == HG_BINARY_SINTER && bonded)
+            b = bonded_or_contact_mx<true>(m, dist, n, vi, vj, wi, wj, sp, keep);
+        else
+            b = bonded_or_contact_mx<false>(m, dist, n, vi, vj, wi, wj, sp, keep);
+        if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
+    }
+    if constexpr (HAM != HG_HAMAKER_NONE) h = hamaker_mx<HAM>(m, dist, n);
+    acc.lin = acc.lin + (b.lin + h);
+    acc.ang = acc.ang + b.ang;
+}
+
+__device__ __forceinline__ accf facet_fold_mx(const hg_params &P, const mixed_consts &m, const facet_dev *facets,
+                                              int nf, double *ss, size_t stride, uint32_t i, d3 xi, f3 vi, f3 wi) {
+    accf s{zerof(), zerof()};
+    for (int f = nf - 1; f >= 0; f--) {
+        const facet_dev F = facets[f];
+        const d3 xf = closest_point_on_triangle(F, xi);
+        if (P.facet_contact) {
+            double *b = ss + size_t(f) * 9 * stride + i;
+            spring9 sp = ld_spring(b, stride);
+            s = s + surface_contact_mx(m, xf, tof(F.v_facet), xi, vi, wi, sp);
+            st_spring(b, stride, sp);
+        }
+        if (P.facet_hamaker) s.lin = s.lin + surface_hamaker_mx(m, xf, xi);
+    }
+    return s;
+}
+
+template <int BIN>
+__device__ __forceinline__ void close_table(walker &W, uint32_t *pcnt_new, uint32_t i, int *flags) {
+    if constexpr (BIN != HG_BINARY_NONE) {
+        W.finish();
+        pcnt_new[i] = W.q < W.cap ? W.q : W.cap;
+        if (W.q > W.cap) atomicOr(&flags[FLAG_PAIR_OVERFLOW], 1);
+        atomicMax(&flags[FLAG_PAIR_MAXCOUNT], int(W.q));
+    }
+}
+
+// Facet vertices advance by v_facet*dt (triangular_facet.h:62-66).
+__device__ __forceinline__ void advance_facets(facet_dev *facets, int nf, double dt) {
+    for (int f = 0; f < nf; f++)
+        for (int k = 0; k < 3; k++) facets[f].vert[k] = facets[f].vert[k] + facets[f].v_facet * dt;
+}
+
+}  // namespace hg

@@ -24,6 +24,19 @@ enum : int { FLAG_NL_OVERFLOW = 0, FLAG_PAIR_OVERFLOW = 1, FLAG_NL_MAXCOUNT = 2,
 
 constexpr int kMaxFacets = 16;
 
+// Pair-state table, old (read) and new (written) halves; slot s of particle i
+// at s*stride + i (spring component c at (s*9 + c)*stride + i).
+struct pair_table {
+    const uint32_t *pj_old;
+    const double *ps_old;
+    const uint32_t *pcnt_old;
+    uint32_t *pj_new;
+    double *ps_new;
+    uint32_t *pcnt_new;
+    uint32_t cap;
+    size_t stride;
+};
+
 // Everything a force/integrate kernel needs, passed by value.
 struct sys_view {
     uint32_t n;
@@ -32,14 +45,7 @@ struct sys_view {
     // neighbour list (transposed: slot k of particle i at k*n + i)
     const uint32_t *nb;
     const uint32_t *ncnt;
-    // pair-state table, old (read) and new (written) halves
-    const uint32_t *pj_old;
-    const double *ps_old;
-    const uint32_t *pcnt_old;
-    uint32_t *pj_new;
-    double *ps_new;
-    uint32_t *pcnt_new;
-    uint32_t cap_s;
+    pair_table pt;
     // facets
     const facet_dev *facets;
     double *ss;  // surface springs: (f*9 + c)*n + i
@@ -69,7 +75,7 @@ struct hg_system_impl {
     // cell-list scratch
     uint32_t hash_size = 0;
     uint32_t *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr, *bucket_fill = nullptr,
-             *cell_items = nullptr;
+             *cell_items = nullptr, *scan_tmp = nullptr;
     double *bbox = nullptr;  // lo[3], hi[3]
     // facets
     int nf = 0;
@@ -117,6 +123,9 @@ hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const 
 hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
                              facet_dev *facets, int nf, double facet_dt, hipStream_t s);
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
+// Exclusive scan of count words; tmp holds scan_tmp_words(count) words.
+hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t count, uint32_t *tmp, hipStream_t s);
+uint32_t scan_tmp_words(uint32_t count);
 hipError_t launch_zero_u32(uint32_t *p, size_t count, hipStream_t s);
 hipError_t launch_zero_f64(double *p, size_t count, hipStream_t s);
 hipError_t launch_diagnostics(const double *x, const double *v, const double *w, uint32_t n, double mass,

@@ -15,153 +15,13 @@
 //     so spring state survives neighbour rebuilds with no separate merge.
 #include <hip/hip_runtime.h>
 
+#include "force_device.h"
 #include "hg_internal.h"
 
 namespace hg {
 
 namespace {
 
-__device__ __forceinline__ d3 ld3(const double *base, uint32_t i) {
-    const double4 q = reinterpret_cast<const double4 *>(base)[i];
-    return d3{q.x, q.y, q.z};
-}
-__device__ __forceinline__ void st3(double *base, uint32_t i, d3 a) {
-    reinterpret_cast<double4 *>(base)[i] = double4{a.x, a.y, a.z, 0.0};
-}
-
-// Walks one particle's old pair-state table in ascending j alongside the
-// partner loop, writing the new table. Entries not visited this pass (pairs
-// that are not listed) are copied unchanged: the reference leaves the dense
-// spring table untouched for pairs it does not evaluate.
-struct walker {
-    const uint32_t *pj_old;
-    const double *ps_old;
-    uint32_t *pj_new;
-    double *ps_new;
-    size_t n;
-    uint32_t i, cnt_old, p, q, cap, ej;
-
-    __device__ void init(const sys_view &S, uint32_t i_) {
-        pj_old = S.pj_old;
-        ps_old = S.ps_old;
-        pj_new = S.pj_new;
-        ps_new = S.ps_new;
-        n = S.n;
-        i = i_;
-        cap = S.cap_s;
-        cnt_old = S.pcnt_old[i];
-        p = 0;
-        q = 0;
-        ej = cnt_old > 0 ? pj_old[i] : kNoEntry;
-    }
-    __device__ void next() {
-        p++;
-        ej = p < cnt_old ? pj_old[size_t(p) * n + i] : kNoEntry;
-    }
-    __device__ spring9 load() const {
-        spring9 s;
-        const double *b = ps_old + size_t(p) * 9 * n + i;
-        s.s[0] = d3{b[0], b[n], b[2 * n]};
-        s.s[1] = d3{b[3 * n], b[4 * n], b[5 * n]};
-        s.s[2] = d3{b[6 * n], b[7 * n], b[8 * n]};
-        return s;
-    }
-    __device__ void store(uint32_t tag, const spring9 &s) {
-        if (q < cap) {
-            pj_new[size_t(q) * n + i] = tag;
-            double *b = ps_new + size_t(q) * 9 * n + i;
-            b[0] = s.s[0].x; b[n] = s.s[0].y; b[2 * n] = s.s[0].z;
-            b[3 * n] = s.s[1].x; b[4 * n] = s.s[1].y; b[5 * n] = s.s[1].z;
-            b[6 * n] = s.s[2].x; b[7 * n] = s.s[2].y; b[8 * n] = s.s[2].z;
-        }
-        q++;
-    }
-    __device__ void advance_to(uint32_t j) {
-        while ((ej & kIdxMask) < j) {
-            store(ej, load());
-            next();
-        }
-    }
-    __device__ bool take(uint32_t j, spring9 &s, bool &bonded) {
-        if ((ej & kIdxMask) == j) {
-            s = load();
-            bonded = (ej & kBondBit) != 0;
-            next();
-            return true;
-        }
-        return false;
-    }
-    __device__ void finish() {
-        while (p < cnt_old) {
-            store(ej, load());
-            next();
-        }
-    }
-};
-
-// One ordered pair through the binary container fold
-// (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
-template <int BIN, int HAM>
-__device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
-                                          walker &W, acc6 &acc) {
-    acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
-    if constexpr (BIN != HG_BINARY_NONE) {
-        W.advance_to(j);
-        spring9 sp = zero_spring();
-        bool bonded = false;
-        W.take(j, sp, bonded);
-        bool keep;
-        if (BIN == HG_BINARY_SINTER && bonded) {
-            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp);
-            keep = true;
-        } else {
-            b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
-        }
-        if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
-    }
-    if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
-    if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
-    acc6 e;
-    if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) e = b + h;
-    else if constexpr (BIN != HG_BINARY_NONE) e = b;
-    else e = h;
-    acc.lin = acc.lin + e.lin;
-    acc.ang = acc.ang + e.ang;
-}
-
-// Unary container over facets (unary_force_container.h:20-32, a right fold),
-// each facet folding its surface functors (triangular_facet.h:32-51).
-__device__ __forceinline__ acc6 facet_fold(const hg_params &P, const sys_view &S, uint32_t i, d3 xi, d3 vi, d3 wi) {
-    acc6 s{zero3(), zero3()};
-    for (int f = S.nf - 1; f >= 0; f--) {
-        const facet_dev F = S.facets[f];
-        d3 xf = closest_point_on_triangle(F, xi);
-        acc6 c{zero3(), zero3()}, h{zero3(), zero3()};
-        if (P.facet_contact) {
-            double *b = S.ss + size_t(f) * 9 * S.n + i;
-            const size_t n = S.n;
-            spring9 sp{{d3{b[0], b[n], b[2 * n]}, d3{b[3 * n], b[4 * n], b[5 * n]}, d3{b[6 * n], b[7 * n], b[8 * n]}}};
-            c = surface_contact_eval(P.surface_contact, xf, F.v_facet, xi, vi, wi, sp);
-            b[0] = sp.s[0].x; b[n] = sp.s[0].y; b[2 * n] = sp.s[0].z;
-            b[3 * n] = sp.s[1].x; b[4 * n] = sp.s[1].y; b[5 * n] = sp.s[1].z;
-            b[6 * n] = sp.s[2].x; b[7 * n] = sp.s[2].y; b[8 * n] = sp.s[2].z;
-        }
-        if (P.facet_hamaker) h = surface_hamaker_eval(P.surface_hamaker, xf, xi);
-        acc6 e = (P.facet_contact && P.facet_hamaker) ? c + h : (P.facet_contact ? c : h);
-        s = (f == S.nf - 1) ? e : e + s;
-    }
-    return s;
-}
-
-template <int BIN>
-__device__ __forceinline__ void close_table(const sys_view &S, uint32_t i, walker &W) {
-    if constexpr (BIN != HG_BINARY_NONE) {
-        W.finish();
-        S.pcnt_new[i] = W.q < W.cap ? W.q : W.cap;
-        if (W.q > W.cap) atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
-        atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(W.q));
-    }
-}
 
 // Neighbour-list acceleration pass: one lane per particle i, partners in the
 // list's ascending order, so the per-particle sum has the reference's order.
@@ -172,17 +32,39 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     acc6 acc{zero3(), zero3()};
     walker W;
-    if constexpr (BIN != HG_BINARY_NONE) W.init(S, i);
+    if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     const uint32_t nn = S.ncnt[i];
     const uint32_t *nbi = S.nb + i;
     for (uint32_t k = 0; k < nn; k++) {
         const uint32_t j = nbi[size_t(k) * S.n];
         pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(S.x, j), ld3(S.v, j), ld3(S.w, j), W, acc);
     }
-    close_table<BIN>(S, i, W);
-    if (S.nf > 0) acc = acc + facet_fold(P, S, i, xi, vi, wi);
+    close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
+    if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
     st3(S.al, i, acc.ang);
+}
+
+// Mixed-precision neighbour-list pass (HG_PRECISION_MIXED, dem_mixed.h).
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts MC, sys_view S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const d3 xi = ld3(S.x, i);
+    const f3 vi = tof(ld3(S.v, i)), wi = tof(ld3(S.w, i));
+    accf acc{zerof(), zerof()};
+    walker W;
+    if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
+    const uint32_t nn = S.ncnt[i];
+    const uint32_t *nbi = S.nb + i;
+    for (uint32_t k = 0; k < nn; k++) {
+        const uint32_t j = nbi[size_t(k) * S.n];
+        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(S.x, j), tof(ld3(S.v, j)), tof(ld3(S.w, j)), W, acc);
+    }
+    close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
+    if (S.nf > 0) acc = acc + facet_fold_mx(P, MC, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
+    st3(S.a, i, tod(acc.lin));
+    st3(S.al, i, tod(acc.ang));
 }
 
 // All-pairs acceleration pass (granular_system): every j != i in ascending
@@ -203,7 +85,7 @@ __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
     acc6 acc{zero3(), zero3()};
     walker W;
     if constexpr (BIN != HG_BINARY_NONE) {
-        if (live) W.init(S, i);
+        if (live) W.init(S.pt, i);
     }
     for (uint32_t base = 0; base < S.n; base += kTile) {
         __syncthreads();
@@ -226,8 +108,8 @@ __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
         }
     }
     if (!live) return;
-    close_table<BIN>(S, i, W);
-    if (S.nf > 0) acc = acc + facet_fold(P, S, i, xi, vi, wi);
+    close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
+    if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
     st3(S.al, i, acc.ang);
 }
@@ -240,10 +122,7 @@ __global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double
                                                     double half, double dt, facet_dev *facets, int nf,
                                                     double facet_dt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pending && i == 0) {
-        for (int f = 0; f < nf; f++)
-            for (int k = 0; k < 3; k++) facets[f].vert[k] = facets[f].vert[k] + facets[f].v_facet * facet_dt;
-    }
+    if (pending && i == 0) advance_facets(facets, nf, facet_dt);
     if (i >= n) return;
     d3 vi = ld3(v, i), wi = ld3(w, i);
     const d3 ai = ld3(a, i), ali = ld3(al, i);
@@ -263,10 +142,7 @@ __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const 
                                                     uint32_t n, double half, facet_dev *facets, int nf,
                                                     double facet_dt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        for (int f = 0; f < nf; f++)
-            for (int k = 0; k < 3; k++) facets[f].vert[k] = facets[f].vert[k] + facets[f].v_facet * facet_dt;
-    }
+    if (i == 0) advance_facets(facets, nf, facet_dt);
     if (i >= n) return;
     st3(v, i, ld3(v, i) + ld3(a, i) * half);
     st3(w, i, ld3(w, i) + ld3(al, i) * half);
@@ -333,26 +209,61 @@ __global__ void __launch_bounds__(256) k_cell_count(const double *x, uint32_t n,
     atomicAdd(&bucket_count[h], 1u);
 }
 
-// Single-block exclusive scan of `count` words.
-__global__ void __launch_bounds__(1024) k_scan(const uint32_t *in, uint32_t *out, uint32_t count) {
-    __shared__ uint32_t sums[1024];
-    const uint32_t per = (count + blockDim.x - 1) / blockDim.x;
-    const uint32_t b = threadIdx.x * per, e = min(count, b + per);
-    uint32_t s = 0;
-    for (uint32_t k = b; k < e; k++) s += in[k];
-    sums[threadIdx.x] = s;
+// Exclusive scan in three launches: per-block scan of kScanChunk words with
+// block totals, a single-block scan of the totals, then the offsets added.
+constexpr uint32_t kScanThreads = 256, kScanPer = 8, kScanChunk = kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t &total) {
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
-        uint32_t add = threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+        uint32_t add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
         __syncthreads();
-        sums[threadIdx.x] += add;
+        sh[threadIdx.x] += add;
         __syncthreads();
     }
-    uint32_t run = sums[threadIdx.x] - s;
-    for (uint32_t k = b; k < e; k++) {
-        out[k] = run;
-        run += in[k];
+    total = sh[blockDim.x - 1];
+    uint32_t r = sh[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_blocks(const uint32_t *in, uint32_t *out, uint32_t count,
+                                                               uint32_t *block_sums) {
+    __shared__ uint32_t sh[kScanThreads];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    uint32_t vals[kScanPer];
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        vals[k] = base + k < count ? in[base + k] : 0u;
+        s += vals[k];
     }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, sh, total);
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        if (base + k < count) out[base + k] = run;
+        run += vals[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_top(uint32_t *block_sums, uint32_t nblocks) {
+    __shared__ uint32_t sh[kScanThreads];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nblocks; base += kScanThreads) {
+        const uint32_t idx = base + threadIdx.x;
+        const uint32_t v = idx < nblocks ? block_sums[idx] : 0u;
+        uint32_t total;
+        const uint32_t r = block_exclusive_scan(v, sh, total);
+        if (idx < nblocks) block_sums[idx] = r + carry;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t *out, uint32_t count, const uint32_t *block_sums) {
+    const uint32_t add = block_sums[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanChunk;
+    for (uint32_t k = threadIdx.x; k < kScanChunk && base + k < count; k += blockDim.x) out[base + k] += add;
 }
 
 __global__ void __launch_bounds__(256) k_cell_fill(uint32_t n, const uint32_t *cell_of, const uint32_t *bucket_start,
@@ -476,7 +387,9 @@ inline unsigned blocks_for(size_t n, unsigned bs) { return unsigned((n + bs - 1)
 template <int BIN, int HAM>
 hipError_t launch_force_t(const hg_params &p, const sys_view &sv, hipStream_t s) {
     if (sv.n == 0) return hipSuccess;
-    if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
+    if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST && p.precision == HG_PRECISION_MIXED) {
+        k_force_nl_mx<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, make_mixed_consts(p), sv);
+    } else if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
         // 64-lane blocks spread a modest particle count over more CUs.
         k_force_nl<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
     } else {
@@ -533,6 +446,17 @@ hipError_t launch_zero_f64(double *p, size_t count, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t count, uint32_t *tmp, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const uint32_t nblocks = (count + kScanChunk - 1) / kScanChunk;
+    k_scan_blocks<<<nblocks, kScanThreads, 0, s>>>(in, out, count, tmp);
+    k_scan_top<<<1, kScanThreads, 0, s>>>(tmp, nblocks);
+    k_scan_add<<<nblocks, kScanThreads, 0, s>>>(out, count, tmp);
+    return hipGetLastError();
+}
+
+uint32_t scan_tmp_words(uint32_t count) { return (count + kScanChunk - 1) / kScanChunk + 1; }
+
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s) {
     const uint32_t n = uint32_t(S.n);
     if (n == 0) return hipSuccess;
@@ -543,7 +467,8 @@ hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s) {
     if ((e = launch_zero_u32(S.bucket_fill, S.hash_size, s)) != hipSuccess) return e;
     k_bbox<<<1, 1024, 0, s>>>(S.x, n, S.bbox);
     k_cell_count<<<blocks_for(n, 256), 256, 0, s>>>(S.x, n, S.bbox, cs, mask, S.cell_of, S.bucket_count);
-    k_scan<<<1, 1024, 0, s>>>(S.bucket_count, S.bucket_start, S.hash_size);
+    if ((e = launch_exclusive_scan(S.bucket_count, S.bucket_start, S.hash_size, S.scan_tmp, s)) != hipSuccess)
+        return e;
     k_cell_fill<<<blocks_for(n, 256), 256, 0, s>>>(n, S.cell_of, S.bucket_start, S.bucket_fill, S.cell_items);
     k_nl_fill<<<blocks_for(n, 64), 64, 0, s>>>(S.x, n, S.bbox, cs, mask, S.p.r_verlet, S.bucket_count,
                                                S.bucket_start, S.cell_items, S.nb, S.ncnt, S.cap_n, S.flags);

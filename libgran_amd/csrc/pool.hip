@@ -1,0 +1,914 @@
+// Pools of independent DEM systems (include/hipgran.h, hg_pool_*): the
+// batched and serving path.
+//
+// All systems of a pool live in one device arena of P particles (P = the
+// pool's particle capacity); every per-particle array uses the same transposed
+// layout as the single-system path with stride P, so a system is a contiguous
+// index range [offset, offset + n) and a workgroup that owns it touches
+// consecutive words.
+//
+// hg_pool_step launches one persistent kernel per rebuild interval. Each
+// workgroup repeatedly takes the next system from a largest-first work queue
+// and runs all steps of the interval for it: kick/drift of its own particles,
+// workgroup barrier, acceleration pass (the same pair fold, walker and facet
+// fold as the single-system kernels), barrier. A system's arithmetic does not
+// depend on the other systems in the pool, so results are identical alone and
+// in any mix. Between intervals a batched hashed cell list rebuilds the
+// neighbour lists of the systems that are due, keyed by (system, cell).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "force_device.h"
+#include "hg_internal.h"
+
+namespace hg {
+
+struct pool_meta {
+    uint32_t offset, n, cur, acc_valid, nf, rebuild, pad0, pad1;
+};
+
+struct pool_view {
+    double *x, *v, *th, *w, *a, *al;
+    const uint32_t *nb;
+    const uint32_t *ncnt;
+    uint32_t *pj0, *pj1, *pcnt0, *pcnt1;
+    double *ps0, *ps1;
+    uint32_t cap_s;
+    facet_dev *facets;
+    uint32_t max_facets;
+    double *ss;
+    const pool_meta *meta;
+    size_t stride;
+    int *flags;
+};
+
+namespace {
+
+template <int BIN, int HAM, int PAIRS, int PREC>
+__device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
+                                                 const pool_meta &M, uint32_t cur, const facet_dev *F) {
+    pair_table T;
+    T.pj_old = cur ? V.pj1 : V.pj0;
+    T.ps_old = cur ? V.ps1 : V.ps0;
+    T.pcnt_old = cur ? V.pcnt1 : V.pcnt0;
+    T.pj_new = cur ? V.pj0 : V.pj1;
+    T.ps_new = cur ? V.ps0 : V.ps1;
+    T.pcnt_new = cur ? V.pcnt0 : V.pcnt1;
+    T.cap = V.cap_s;
+    T.stride = V.stride;
+    const uint32_t end = M.offset + M.n;
+    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+        const d3 xi = ld3(V.x, g), vi = ld3(V.v, g), wi = ld3(V.w, g);
+        walker W;
+        if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
+        if constexpr (PREC == HG_PRECISION_FP64) {
+            acc6 acc{zero3(), zero3()};
+            if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
+                const uint32_t nn = V.ncnt[g];
+                for (uint32_t k = 0; k < nn; k++) {
+                    const uint32_t j = V.nb[size_t(k) * V.stride + g];
+                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                }
+            } else {
+                for (uint32_t j = M.offset; j < end; j++) {
+                    if (j == g) continue;
+                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                }
+            }
+            close_table<BIN>(W, T.pcnt_new, g, V.flags);
+            if (M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
+            st3(V.a, g, acc.lin);
+            st3(V.al, g, acc.ang);
+        } else {
+            accf acc{zerof(), zerof()};
+            const f3 vf = tof(vi), wf = tof(wi);
+            if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
+                const uint32_t nn = V.ncnt[g];
+                for (uint32_t k = 0; k < nn; k++) {
+                    const uint32_t j = V.nb[size_t(k) * V.stride + g];
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vf, wf, ld3(V.x, j), tof(ld3(V.v, j)), tof(ld3(V.w, j)), W,
+                                           acc);
+                }
+            } else {
+                for (uint32_t j = M.offset; j < end; j++) {
+                    if (j == g) continue;
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vf, wf, ld3(V.x, j), tof(ld3(V.v, j)), tof(ld3(V.w, j)), W,
+                                           acc);
+                }
+            }
+            close_table<BIN>(W, T.pcnt_new, g, V.flags);
+            if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vf, wf);
+            st3(V.a, g, tod(acc.lin));
+            st3(V.al, g, tod(acc.ang));
+        }
+    }
+}
+
+// Kick (completing the previous step when `pending`), kick, drift; the same
+// operations in the same order as k_kick_drift.
+__device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_meta &M, bool pending, double half,
+                                                double dt) {
+    const uint32_t end = M.offset + M.n;
+    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+        d3 vi = ld3(V.v, g), wi = ld3(V.w, g);
+        const d3 ai = ld3(V.a, g), ali = ld3(V.al, g);
+        if (pending) {
+            vi = vi + ai * half;
+            wi = wi + ali * half;
+        }
+        vi = vi + ai * half;
+        wi = wi + ali * half;
+        st3(V.x, g, ld3(V.x, g) + vi * dt);
+        st3(V.th, g, ld3(V.th, g) + wi * dt);
+        st3(V.v, g, vi);
+        st3(V.w, g, wi);
+    }
+}
+
+__device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_meta &M, double half) {
+    const uint32_t end = M.offset + M.n;
+    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+        st3(V.v, g, ld3(V.v, g) + ld3(V.a, g) * half);
+        st3(V.w, g, ld3(V.w, g) + ld3(V.al, g) * half);
+    }
+}
+
+// One rebuild interval of every queued system. Exit: every workgroup leaves
+// once the shared counter passes `count`.
+template <int BIN, int HAM, int PAIRS, int PREC>
+__global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
+                                                    uint32_t count, uint32_t *counter, uint32_t nsteps, double dt) {
+    __shared__ uint32_t s_item;
+    const double half = dt / 2.0;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        __syncthreads();
+        if (item >= count) break;
+        const uint32_t slot = order[item];
+        const pool_meta M = V.meta[slot];
+        facet_dev *F = V.facets + size_t(slot) * V.max_facets;
+        uint32_t cur = M.cur;
+        if (!M.acc_valid) {
+            pool_force_phase<BIN, HAM, PAIRS, PREC>(P, MC, V, M, cur, F);
+            cur ^= 1u;
+            __syncthreads();
+        }
+        for (uint32_t s = 0; s < nsteps; s++) {
+            pool_kick_drift(V, M, s > 0, half, dt);
+            if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
+            __syncthreads();
+            pool_force_phase<BIN, HAM, PAIRS, PREC>(P, MC, V, M, cur, F);
+            cur ^= 1u;
+            __syncthreads();
+        }
+        if (nsteps > 0) {
+            pool_final_kick(V, M, half);
+            if (threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------ batched rebuild -----
+__device__ __forceinline__ uint32_t pool_hash(uint32_t slot, long long cx, long long cy, long long cz, uint32_t mask) {
+    uint64_t h = uint64_t(cx) * 73856093ull ^ uint64_t(cy) * 19349663ull ^ uint64_t(cz) * 83492791ull ^
+                 uint64_t(slot) * 2654435761ull;
+    h ^= h >> 29;
+    h *= 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 32;
+    return uint32_t(h) & mask;
+}
+
+__device__ __forceinline__ void pool_cell(d3 q, double cs, long long &cx, long long &cy, long long &cz) {
+    cx = (long long)floor(q.x / cs);
+    cy = (long long)floor(q.y / cs);
+    cz = (long long)floor(q.z / cs);
+}
+
+__global__ void __launch_bounds__(256) k_pool_cell_count(const double *x, const uint32_t *owner, const pool_meta *meta,
+                                                         uint32_t hw, double cs, uint32_t mask, uint32_t *cell_of,
+                                                         uint32_t *bucket_count) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= hw) return;
+    const uint32_t slot = owner[g];
+    if (slot == kNoEntry || !meta[slot].rebuild) {
+        cell_of[g] = kNoEntry;
+        return;
+    }
+    long long cx, cy, cz;
+    pool_cell(ld3(x, g), cs, cx, cy, cz);
+    const uint32_t h = pool_hash(slot, cx, cy, cz, mask);
+    cell_of[g] = h;
+    atomicAdd(&bucket_count[h], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_pool_cell_fill(uint32_t hw, const uint32_t *cell_of,
+                                                        const uint32_t *bucket_start, uint32_t *bucket_fill,
+                                                        uint32_t *cell_items) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= hw) return;
+    const uint32_t h = cell_of[g];
+    if (h == kNoEntry) return;
+    cell_items[bucket_start[h] + atomicAdd(&bucket_fill[h], 1u)] = g;
+}
+
+__global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uint32_t *owner, const uint32_t *cell_of,
+                                                      uint32_t hw, double cs, uint32_t mask, double r_verlet,
+                                                      const uint32_t *bucket_count, const uint32_t *bucket_start,
+                                                      const uint32_t *cell_items, uint32_t *nb, uint32_t *ncnt,
+                                                      size_t stride, uint32_t cap, int *flags) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= hw || cell_of[g] == kNoEntry) return;
+    const uint32_t slot = owner[g];
+    const d3 xi = ld3(x, g);
+    long long cx, cy, cz;
+    pool_cell(xi, cs, cx, cy, cz);
+    uint32_t seen[27];
+    int nseen = 0;
+    uint32_t cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const uint32_t h = pool_hash(slot, cx + dx, cy + dy, cz + dz, mask);
+                bool dup = false;
+                for (int s = 0; s < nseen; s++) dup |= seen[s] == h;
+                if (dup) continue;
+                seen[nseen++] = h;
+                const uint32_t b = bucket_start[h], e = b + bucket_count[h];
+                for (uint32_t q = b; q < e; q++) {
+                    const uint32_t j = cell_items[q];
+                    if (j == g || owner[j] != slot) continue;
+                    if (norm(xi - ld3(x, j)) < r_verlet) {
+                        if (cnt < cap) {
+                            uint32_t pos = cnt;
+                            while (pos > 0) {
+                                const uint32_t prev = nb[size_t(pos - 1) * stride + g];
+                                if (prev < j) break;
+                                nb[size_t(pos) * stride + g] = prev;
+                                pos--;
+                            }
+                            nb[size_t(pos) * stride + g] = j;
+                        }
+                        cnt++;
+                    }
+                }
+            }
+    ncnt[g] = cnt < cap ? cnt : cap;
+    if (cnt > cap) atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
+    atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+}
+
+// Zeroes `rows` strided rows [offset, offset+n) of an array with row stride.
+__global__ void k_zero_rows_f64(double *p, size_t stride, uint32_t offset, uint32_t n, uint32_t rows) {
+    const size_t total = size_t(rows) * n;
+    for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < total; k += size_t(gridDim.x) * blockDim.x)
+        p[(k / n) * stride + offset + (k % n)] = 0.0;
+}
+
+inline unsigned nblk(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
+
+template <int BIN, int HAM, int PAIRS>
+hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, hipStream_t s) {
+    const mixed_consts mc = make_mixed_consts(p);
+    if (p.precision == HG_PRECISION_MIXED)
+        k_pool_chunk<BIN, HAM, PAIRS, HG_PRECISION_MIXED><<<grid, 256, 0, s>>>(p, mc, V, order, count, counter, nsteps,
+                                                                              dt);
+    else
+        k_pool_chunk<BIN, HAM, PAIRS, HG_PRECISION_FP64><<<grid, 256, 0, s>>>(p, mc, V, order, count, counter, nsteps,
+                                                                             dt);
+    return hipGetLastError();
+}
+
+template <int PAIRS>
+hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, hipStream_t s) {
+#define HG_PCASE(B, H) \
+    if (p.binary_model == B && p.hamaker_model == H) \
+        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, s);
+    HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
+    HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
+    HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
+    HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
+    HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
+    HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
+    HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+#undef HG_PCASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+}  // namespace hg
+
+using namespace hg;
+
+struct hg_pool {
+    hg_params p{};
+    size_t P = 0, S = 0, MF = 0;
+    hipStream_t stream = nullptr;
+    double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
+    uint32_t cap_n = 0, *nb = nullptr, *ncnt = nullptr;
+    uint32_t cap_s = 0, *pj[2] = {nullptr, nullptr}, *pcnt[2] = {nullptr, nullptr};
+    double *ps[2] = {nullptr, nullptr};
+    uint32_t *owner = nullptr;
+    facet_dev *facets = nullptr;
+    double *ss = nullptr;
+    pool_meta *meta_d = nullptr;
+    std::vector<pool_meta> meta;
+    std::vector<uint64_t> step_index;
+    std::vector<double> t;
+    std::vector<char> active;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> bonds;
+    std::vector<size_t> skipped;
+    uint32_t *order_d = nullptr, *counter_d = nullptr;
+    std::vector<uint32_t> order;
+    bool order_dirty = true;
+    uint32_t hash_size = 0, *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr,
+             *bucket_fill = nullptr, *cell_items = nullptr, *scan_tmp = nullptr;
+    int *flags = nullptr, *flags_host = nullptr;
+    std::map<uint32_t, uint32_t> free_ranges;  // offset -> length
+    uint32_t high_water = 0;
+    std::vector<void *> allocations;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double last_chunk_ms = 0.0, last_rebuild_ms = 0.0;
+    uint64_t last_chunks = 0;
+    int grid = 0;
+};
+
+namespace {
+
+template <typename T>
+hg_status palloc(hg_pool *Q, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? HG_ERR_OOM : HG_ERR_HIP,
+                    "pool: hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+    Q->allocations.push_back(*p);
+    return HG_OK;
+}
+
+#define PG_TRY(expr)                  \
+    do {                              \
+        hg_status st_ = (expr);       \
+        if (st_ != HG_OK) return st_; \
+    } while (0)
+
+uint32_t pool_cap_n(const hg_params &p) {
+    if (p.pair_mode != HG_PAIRS_NEIGHBOR_LIST) return 0;
+    if (p.neighbor_capacity) return p.neighbor_capacity;
+    const double r = p.contact.r_part > 0 ? p.contact.r_part : (p.sinter.r_part > 0 ? p.sinter.r_part : p.hamaker.r_part);
+    double ratio = r > 0 ? p.r_verlet / r : 64.0;
+    double bound = 0.7405 * (std::pow(ratio + 1.0, 3.0) - 1.0) * 1.3;
+    return uint32_t(std::ceil(std::max(bound, 8.0) / 8.0) * 8.0);
+}
+
+uint32_t pool_cap_s(const hg_params &p) {
+    if (p.binary_model == HG_BINARY_NONE) return 0;
+    return p.pair_capacity ? p.pair_capacity : 24u;
+}
+
+hg_status check_pool_flags(hg_pool *Q, const char *where) {
+    HG_HIP(hipMemcpyAsync(Q->flags_host, Q->flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    if (Q->flags_host[FLAG_NL_OVERFLOW] || Q->flags_host[FLAG_PAIR_OVERFLOW]) {
+        const bool nl = Q->flags_host[FLAG_NL_OVERFLOW] != 0;
+        const int need = Q->flags_host[nl ? FLAG_NL_MAXCOUNT : FLAG_PAIR_MAXCOUNT];
+        HG_HIP(hipMemsetAsync(Q->flags, 0, FLAG_COUNT * sizeof(int), Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        return fail(HG_ERR_CAPACITY, std::string(where) + (nl ? ": neighbour list needs " : ": pair-state table needs ") +
+                                         std::to_string(need) + " slots per particle, capacity is " +
+                                         std::to_string(nl ? Q->cap_n : Q->cap_s));
+    }
+    return HG_OK;
+}
+
+pool_view make_pool_view(hg_pool *Q) {
+    pool_view V;
+    V.x = Q->x; V.v = Q->v; V.th = Q->th; V.w = Q->w; V.a = Q->a; V.al = Q->al;
+    V.nb = Q->nb;
+    V.ncnt = Q->ncnt;
+    V.pj0 = Q->pj[0]; V.pj1 = Q->pj[1];
+    V.pcnt0 = Q->pcnt[0]; V.pcnt1 = Q->pcnt[1];
+    V.ps0 = Q->ps[0]; V.ps1 = Q->ps[1];
+    V.cap_s = Q->cap_s;
+    V.facets = Q->facets;
+    V.max_facets = uint32_t(Q->MF);
+    V.ss = Q->ss;
+    V.meta = Q->meta_d;
+    V.stride = Q->P;
+    V.flags = Q->flags;
+    return V;
+}
+
+hg_status valid_slot(hg_pool *Q, int32_t slot) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (slot < 0 || size_t(slot) >= Q->S || !Q->active[slot]) return fail(HG_ERR_INVALID, "no such system in the pool");
+    return HG_OK;
+}
+
+hg_status upload_meta(hg_pool *Q) {
+    HG_HIP(hipMemcpyAsync(Q->meta_d, Q->meta.data(), Q->S * sizeof(pool_meta), hipMemcpyHostToDevice, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    return HG_OK;
+}
+
+// Copies strided rows [offset, offset+n) of a device array into a host array
+// with row stride n.
+template <typename T>
+hg_status download_rows(hg_pool *Q, const T *dev, uint32_t offset, uint32_t n, uint32_t rows, T *host) {
+    if (rows == 0 || n == 0) return HG_OK;
+    HG_HIP(hipMemcpy2DAsync(host, n * sizeof(T), dev + offset, Q->P * sizeof(T), n * sizeof(T), rows,
+                            hipMemcpyDeviceToHost, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    return HG_OK;
+}
+
+hg_status rebuild_due(hg_pool *Q) {
+    const uint32_t hw = Q->high_water;
+    if (hw == 0) return HG_OK;
+    const double cs = Q->p.r_verlet * (1.0 + 1e-9);
+    const uint32_t mask = Q->hash_size - 1;
+    HG_HIP(launch_zero_u32(Q->bucket_count, Q->hash_size, Q->stream));
+    HG_HIP(launch_zero_u32(Q->bucket_fill, Q->hash_size, Q->stream));
+    k_pool_cell_count<<<nblk(hw, 256), 256, 0, Q->stream>>>(Q->x, Q->owner, Q->meta_d, hw, cs, mask, Q->cell_of,
+                                                           Q->bucket_count);
+    HG_HIP(hipGetLastError());
+    HG_HIP(launch_exclusive_scan(Q->bucket_count, Q->bucket_start, Q->hash_size, Q->scan_tmp, Q->stream));
+    k_pool_cell_fill<<<nblk(hw, 256), 256, 0, Q->stream>>>(hw, Q->cell_of, Q->bucket_start, Q->bucket_fill,
+                                                          Q->cell_items);
+    HG_HIP(hipGetLastError());
+    k_pool_nl_fill<<<nblk(hw, 64), 64, 0, Q->stream>>>(Q->x, Q->owner, Q->cell_of, hw, cs, mask, Q->p.r_verlet,
+                                                      Q->bucket_count, Q->bucket_start, Q->cell_items, Q->nb,
+                                                      Q->ncnt, Q->P, Q->cap_n, Q->flags);
+    HG_HIP(hipGetLastError());
+    return HG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t hg_pool_bytes_per_particle(const hg_params *p, size_t max_facets) {
+    if (!p) return 0;
+    const size_t cap_n = pool_cap_n(*p), cap_s = pool_cap_s(*p);
+    size_t b = 6 * 32;                   // x, v, theta, omega, a, alpha
+    b += cap_n * 4 + 4;                  // neighbour slots + count
+    b += 2 * (cap_s * (4 + 72) + 4);     // pair-state table, double buffered
+    b += max_facets * 72;                // facet springs
+    b += 4;                              // owner
+    if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST) b += 4 + 4 + 3 * 2 * 4 + 4;  // cell list scratch
+    return b;
+}
+
+hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t max_systems, size_t max_facets,
+                         hg_pool **out) {
+    if (!out || !params) return fail(HG_ERR_INVALID, "hg_pool_create: params and out are required");
+    *out = nullptr;
+    if (max_particles == 0 || max_particles >= (size_t(1) << 31) || max_systems == 0)
+        return fail(HG_ERR_INVALID, "hg_pool_create: need 0 < max_particles < 2^31 and max_systems > 0");
+    if (max_facets > size_t(kMaxFacets)) return fail(HG_ERR_INVALID, "hg_pool_create: at most 16 facets per system");
+    if (params->precision != HG_PRECISION_FP64 && params->precision != HG_PRECISION_MIXED)
+        return fail(HG_ERR_INVALID, "hg_pool_create: unknown precision");
+    if (params->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(params->r_verlet > 0.0))
+        return fail(HG_ERR_INVALID, "hg_pool_create: neighbour-list mode needs r_verlet > 0");
+    int dev = 0;
+    HG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HG_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(HG_ERR_NO_DEVICE, std::string("hipgran is built for gfx950; current device is ") + prop.gcnArchName);
+
+    hg_pool *Q = new hg_pool();
+    auto bail = [&](hg_status st) {
+        std::string msg = hg_last_error();
+        hg_pool_destroy(Q);
+        fail(st, msg);
+        return st;
+    };
+#define PG_TRYC(expr)                           \
+    do {                                        \
+        hg_status st_ = (expr);                 \
+        if (st_ != HG_OK) return bail(st_);     \
+    } while (0)
+#define PG_HIPC(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return bail(fail(HG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    Q->p = *params;
+    Q->P = max_particles;
+    Q->S = max_systems;
+    Q->MF = max_facets;
+    Q->grid = prop.multiProcessorCount * 2;
+    PG_HIPC(hipStreamCreateWithFlags(&Q->stream, hipStreamNonBlocking));
+    for (auto &e : Q->ev) PG_HIPC(hipEventCreate(&e));
+    const size_t P = Q->P;
+    double **fields[6] = {&Q->x, &Q->v, &Q->th, &Q->w, &Q->a, &Q->al};
+    for (auto f : fields) PG_TRYC(palloc(Q, f, 4 * P));
+    Q->cap_n = pool_cap_n(Q->p);
+    Q->cap_s = pool_cap_s(Q->p);
+    if (Q->cap_n) {
+        PG_TRYC(palloc(Q, &Q->nb, size_t(Q->cap_n) * P));
+        PG_TRYC(palloc(Q, &Q->ncnt, P));
+        PG_HIPC(launch_zero_u32(Q->ncnt, P, Q->stream));
+        uint32_t hs = 1024;
+        while (hs < 2 * P && hs < (1u << 30)) hs <<= 1;
+        Q->hash_size = hs;
+        PG_TRYC(palloc(Q, &Q->cell_of, P));
+        PG_TRYC(palloc(Q, &Q->cell_items, P));
+        PG_TRYC(palloc(Q, &Q->bucket_count, hs));
+        PG_TRYC(palloc(Q, &Q->bucket_start, hs));
+        PG_TRYC(palloc(Q, &Q->bucket_fill, hs));
+        PG_TRYC(palloc(Q, &Q->scan_tmp, scan_tmp_words(hs)));
+    }
+    if (Q->cap_s) {
+        for (int b = 0; b < 2; b++) {
+            PG_TRYC(palloc(Q, &Q->pj[b], size_t(Q->cap_s) * P));
+            PG_TRYC(palloc(Q, &Q->ps[b], size_t(Q->cap_s) * 9 * P));
+            PG_TRYC(palloc(Q, &Q->pcnt[b], P));
+        }
+    }
+    PG_TRYC(palloc(Q, &Q->owner, P));
+    {
+        std::vector<uint32_t> none(P, kNoEntry);
+        PG_HIPC(hipMemcpy(Q->owner, none.data(), P * 4, hipMemcpyHostToDevice));
+    }
+    PG_TRYC(palloc(Q, &Q->facets, std::max<size_t>(1, Q->S * std::max<size_t>(Q->MF, 1))));
+    if (Q->MF) PG_TRYC(palloc(Q, &Q->ss, Q->MF * 9 * P));
+    PG_TRYC(palloc(Q, &Q->meta_d, Q->S));
+    PG_TRYC(palloc(Q, &Q->order_d, Q->S));
+    PG_TRYC(palloc(Q, &Q->counter_d, 1));
+    PG_TRYC(palloc(Q, &Q->flags, FLAG_COUNT));
+    PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->flags_host), FLAG_COUNT * sizeof(int)));
+    PG_HIPC(hipMemsetAsync(Q->flags, 0, FLAG_COUNT * sizeof(int), Q->stream));
+    Q->meta.assign(Q->S, pool_meta{0, 0, 0, 0, 0, 0, 0, 0});
+    Q->step_index.assign(Q->S, 0);
+    Q->t.assign(Q->S, 0.0);
+    Q->active.assign(Q->S, 0);
+    Q->bonds.assign(Q->S, {});
+    Q->skipped.assign(Q->S, 0);
+    Q->free_ranges[0] = uint32_t(P);
+    PG_TRYC(upload_meta(Q));
+    *out = Q;
+    return HG_OK;
+#undef PG_TRYC
+#undef PG_HIPC
+}
+
+hg_status hg_pool_destroy(hg_pool *Q) {
+    if (!Q) return HG_OK;
+    if (Q->stream) (void)hipStreamSynchronize(Q->stream);
+    for (void *p : Q->allocations) (void)hipFree(p);
+    for (auto e : Q->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (Q->flags_host) (void)hipHostFree(Q->flags_host);
+    if (Q->stream) (void)hipStreamDestroy(Q->stream);
+    delete Q;
+    return HG_OK;
+}
+
+hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, const double *theta0,
+                      const double *omega0, int32_t *slot_out) {
+    if (!Q || !x0 || !slot_out) return fail(HG_ERR_INVALID, "hg_pool_add: pool, x0 and slot are required");
+    if (n == 0) return fail(HG_ERR_INVALID, "hg_pool_add: empty system");
+    int32_t slot = -1;
+    for (size_t s = 0; s < Q->S; s++)
+        if (!Q->active[s]) {
+            slot = int32_t(s);
+            break;
+        }
+    if (slot < 0) return fail(HG_ERR_CAPACITY, "hg_pool_add: every system slot is in use");
+    auto it = Q->free_ranges.begin();
+    for (; it != Q->free_ranges.end(); ++it)
+        if (it->second >= n) break;
+    if (it == Q->free_ranges.end()) return fail(HG_ERR_CAPACITY, "hg_pool_add: no free particle range of that size");
+    const uint32_t offset = it->first;
+    // topology and the initial pair-state rows before committing anything
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped = 0;
+    std::vector<std::vector<uint32_t>> rows(n);
+    if (Q->p.binary_model == HG_BINARY_SINTER) {
+        sinter_topology_host(n, x0, Q->p.sinter.r_part, Q->p.sinter.d_crit, bonds, skipped);
+        for (auto &b : bonds) {
+            rows[b.first].push_back((offset + b.second) | kBondBit);
+            rows[b.second].push_back((offset + b.first) | kBondBit);
+        }
+        for (auto &r : rows) std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return (a & kIdxMask) < (b & kIdxMask); });
+        for (auto &r : rows)
+            if (r.size() > Q->cap_s)
+                return fail(HG_ERR_CAPACITY, "hg_pool_add: a particle has " + std::to_string(r.size()) +
+                                                 " bonds, pair capacity is " + std::to_string(Q->cap_s));
+    }
+    const uint32_t len = it->second;
+    Q->free_ranges.erase(it);
+    if (len > n) Q->free_ranges[offset + uint32_t(n)] = len - uint32_t(n);
+    Q->high_water = std::max<uint32_t>(Q->high_water, offset + uint32_t(n));
+
+    std::vector<double> buf(4 * n, 0.0);
+    const double *src[4] = {x0, v0, theta0, omega0};
+    double *dst[4] = {Q->x, Q->v, Q->th, Q->w};
+    for (int f = 0; f < 4; f++) {
+        std::fill(buf.begin(), buf.end(), 0.0);
+        if (src[f])
+            for (size_t i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) buf[4 * i + c] = src[f][3 * i + c];
+        HG_HIP(hipMemcpyAsync(dst[f] + 4 * size_t(offset), buf.data(), buf.size() * 8, hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+    }
+    HG_HIP(launch_zero_f64(Q->a + 4 * size_t(offset), 4 * n, Q->stream));
+    HG_HIP(launch_zero_f64(Q->al + 4 * size_t(offset), 4 * n, Q->stream));
+    {
+        std::vector<uint32_t> own(n, uint32_t(slot));
+        HG_HIP(hipMemcpyAsync(Q->owner + offset, own.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+    }
+    if (Q->cap_s) {
+        std::vector<uint32_t> pj(size_t(Q->cap_s) * n, kNoEntry), cnt(n);
+        for (size_t i = 0; i < n; i++) {
+            cnt[i] = uint32_t(rows[i].size());
+            for (size_t s = 0; s < rows[i].size(); s++) pj[s * n + i] = rows[i][s];
+        }
+        HG_HIP(hipMemcpy2DAsync(Q->pj[0] + offset, Q->P * 4, pj.data(), n * 4, n * 4, Q->cap_s,
+                                hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipMemcpyAsync(Q->pcnt[0] + offset, cnt.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
+        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ps[0], Q->P, offset, uint32_t(n), Q->cap_s * 9);
+        HG_HIP(hipGetLastError());
+        HG_HIP(hipStreamSynchronize(Q->stream));
+    }
+    if (Q->cap_n) HG_HIP(launch_zero_u32(Q->ncnt + offset, n, Q->stream));
+    Q->meta[slot] = pool_meta{offset, uint32_t(n), 0, 0, 0, 1, 0, 0};
+    Q->step_index[slot] = 0;
+    Q->t[slot] = 0.0;
+    Q->active[slot] = 1;
+    Q->bonds[slot] = std::move(bonds);
+    Q->skipped[slot] = skipped;
+    Q->order_dirty = true;
+    PG_TRY(upload_meta(Q));
+    *slot_out = slot;
+    return HG_OK;
+}
+
+hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
+    PG_TRY(valid_slot(Q, slot));
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    {
+        std::vector<uint32_t> none(n, kNoEntry);
+        HG_HIP(hipMemcpyAsync(Q->owner + off, none.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+    }
+    Q->active[slot] = 0;
+    Q->meta[slot] = pool_meta{0, 0, 0, 0, 0, 0, 0, 0};
+    Q->bonds[slot].clear();
+    Q->order_dirty = true;
+    // free the range and coalesce with neighbours
+    auto it = Q->free_ranges.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != Q->free_ranges.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        Q->free_ranges.erase(nx);
+    }
+    if (it != Q->free_ranges.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            Q->free_ranges.erase(it);
+        }
+    }
+    // lower the high-water mark when the top range is free
+    auto last = std::prev(Q->free_ranges.end());
+    if (last->first + last->second == Q->P) Q->high_water = std::min<uint32_t>(Q->high_water, last->first);
+    return upload_meta(Q);
+}
+
+hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *verts, const double *v_facet) {
+    PG_TRY(valid_slot(Q, slot));
+    if (nf > Q->MF) return fail(HG_ERR_INVALID, "hg_pool_set_facets: more facets than the pool's max_facets");
+    if (nf > 0 && (!verts || !v_facet)) return fail(HG_ERR_INVALID, "hg_pool_set_facets: verts and v_facet required");
+    std::vector<facet_dev> F(std::max<size_t>(nf, 1));
+    for (size_t f = 0; f < nf; f++) {
+        const double *q = verts + 9 * f;
+        for (int k = 0; k < 3; k++) F[f].vert[k] = d3{q[3 * k], q[3 * k + 1], q[3 * k + 2]};
+        F[f].v_facet = d3{v_facet[3 * f], v_facet[3 * f + 1], v_facet[3 * f + 2]};
+        d3 u = F[f].vert[1] - F[f].vert[0], w = F[f].vert[2] - F[f].vert[0];
+        F[f].area = 1.0 / 2.0 * norm(cross(u, w));
+        F[f].normal = normalized(cross(u, w));
+        F[f].pad[0] = F[f].pad[1] = 0.0;
+    }
+    if (nf > 0) {
+        HG_HIP(hipMemcpyAsync(Q->facets + size_t(slot) * Q->MF, F.data(), nf * sizeof(facet_dev),
+                              hipMemcpyHostToDevice, Q->stream));
+        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ss, Q->P, Q->meta[slot].offset, Q->meta[slot].n,
+                                                    uint32_t(nf * 9));
+        HG_HIP(hipGetLastError());
+    }
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    Q->meta[slot].nf = uint32_t(nf);
+    Q->meta[slot].acc_valid = 0;
+    return upload_meta(Q);
+}
+
+hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    Q->last_chunk_ms = Q->last_rebuild_ms = 0.0;
+    Q->last_chunks = 0;
+    if (nsteps == 0) return HG_OK;
+    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
+    if (nl && rebuild_every && nsteps % rebuild_every)
+        return fail(HG_ERR_INVALID, "hg_pool_step: nsteps must be a multiple of rebuild_every");
+    if (Q->order_dirty) {
+        Q->order.clear();
+        for (size_t s = 0; s < Q->S; s++)
+            if (Q->active[s]) Q->order.push_back(uint32_t(s));
+        std::stable_sort(Q->order.begin(), Q->order.end(),
+                         [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
+        if (!Q->order.empty())
+            HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data(), Q->order.size() * 4, hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        Q->order_dirty = false;
+    }
+    const uint32_t count = uint32_t(Q->order.size());
+    if (count == 0) return HG_OK;
+    // chunks: one rebuild interval each in neighbour-list mode; bounded
+    // lengths otherwise so a launch stays short
+    const uint64_t chunk = (nl && rebuild_every) ? rebuild_every : std::min<uint64_t>(nsteps, 2000);
+    const pool_view V = make_pool_view(Q);
+    const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
+    for (uint64_t done = 0; done < nsteps;) {
+        const uint32_t len = uint32_t(std::min<uint64_t>(chunk, nsteps - done));
+        if (nl) {
+            bool any = false;
+            for (uint32_t s : Q->order) {
+                const bool due = Q->meta[s].rebuild || (rebuild_every && Q->step_index[s] % rebuild_every == 0);
+                Q->meta[s].rebuild = due ? 1u : 0u;
+                any |= due;
+            }
+            if (any) {
+                PG_TRY(upload_meta(Q));
+                HG_HIP(hipEventRecord(Q->ev[0], Q->stream));
+                PG_TRY(rebuild_due(Q));
+                HG_HIP(hipEventRecord(Q->ev[1], Q->stream));
+                HG_HIP(hipEventSynchronize(Q->ev[1]));
+                float ms = 0.f;
+                HG_HIP(hipEventElapsedTime(&ms, Q->ev[0], Q->ev[1]));
+                Q->last_rebuild_ms += ms;
+                for (uint32_t s : Q->order) Q->meta[s].rebuild = 0;
+            }
+        }
+        PG_TRY(upload_meta(Q));
+        HG_HIP(launch_zero_u32(Q->counter_d, 1, Q->stream));
+        HG_HIP(hipEventRecord(Q->ev[2], Q->stream));
+        hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt,
+                                                                   grid, Q->stream)
+                          : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, grid,
+                                                         Q->stream);
+        HG_HIP(e);
+        HG_HIP(hipEventRecord(Q->ev[3], Q->stream));
+        PG_TRY(check_pool_flags(Q, "hg_pool_step"));
+        float ms = 0.f;
+        HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
+        Q->last_chunk_ms += ms;
+        Q->last_chunks++;
+        // host mirror of what the kernel did to each system
+        for (uint32_t s : Q->order) {
+            pool_meta &M = Q->meta[s];
+            if (Q->cap_s) {
+                if (!M.acc_valid) M.cur ^= 1u;
+                M.cur ^= (len & 1u);
+            }
+            M.acc_valid = 1;
+            Q->step_index[s] += len;
+            for (uint32_t k = 0; k < len; k++) Q->t[s] = Q->t[s] + dt;
+        }
+        done += len;
+    }
+    return upload_meta(Q);
+}
+
+hg_status hg_pool_get_state(hg_pool *Q, int32_t slot, double *x, double *v, double *theta, double *omega) {
+    PG_TRY(valid_slot(Q, slot));
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    std::vector<double> buf(4 * size_t(n));
+    const double *src[4] = {Q->x, Q->v, Q->th, Q->w};
+    double *dst[4] = {x, v, theta, omega};
+    for (int f = 0; f < 4; f++) {
+        if (!dst[f]) continue;
+        HG_HIP(hipMemcpyAsync(buf.data(), src[f] + 4 * size_t(off), buf.size() * 8, hipMemcpyDeviceToHost, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) dst[f][3 * i + c] = buf[4 * i + c];
+    }
+    return HG_OK;
+}
+
+hg_status hg_pool_get_accelerations(hg_pool *Q, int32_t slot, double *a, double *alpha) {
+    PG_TRY(valid_slot(Q, slot));
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    std::vector<double> buf(4 * size_t(n));
+    const double *src[2] = {Q->a, Q->al};
+    double *dst[2] = {a, alpha};
+    for (int f = 0; f < 2; f++) {
+        if (!dst[f]) continue;
+        HG_HIP(hipMemcpyAsync(buf.data(), src[f] + 4 * size_t(off), buf.size() * 8, hipMemcpyDeviceToHost, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) dst[f][3 * i + c] = buf[4 * i + c];
+    }
+    return HG_OK;
+}
+
+hg_status hg_pool_get_neighbors(hg_pool *Q, int32_t slot, uint64_t *offsets, uint32_t *idx, size_t cap,
+                                size_t *total) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!offsets) return fail(HG_ERR_INVALID, "offsets is required");
+    if (!Q->cap_n) return fail(HG_ERR_INVALID, "pool has no neighbour lists");
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    std::vector<uint32_t> nb(size_t(Q->cap_n) * n), cnt(n);
+    PG_TRY(download_rows(Q, Q->nb, off, n, Q->cap_n, nb.data()));
+    PG_TRY(download_rows(Q, Q->ncnt, off, n, 1, cnt.data()));
+    size_t tot = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        offsets[i] = tot;
+        for (uint32_t k = 0; k < cnt[i]; k++) {
+            if (idx && tot < cap) idx[tot] = nb[size_t(k) * n + i] - off;
+            tot++;
+        }
+    }
+    offsets[n] = tot;
+    if (total) *total = tot;
+    return HG_OK;
+}
+
+hg_status hg_pool_get_springs(hg_pool *Q, int32_t slot, int which, uint32_t *ij, double *springs, size_t cap,
+                              size_t *total) {
+    PG_TRY(valid_slot(Q, slot));
+    if (total) *total = 0;
+    if (!Q->cap_s) return HG_OK;
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n, cur = Q->meta[slot].cur;
+    std::vector<uint32_t> pj(size_t(Q->cap_s) * n), cnt(n);
+    std::vector<double> ps(size_t(Q->cap_s) * 9 * n);
+    PG_TRY(download_rows(Q, Q->pj[cur], off, n, Q->cap_s, pj.data()));
+    PG_TRY(download_rows(Q, Q->ps[cur], off, n, Q->cap_s * 9, ps.data()));
+    PG_TRY(download_rows(Q, Q->pcnt[cur], off, n, 1, cnt.data()));
+    size_t tot = 0;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t s = 0; s < cnt[i]; s++) {
+            const uint32_t tag = pj[size_t(s) * n + i];
+            if (((tag & kBondBit) != 0) != (which == 1)) continue;
+            if (tot < cap) {
+                if (ij) {
+                    ij[2 * tot] = i;
+                    ij[2 * tot + 1] = (tag & kIdxMask) - off;
+                }
+                if (springs)
+                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * 9 + c) * n + i];
+            }
+            tot++;
+        }
+    if (total) *total = tot;
+    return HG_OK;
+}
+
+hg_status hg_pool_get_bonds(hg_pool *Q, int32_t slot, uint32_t *pairs, size_t cap, size_t *nbonds,
+                            size_t *nskipped) {
+    PG_TRY(valid_slot(Q, slot));
+    const auto &B = Q->bonds[slot];
+    for (size_t b = 0; pairs && b < B.size() && b < cap; b++) {
+        pairs[2 * b] = B[b].first;
+        pairs[2 * b + 1] = B[b].second;
+    }
+    if (nbonds) *nbonds = B.size();
+    if (nskipped) *nskipped = Q->skipped[slot];
+    return HG_OK;
+}
+
+hg_status hg_pool_info(hg_pool *Q, int32_t slot, size_t *n, double *t, uint64_t *step_index) {
+    PG_TRY(valid_slot(Q, slot));
+    if (n) *n = Q->meta[slot].n;
+    if (t) *t = Q->t[slot];
+    if (step_index) *step_index = Q->step_index[slot];
+    return HG_OK;
+}
+
+hg_status hg_pool_last_step_profile(hg_pool *Q, double *chunk_ms, double *rebuild_ms, uint64_t *chunks) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (chunk_ms) *chunk_ms = Q->last_chunk_ms;
+    if (rebuild_ms) *rebuild_ms = Q->last_rebuild_ms;
+    if (chunks) *chunks = Q->last_chunks;
+    return HG_OK;
+}
+
+}  // extern "C"

@@ -163,13 +163,14 @@ sys_view make_view(hg_system *S) {
     v.al = S->al;
     v.nb = S->nb;
     v.ncnt = S->ncnt;
-    v.pj_old = S->pj[S->cur];
-    v.ps_old = S->ps[S->cur];
-    v.pcnt_old = S->pcnt[S->cur];
-    v.pj_new = S->pj[S->cur ^ 1];
-    v.ps_new = S->ps[S->cur ^ 1];
-    v.pcnt_new = S->pcnt[S->cur ^ 1];
-    v.cap_s = S->cap_s;
+    v.pt.pj_old = S->pj[S->cur];
+    v.pt.ps_old = S->ps[S->cur];
+    v.pt.pcnt_old = S->pcnt[S->cur];
+    v.pt.pj_new = S->pj[S->cur ^ 1];
+    v.pt.ps_new = S->ps[S->cur ^ 1];
+    v.pt.pcnt_new = S->pcnt[S->cur ^ 1];
+    v.pt.cap = S->cap_s;
+    v.pt.stride = S->n;
     v.facets = S->facets;
     v.ss = S->ss;
     v.nf = S->nf;
@@ -196,9 +197,11 @@ hg_status validate(const hg_params *p, size_t n) {
         return fail(HG_ERR_INVALID, "unknown pair_mode");
     if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(p->r_verlet > 0.0))
         return fail(HG_ERR_INVALID, "neighbour-list mode needs r_verlet > 0");
-    if (p->precision != HG_PRECISION_FP64)
-        return fail(HG_ERR_INVALID, "single-system path supports HG_PRECISION_FP64 only; mixed precision runs "
-                                    "through the batched path");
+    if (p->precision != HG_PRECISION_FP64 && p->precision != HG_PRECISION_MIXED)
+        return fail(HG_ERR_INVALID, "unknown precision");
+    if (p->precision == HG_PRECISION_MIXED && p->pair_mode != HG_PAIRS_NEIGHBOR_LIST)
+        return fail(HG_ERR_INVALID, "a single system runs HG_PRECISION_MIXED in neighbour-list mode only; "
+                                    "all-pairs systems run it through a pool");
     return HG_OK;
 }
 
@@ -311,6 +314,7 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
         HG_TRYC(dalloc(S, &S->bucket_count, S->hash_size));
         HG_TRYC(dalloc(S, &S->bucket_start, S->hash_size));
         HG_TRYC(dalloc(S, &S->bucket_fill, S->hash_size));
+        HG_TRYC(dalloc(S, &S->scan_tmp, scan_tmp_words(S->hash_size)));
         HG_TRYC(dalloc(S, &S->bbox, 8));
         HG_HIPC(launch_neighbor_build(*S, S->stream));
         HG_TRYC(check_flags(S, "hg_create"));

@@ -447,6 +447,35 @@ public:
         return have_b ? b : h;
     }
 
+    // The fold of one ordered pair on copies of its springs (no state change):
+    // the denominator of per-particle force errors, sum_j |a_ij|.
+    acc2 pair_peek(size_t i, uint32_t j) const {
+        bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
+        acc2 b{vzero, vzero}, h{vzero, vzero};
+        auto find = [&](const spring_rows &rows) {
+            spring3 s{{vzero, vzero, vzero}};
+            const auto &r = rows.rows[i];
+            auto it = std::lower_bound(r.begin(), r.end(), j,
+                                       [](const std::pair<uint32_t, spring3> &e, uint32_t key) { return e.first < key; });
+            if (it != r.end() && it->first == j) s = it->second;
+            return s;
+        };
+        if (have_b) {
+            if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
+                spring3 s = find(sinter_springs);
+                b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], s);
+            } else {
+                spring3 s = find(contact_springs);
+                bool reset;
+                b = contact_eval(p.contact, x[i], x[j], v[i], v[j], w[i], w[j], s, reset);
+            }
+        }
+        if (have_h)
+            h = p.hamaker_model == HAM_FULL ? hamaker_eval(p.hamaker, x[i], x[j]) : lr_hamaker_eval(p.hamaker, x[i], x[j]);
+        if (have_b && have_h) return b + h;
+        return have_b ? b : h;
+    }
+
     // triangular_facet.h:32-51 then the unary container's right fold over facets.
     acc2 unary(size_t i) {
         size_t nf = facets.size();

@@ -289,6 +289,31 @@ size_t or_sys_get_springs(void *s, int which, uint32_t *ij, double *springs, siz
     return tot;
 }
 
+// Per particle: sum over evaluated partners of |a_ij| (linear) and |alpha_ij|
+// (angular) at the current state, without changing any spring. out = 2*n.
+void or_sys_pair_magnitudes(void *s, double *out) {
+    auto *S = static_cast<dem_system *>(s);
+    const long nn = long(S->n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long ii = 0; ii < nn; ii++) {
+        size_t i = size_t(ii);
+        double sl = 0.0, sa = 0.0;
+        auto add = [&](uint32_t j) {
+            acc2 e = S->pair_peek(i, j);
+            sl += norm(e.lin);
+            sa += norm(e.ang);
+        };
+        if (S->p.neighbor_mode == 1) {
+            for (uint32_t j : S->nbr[i]) add(j);
+        } else {
+            for (uint32_t j = 0; j < uint32_t(S->n); j++)
+                if (j != i) add(j);
+        }
+        out[2 * i] = sl;
+        out[2 * i + 1] = sa;
+    }
+}
+
 // test/compute_energy.cpp:7-58 and test/mass_distribution.cpp:7-27.
 // out: ke_total, ke_trans, ke_rot, |P|, |L|, rg, cm[3]
 void or_sys_diagnostics(void *s, double mass, double inertia, double *out) {

@@ -130,6 +130,7 @@ def lib():
         L.or_sys_get_springs.restype = C.c_size_t
         L.or_sys_get_springs.argtypes = [C.c_void_p, C.c_int, _u32p, _dp, C.c_size_t]
         L.or_sys_diagnostics.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
+        L.or_sys_pair_magnitudes.argtypes = [C.c_void_p, _dp]
         L.or_contact_seq.argtypes = [_dp, C.c_size_t, _dp, _dp, _dp]
         L.or_sinter_seq.argtypes = [_dp, _dp, C.c_size_t, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.or_hamaker_seq.argtypes = [C.c_int, _dp, C.c_size_t, _dp, _dp]
@@ -342,6 +343,12 @@ class OracleSystem:
         s = np.zeros((max(tot, 1), 9))
         lib().or_sys_get_springs(self._h, which, u32ptr(ij), dptr(s), tot)
         return ij[:tot], s[:tot]
+
+    def pair_magnitudes(self):
+        """(n, 2): sum_j |a_ij| and sum_j |alpha_ij| at the current state."""
+        out = np.zeros((self.n, 2))
+        lib().or_sys_pair_magnitudes(self._h, dptr(out))
+        return out
 
     def diagnostics(self, mass=MASS, inertia=INERTIA):
         out = np.zeros(9)

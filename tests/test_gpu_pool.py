@@ -1,0 +1,225 @@
+"""Pools (batched / serving path) and mixed precision on the GPU.
+
+FP64 pools must reproduce the oracle bit for bit per system, whatever else
+shares the pool. Mixed precision must stay within the stated force tolerance
+of the FP64 oracle (1e-5 relative) and give bit-identical results for a system
+alone and in a mix.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import _scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+R = O.R_PART
+DT = 1e-13
+
+
+def _L():
+    import libgran_amd
+    return libgran_amd
+
+
+def announce(name):
+    print(f"[gpu-pool] {name}", flush=True)
+
+
+def agg_case(n, seed, df=1.8):
+    L = _L()
+    x = L.generate_aggregate(n, R, df, 1.3, seed)
+    v0 = np.random.default_rng(seed).normal(0.0, 1.0, x.shape)
+    return x, v0
+
+
+def nl_params(precision=0):
+    L = _L()
+    c, s, h = S.aggregate_params()
+    return L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R,
+                         contact=c, sinter=s, hamaker=h, precision=precision)
+
+
+def oracle_nl(x, v0):
+    c, s, h = S.aggregate_params()
+    return O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R, contact=c,
+                                          sinter=s, hamaker=h), x, v0)
+
+
+def same(a, b, what):
+    for k in ("x", "v", "theta", "omega", "a", "alpha"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}: {k}")
+
+
+def test_pool_fp64_matches_oracle_per_system():
+    announce("pool FP64 vs oracle, 5 aggregates")
+    L = _L()
+    cases = [agg_case(n, seed) for n, seed in ((50, 10), (137, 11), (400, 12), (1200, 13), (3000, 14))]
+    pool = L.Pool(nl_params(), max_particles=6000, max_systems=8)
+    slots = [pool.add(x, v) for x, v in cases]
+    pool.step(DT, 200, rebuild_every=50)
+    for (x, v), slot in zip(cases, slots):
+        o = oracle_nl(x, v)
+        o.step(DT, 200, rebuild_every=50)
+        same(o.state(), pool.state(slot), f"slot {slot} n={x.shape[0]}")
+        off1, idx1 = o.neighbors()
+        off2, idx2 = pool.neighbors(slot)
+        np.testing.assert_array_equal(off2, off1)
+        np.testing.assert_array_equal(idx2, idx1)
+        ij1, s1 = o.springs(1)
+        ij2, s2 = pool.springs(slot, 1)
+        np.testing.assert_array_equal(ij2, ij1)
+        np.testing.assert_array_equal(s2, s1)
+        assert pool.info(slot)[2] == 200
+
+
+def test_pool_all_pairs_reference_lattices():
+    """All-pairs pool: the contact+Hamaker lattice (test/hamaker.cpp) twice
+    with different impactor speeds, next to each other."""
+    announce("pool all pairs lattices")
+    L = _L()
+    x, v = S.lattice_with_impactor()
+    v2 = v.copy()
+    v2[-1] = [0.0, 2.0, -20.0]
+    ham = O.hamaker_params(A=1e-20, h0=1e-9)
+    p = L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=O.contact_params(), hamaker=ham)
+    pool = L.Pool(p, max_particles=64, max_systems=4)
+    s1, s2 = pool.add(x, v), pool.add(x, v2)
+    pool.step(DT, 12000)
+    for vv, slot in ((v, s1), (v2, s2)):
+        o = O.OracleSystem(O.system_params(O.BIN_CONTACT, O.HAM_FULL, hamaker=ham), x, vv)
+        o.step(DT, 12000)
+        same(o.state(), pool.state(slot), f"lattice slot {slot}")
+
+
+def test_pool_with_facets_matches_oracle():
+    announce("pool facets vs oracle")
+    L = _L()
+    c, s, h = S.aggregate_params()
+    sh = O.hamaker_params(A=1e-20, h0=1e-10)
+    p = L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R,
+                      contact=c, sinter=s, hamaker=h, facet_contact=1, facet_hamaker=1, surface_contact=c,
+                      surface_hamaker=sh)
+    pool = L.Pool(p, max_particles=2000, max_systems=4, max_facets=4)
+    runs = []
+    for seed, speed in ((21, 4.0), (22, -3.0)):
+        x, v0 = agg_case(300, seed)
+        zmin, zmax = x[:, 2].min(), x[:, 2].max()
+        a = 80 * R
+        verts = np.array([[[-a, -a, zmin - 1.05 * R], [a, -a, zmin - 1.05 * R], [-a, a, zmin - 1.05 * R]],
+                          [[a, a, zmin - 1.05 * R], [-a, a, zmin - 1.05 * R], [a, -a, zmin - 1.05 * R]],
+                          [[-a, -a, zmax + 1.05 * R], [-a, a, zmax + 1.05 * R], [a, -a, zmax + 1.05 * R]],
+                          [[a, a, zmax + 1.05 * R], [a, -a, zmax + 1.05 * R], [-a, a, zmax + 1.05 * R]]])
+        vel = np.array([[0, 0, speed], [0, 0, speed], [0, 0, -speed], [0, 0, -speed]], dtype=float)
+        slot = pool.add(x, v0)
+        pool.set_facets(slot, verts, vel)
+        runs.append((x, v0, verts, vel, slot))
+    pool.step(DT, 500, rebuild_every=50)
+    for x, v0, verts, vel, slot in runs:
+        o = O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R, contact=c,
+                                           sinter=s, hamaker=h, facet_contact=1, facet_hamaker=1, surface_contact=c,
+                                           surface_hamaker=sh), x, v0)
+        o.set_facets(verts, vel)
+        o.step(DT, 500, rebuild_every=50)
+        same(o.state(), pool.state(slot), f"facet slot {slot}")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pool_result_alone_equals_in_mix(precision):
+    announce(f"pool alone vs mix, precision {precision}")
+    L = _L()
+    x, v0 = agg_case(700, 31)
+    alone = L.Pool(nl_params(precision), max_particles=1000, max_systems=2)
+    sa = alone.add(x, v0)
+    alone.step(DT, 300, rebuild_every=50)
+    mix = L.Pool(nl_params(precision), max_particles=20000, max_systems=16)
+    others = [agg_case(n, 40 + k) for k, n in enumerate((2500, 60, 900, 4000, 333))]
+    for xo, vo in others[:3]:
+        mix.add(xo, vo)
+    sm = mix.add(x, v0)
+    for xo, vo in others[3:]:
+        mix.add(xo, vo)
+    mix.step(DT, 300, rebuild_every=50)
+    same(alone.state(sa), mix.state(sm), "alone vs mix")
+
+
+def test_pool_continuous_admission_and_retirement():
+    """Systems join and leave between steps; each result equals its own
+    alone-run with the same number of steps."""
+    announce("pool continuous batching")
+    L = _L()
+    pool = L.Pool(nl_params(1), max_particles=5000, max_systems=8)
+    a = agg_case(800, 51)
+    b = agg_case(1500, 52)
+    c = agg_case(1200, 53)
+    sa, sb = pool.add(*a), pool.add(*b)
+    pool.step(DT, 100, rebuild_every=50)
+    pool.remove(sa)
+    sc = pool.add(*c)  # reuses a's slot and part of its range
+    pool.step(DT, 150, rebuild_every=50)
+    res_b, res_c = pool.state(sb), pool.state(sc)
+    assert pool.info(sb)[2] == 250 and pool.info(sc)[2] == 150
+    for (x, v), steps, res in ((b, 250, res_b), (c, 150, res_c)):
+        solo = L.Pool(nl_params(1), max_particles=2000, max_systems=1)
+        s = solo.add(x, v)
+        solo.step(DT, steps, rebuild_every=50)
+        same(solo.state(s), res, "continuous vs alone")
+
+
+def _force_errors(a_gpu, a_ref, mags):
+    diff = np.linalg.norm(a_gpu - a_ref, axis=1)
+    normwise = np.linalg.norm(a_gpu - a_ref) / np.linalg.norm(a_ref)
+    denom = np.where(mags > 0, mags, 1.0)
+    per = diff / denom
+    return normwise, per.max(), np.sqrt(np.mean(per ** 2))
+
+
+def test_mixed_precision_force_error_full_size():
+    """N=20000 aggregate (seed 1): one acceleration pass in MIXED against the
+    FP64 oracle from the same evolved state. Tolerance (stated): 1e-5,
+    normwise and per particle relative to sum_j |a_ij|."""
+    announce("mixed precision force error N=20000")
+    L = _L()
+    x, v0 = agg_case(20000, 1)
+    o = oracle_nl(x, v0)
+    o.step(DT, 100, rebuild_every=50)
+    st = o.state()
+    # fresh systems on the evolved state: springs zero on both sides
+    o2 = oracle_nl(st["x"], st["v"])
+    o2.set_state(omega=st["omega"])
+    g = L.System(nl_params(1), st["x"], st["v"], omega0=st["omega"])
+    o2.compute_accelerations()
+    g.compute_accelerations()
+    ref = o2.state()
+    got = g.state()
+    mags = o2.pair_magnitudes()
+    nw, pmax, prms = _force_errors(got["a"], ref["a"], mags[:, 0])
+    nwa, pmaxa, prmsa = _force_errors(got["alpha"], ref["alpha"], mags[:, 1])
+    print(f"[mixed] linear normwise {nw:.3e} per-particle max {pmax:.3e} rms {prms:.3e}; "
+          f"angular normwise {nwa:.3e} max {pmaxa:.3e} rms {prmsa:.3e}", flush=True)
+    assert nw < 1e-5 and pmax < 1e-5
+    assert pmaxa < 1e-5
+
+
+def test_mixed_precision_short_trajectory():
+    """200 steps with springs building up: MIXED vs the FP64 oracle. Force
+    error stays within 1e-5 (per particle, relative to sum_j |a_ij|) at every
+    compared step; positions deviate by far less than a nanometre."""
+    announce("mixed precision trajectory N=3000")
+    L = _L()
+    x, v0 = agg_case(3000, 2)
+    o = oracle_nl(x, v0)
+    g = L.System(nl_params(1), x, v0)
+    worst = 0.0
+    for k in range(4):
+        o.step(DT, 50, rebuild_every=50)
+        g.step(DT, 50, rebuild_every=50)
+        ref, got = o.state(), g.state()
+        mags = o.pair_magnitudes()[:, 0]
+        _, pmax, _ = _force_errors(got["a"], ref["a"], mags)
+        worst = max(worst, pmax)
+        dx = np.abs(got["x"] - ref["x"]).max()
+        print(f"[mixed] step {50 * (k + 1)}: per-particle force error max {pmax:.3e}, max |dx| {dx:.3e} m",
+              flush=True)
+    assert worst < 1e-5
+    assert dx < 1e-12
