@@ -82,55 +82,74 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
     return m;
 }
 
-// Rotates a spring into the plane normal to n and restores its length (FP64).
-__device__ __forceinline__ d3 rotate_spring(d3 xi, f3 nf) {
-    const d3 n = tod(nf);
+// Rotates a spring into the plane normal to n and restores its length, in
+// FP64 with the FP64 normal. The torsion spring accumulates along n, so its
+// projection leaves only the residual from the turn of n since the previous
+// step (~|v| dt / r, about 1e-5 rad here), which the rescale then restores to
+// full length: the result's direction is that residual's, and an FP32 normal
+// (error ~6e-8 rad) would perturb it by ~1%.
+__device__ __forceinline__ d3 rotate_spring(d3 xi, d3 n) {
     d3 xr = xi - dot(xi, n) * n;
     const double l2 = dot(xr, xr);
     if (l2 > 0.0) xr = xr * sqrt(dot(xi, xi) / l2);
     return xr;
 }
 
-// contact_force.h:105-147 with FP32 forces and an FP64 spring.
-__device__ __forceinline__ f3 shear_contact_mx(d3 &xi, f3 n, float k, float g, float fn, float mus, float mud, f3 rv,
-                                               double dt) {
+// contact_force.h:105-147 with FP32 forces and an FP64 spring. The spring's
+// own updates use the FP64 relative velocity rv64: the static increment
+// rv dt and the dynamic reset -(f_d t + gamma rv)/k (with t from the FP64
+// test force) set the direction that the next step's projection keeps.
+__device__ __forceinline__ f3 shear_contact_mx(d3 &xi, d3 n64, float k, float g, float fn, float mus, float mud,
+                                               f3 rv, d3 rv64, double dt) {
 #pragma clang fp contract(fast)
-    xi = rotate_spring(xi, n);
+    xi = rotate_spring(xi, n64);
     const f3 f0 = (-k) * tof(xi) - g * rv;
     const float sf = mus * fn, df = mud * fn;
     const float f0n = sqrtf(dotf(f0, f0));
     const f3 t = f0n > 0.f ? f0 * (1.0f / f0n) : zerof();
     float fsel;
     if (f0n <= sf) {
-        xi = xi + tod(rv) * dt;
+        xi = xi + rv64 * dt;
         fsel = sf;
     } else {
-        xi = tod((-1.0f / k) * (df * t + g * rv));
+        const double kd = double(k), gd = double(g);
+        const d3 f064 = (-kd) * xi - gd * rv64;
+        const double l = sqrt(dot(f064, f064));
+        const d3 t64 = l > 0.0 ? f064 * (1.0 / l) : zero3();
+        xi = (-1.0 / kd) * (double(df) * t64 + gd * rv64);
         fsel = df;
     }
     return ((f0n < fsel) ? f0n : fsel) * t;
 }
 
 // alt_sinter_bridge.h:137-156 with FP32 forces and an FP64 spring.
-__device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, f3 n, float k, float g, f3 rv, double dt) {
+__device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, d3 n64, float k, float g, f3 rv, d3 rv64, double dt) {
 #pragma clang fp contract(fast)
-    xi = rotate_spring(xi, n);
+    xi = rotate_spring(xi, n64);
     const f3 f0 = (-k) * tof(xi) - g * rv;
-    xi = xi + tod(rv) * dt;
+    xi = xi + rv64 * dt;
     return f0;
 }
 
 // Shared kinematics of contact and bonded pairs: n from the FP64 distance.
-__device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, f3 &n) {
+__device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, d3 &n64, f3 &n) {
     const d3 d = xi - xj;
     dist = sqrt(dot(d, d));
     const double inv = dist > 0.0 ? 1.0 / dist : 0.0;
-    n = tof(d * inv);
+    n64 = d * inv;
+    n = tof(n64);
 }
 
+// The relative-velocity decomposition stays in FP64: when a pair's relative
+// velocity is nearly parallel to n, the tangential part
+// v_t = v_ij - (v_ij . n) n is a small difference that an FP32 normal would
+// leave with an absolute error of eps_32 |v_ij|, seen in full by the torque.
+// These are multiplies and adds only; divisions and square roots (normals of
+// forces, Coulomb test) are FP32. Identities used:
+// n x w_i + n x w_j = n x (w_i + w_j), n x w_i - n x w_j = n x (w_i - w_j).
 template <bool BONDED>
-__device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, double dist, f3 n, f3 vi, f3 vj, f3 wi,
-                                                     f3 wj, spring9 &sp, bool &active) {
+__device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, double dist, d3 n64, f3 n, d3 vi, d3 vj,
+                                                     d3 wi, d3 wj, spring9 &sp, bool &active) {
 #pragma clang fp contract(fast)
     const double ovd = (BONDED ? m.s_r2 : m.c_r2) - dist;
     if (!BONDED && ovd <= 0.0) {
@@ -140,25 +159,26 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
     }
     active = true;
     const float ov = float(ovd);
-    const float r = BONDED ? m.s_r : m.c_r;
-    const float rpp = r - 0.5f * ov;
-    const f3 dv = vi - vj;
-    const float v_n = -dotf(dv, n);
+    const double rd = 0.5 * (BONDED ? m.s_r2 : m.c_r2);
+    const double rppd = rd - 0.5 * ovd;
+    const float r = float(rd), rpp = float(rppd);
+    const d3 dv = vi - vj, dw = wi - wj, sw = wi + wj;
+    const float v_n = float(-dot(dv, n64));
     const float f_n = (BONDED ? m.s_k : m.c_k) * ov + (BONDED ? m.s_gn : m.c_gn) * v_n;
-    const f3 cwi = crossf(n, wi), cwj = crossf(n, wj);
-    const f3 v_ij = dv + rpp * cwi + rpp * cwj;
-    const f3 v_t = v_ij - dotf(v_ij, n) * n;
-    const f3 v_r = (-rpp) * (cwi - cwj);
-    const f3 v_o = (r * (dotf(n, wi) - dotf(n, wj))) * n;
+    const d3 v_ij = dv + rppd * cross(n64, sw);
+    const d3 vt64 = v_ij - dot(v_ij, n64) * n64;
+    const d3 vr64 = (-rppd) * cross(n64, dw);
+    const d3 vo64 = (rd * dot(n64, dw)) * n64;
+    const f3 v_t = tof(vt64), v_r = tof(vr64), v_o = tof(vo64);
     f3 f_t, f_r, f_o;
     if (BONDED) {
-        f_t = shear_sinter_mx(sp.s[0], n, m.s_kt, m.s_gt, v_t, m.s_dt);
-        f_r = shear_sinter_mx(sp.s[1], n, m.s_kr, m.s_gr, v_r, m.s_dt);
-        f_o = shear_sinter_mx(sp.s[2], n, m.s_ko, m.s_go, v_o, m.s_dt);
+        f_t = shear_sinter_mx(sp.s[0], n64, m.s_kt, m.s_gt, v_t, vt64, m.s_dt);
+        f_r = shear_sinter_mx(sp.s[1], n64, m.s_kr, m.s_gr, v_r, vr64, m.s_dt);
+        f_o = shear_sinter_mx(sp.s[2], n64, m.s_ko, m.s_go, v_o, vo64, m.s_dt);
     } else {
-        f_t = shear_contact_mx(sp.s[0], n, m.c_kt, m.c_gt, f_n, m.c_mus, m.c_mud, v_t, m.c_dt);
-        f_r = shear_contact_mx(sp.s[1], n, m.c_kr, m.c_gr, f_n, m.c_mur, m.c_mudr, v_r, m.c_dt);
-        f_o = shear_contact_mx(sp.s[2], n, m.c_ko, m.c_go, f_n, m.c_muo, m.c_mudo, v_o, m.c_dt);
+        f_t = shear_contact_mx(sp.s[0], n64, m.c_kt, m.c_gt, f_n, m.c_mus, m.c_mud, v_t, vt64, m.c_dt);
+        f_r = shear_contact_mx(sp.s[1], n64, m.c_kr, m.c_gr, f_n, m.c_mur, m.c_mudr, v_r, vr64, m.c_dt);
+        f_o = shear_contact_mx(sp.s[2], n64, m.c_ko, m.c_go, f_n, m.c_muo, m.c_mudo, v_o, vo64, m.c_dt);
     }
     const f3 tau = (-rpp) * crossf(n, f_t) + r * crossf(n, f_r) + r * f_o;
     return {(f_n * n + f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
@@ -180,7 +200,7 @@ __device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 
     return (-s * m.h_im) * n;
 }
 
-__device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf, f3 vf, d3 xi, f3 vi, f3 wi,
+__device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf, d3 vf_, d3 xi, d3 vi_, d3 w64,
                                                    spring9 &sp) {
 #pragma clang fp contract(fast)
     const d3 d = xi - xf;
@@ -190,20 +210,22 @@ __device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf,
         sp = zero_spring();
         return {zerof(), zerof()};
     }
-    const f3 n = tof(d * (1.0 / dist));
+    const d3 n64 = d * (1.0 / dist);
+    const f3 n = tof(n64);
     const float ov = float(ovd), r = m.sc_r;
-    const float rpp = r - ov;
-    const f3 dv = vi - vf;
-    const float v_n = -dotf(dv, n);
+    const double rppd = m.sc_rd - ovd;
+    const float rpp = float(rppd);
+    const d3 dv = vi_ - vf_;
+    const float v_n = float(-dot(dv, n64));
     const float f_n = m.sc_k * ov + m.sc_gn * v_n;
-    const f3 cw = crossf(n, wi);
-    const f3 v_ij = dv + rpp * cw;
-    const f3 v_t = v_ij - dotf(v_ij, n) * n;
-    const f3 v_r = (-rpp) * cw;
-    const f3 v_o = (r * dotf(n, wi)) * n;
-    const f3 f_t = shear_contact_mx(sp.s[0], n, m.sc_kt, m.sc_gt, f_n, m.sc_mus, m.sc_mud, v_t, m.sc_dt);
-    const f3 f_r = shear_contact_mx(sp.s[1], n, m.sc_kr, m.sc_gr, f_n, m.sc_mur, m.sc_mudr, v_r, m.sc_dt);
-    const f3 f_o = shear_contact_mx(sp.s[2], n, m.sc_ko, m.sc_go, f_n, m.sc_muo, m.sc_mudo, v_o, m.sc_dt);
+    const d3 v_ij = dv + rppd * cross(n64, w64);
+    const d3 vt64 = v_ij - dot(v_ij, n64) * n64;
+    const d3 vr64 = (-rppd) * cross(n64, w64);
+    const d3 vo64 = (m.sc_rd * dot(n64, w64)) * n64;
+    const f3 v_t = tof(vt64), v_r = tof(vr64), v_o = tof(vo64);
+    const f3 f_t = shear_contact_mx(sp.s[0], n64, m.sc_kt, m.sc_gt, f_n, m.sc_mus, m.sc_mud, v_t, vt64, m.sc_dt);
+    const f3 f_r = shear_contact_mx(sp.s[1], n64, m.sc_kr, m.sc_gr, f_n, m.sc_mur, m.sc_mudr, v_r, vr64, m.sc_dt);
+    const f3 f_o = shear_contact_mx(sp.s[2], n64, m.sc_ko, m.sc_go, f_n, m.sc_muo, m.sc_mudo, v_o, vo64, m.sc_dt);
     const f3 tau = (-rpp) * crossf(n, f_t) + r * crossf(n, f_r) + r * f_o;
     return {(f_n * n + f_t) * m.sc_im, tau * m.sc_ii};
 }

@@ -148,11 +148,12 @@ __device__ __forceinline__ acc6 facet_fold(const hg_params &P, const facet_dev *
 // Mixed-precision fold of one ordered pair (dem_mixed.h); the spring walk is
 // the FP64 one.
 template <int BIN, int HAM>
-__device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, f3 vi, f3 wi, d3 xj, f3 vj,
-                                             f3 wj, walker &W, accf &acc) {
+__device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj,
+                                             d3 wj, walker &W, accf &acc) {
     double dist;
+    d3 n64;
     f3 n;
-    pair_kinematics_mx(xi, xj, dist, n);
+    pair_kinematics_mx(xi, xj, dist, n64, n);
     accf b{zerof(), zerof()};
     f3 h = zerof();
     if constexpr (BIN != HG_BINARY_NONE) {
@@ -162,9 +163,9 @@ __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, 
         W.take(j, sp, bonded);
         bool keep;
         if (BIN == HG_BINARY_SINTER && bonded)
-            b = bonded_or_contact_mx<true>(m, dist, n, vi, vj, wi, wj, sp, keep);
+            b = bonded_or_contact_mx<true>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
         else
-            b = bonded_or_contact_mx<false>(m, dist, n, vi, vj, wi, wj, sp, keep);
+            b = bonded_or_contact_mx<false>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
         if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
     }
     if constexpr (HAM != HG_HAMAKER_NONE) h = hamaker_mx<HAM>(m, dist, n);
@@ -173,7 +174,7 @@ __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, 
 }
 
 __device__ __forceinline__ accf facet_fold_mx(const hg_params &P, const mixed_consts &m, const facet_dev *facets,
-                                              int nf, double *ss, size_t stride, uint32_t i, d3 xi, f3 vi, f3 wi) {
+                                              int nf, double *ss, size_t stride, uint32_t i, d3 xi, d3 vi, d3 wi) {
     accf s{zerof(), zerof()};
     for (int f = nf - 1; f >= 0; f--) {
         const facet_dev F = facets[f];
@@ -181,7 +182,7 @@ __device__ __forceinline__ accf facet_fold_mx(const hg_params &P, const mixed_co
         if (P.facet_contact) {
             double *b = ss + size_t(f) * 9 * stride + i;
             spring9 sp = ld_spring(b, stride);
-            s = s + surface_contact_mx(m, xf, tof(F.v_facet), xi, vi, wi, sp);
+            s = s + surface_contact_mx(m, xf, F.v_facet, xi, vi, wi, sp);
             st_spring(b, stride, sp);
         }
         if (P.facet_hamaker) s.lin = s.lin + surface_hamaker_mx(m, xf, xi);

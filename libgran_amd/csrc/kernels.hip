@@ -50,8 +50,7 @@ template <int BIN, int HAM>
 __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts MC, sys_view S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.n) return;
-    const d3 xi = ld3(S.x, i);
-    const f3 vi = tof(ld3(S.v, i)), wi = tof(ld3(S.w, i));
+    const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     accf acc{zerof(), zerof()};
     walker W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
@@ -59,7 +58,7 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     const uint32_t *nbi = S.nb + i;
     for (uint32_t k = 0; k < nn; k++) {
         const uint32_t j = nbi[size_t(k) * S.n];
-        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(S.x, j), tof(ld3(S.v, j)), tof(ld3(S.w, j)), W, acc);
+        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(S.x, j), ld3(S.v, j), ld3(S.w, j), W, acc);
     }
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold_mx(P, MC, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);

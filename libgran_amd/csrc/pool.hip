@@ -87,23 +87,20 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
             st3(V.al, g, acc.ang);
         } else {
             accf acc{zerof(), zerof()};
-            const f3 vf = tof(vi), wf = tof(wi);
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
                 const uint32_t nn = V.ncnt[g];
                 for (uint32_t k = 0; k < nn; k++) {
                     const uint32_t j = V.nb[size_t(k) * V.stride + g];
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vf, wf, ld3(V.x, j), tof(ld3(V.v, j)), tof(ld3(V.w, j)), W,
-                                           acc);
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
                 }
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vf, wf, ld3(V.x, j), tof(ld3(V.v, j)), tof(ld3(V.w, j)), W,
-                                           acc);
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags);
-            if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vf, wf);
+            if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             st3(V.a, g, tod(acc.lin));
             st3(V.al, g, tod(acc.ang));
         }
