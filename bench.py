@@ -1,17 +1,22 @@
 """Throughput benchmark of the DEM per-timestep path on one MI355X.
 
-Workload (default): one N=20,000 soot aggregate (TD-CCA, seed 1, Df 1.8,
-kf 1.3, r = 14 nm, rho = 1700 kg/m^3), alt_sinter bonds (gap < 1e-3 r, k = 1e4
-N/m, damping ratio 0.5) embedding Luding contact (mu = 1), limited-range
-Hamaker (A = 1e-20 J, cutoff 3r, h0 = 0.1 nm), neighbour list r_list = 3.2r
-rebuilt every 50 steps, dt = 1e-13 s, FP64, Gaussian initial velocities
-(sigma 1 m/s). A step is one velocity-Verlet step of the whole aggregate.
+Headline workload (`value`): the 4,096-aggregate batch. Particle counts
+N_k = clip(round(exp(ln 400 + z_k)), 50, 20000) with z_k ~ N(0, 1), fractal
+dimensions Df_k ~ U(1.6, 2.2) (numpy default_rng(2024)), kf = 1.3, TD-CCA
+aggregates with seeds 0..4095; r = 14 nm, rho = 1700 kg/m^3; alt_sinter bonds
+(gap < 1e-3 r, k = 1e4 N/m, damping ratio 0.5) embedding Luding contact
+(mu = 1), limited-range Hamaker (A = 1e-20 J, cutoff 3r, h0 = 0.1 nm),
+neighbour lists of radius 3.2r rebuilt every 50 steps, dt = 1e-13 s, Gaussian
+initial velocities (sigma 1 m/s). All aggregates are stepped together in one
+pool, mixed precision (FP32 forces, FP64 state and springs), 2,000 steps.
 
-Prints one JSON line: particle-steps/s, the roofline of the dominant kernel
-(the acceleration pass) from HIP events on its own stream over the timed
-region, and the CPU oracle timed on this host's cores on a bounded sample.
+Beside it on the same line: the same batch in an FP64 pool, the plain FP64
+GPU path (one system at a time, per-step launches) on a stratified sample of
+the batch, the single N=20,000 aggregate (FP64 and mixed), the roofline of the
+dominant kernel from HIP events on its own stream, and the CPU oracle on this
+host's cores on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps 5000] [--warmup 200]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
 """
 import argparse
 import json
@@ -30,8 +35,8 @@ RHO = 1700.0
 MASS = 4.0 / 3.0 * math.pi * R_PART ** 3 * RHO
 INERTIA = 2.0 / 5.0 * MASS * R_PART ** 2
 DT = 1e-13
+REBUILD = 50
 HBM_PEAK_GBS = 8000.0
-FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def aggregate_params(k=1.0e4, zeta=0.5, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.0 * R_PART, d_crit=1e-3 * R_PART):
@@ -42,107 +47,175 @@ def aggregate_params(k=1.0e4, zeta=0.5, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.
     return contact, sinter, ham
 
 
-def workload(n=20000, seed=1, df=1.8, kf=1.3):
-    import libgran_amd as L
-    x = L.generate_aggregate(n, R_PART, df, kf, seed)
-    v0 = np.random.default_rng(seed).normal(0.0, 1.0, x.shape)
-    return x, v0
+def params(L, precision):
+    c, s, h = aggregate_params()
+    return L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R_PART,
+                         contact=c, sinter=s, hamaker=h, precision=precision)
 
 
-def force_bytes_per_particle(kbar, cbar):
-    """Algorithmic HBM bytes of one acceleration pass per particle (FP64):
-    own x, v, omega (72) + a, alpha written (48) + neighbour count (4) and ids
-    (4 k) + pair-state count read/write (8) + pair-state entries read and
-    written (c * 2 * (4 + 72)). Partner gathers are not counted: every
-    particle's state is counted once, further reads are cache-served."""
-    return 72 + 48 + 4 + 4 * kbar + 8 + cbar * 2 * (4 + 72)
+def batch_workload(L, count=4096):
+    rng = np.random.default_rng(2024)
+    z = rng.standard_normal(count)
+    sizes = np.clip(np.round(np.exp(np.log(400.0) + z)), 50, 20000).astype(np.uint32)
+    dfs = rng.uniform(1.6, 2.2, count)
+    xs = L.generate_aggregates(sizes, dfs, 1.3, np.arange(count, dtype=np.uint64), R_PART)
+    vs = [np.random.default_rng(1000 + k).normal(0.0, 1.0, x.shape) for k, x in enumerate(xs)]
+    return sizes, xs, vs
+
+
+def state_bytes_per_particle_step(kbar, cbar):
+    """Algorithmic HBM bytes per particle-step (FP64 state, FP64 springs):
+    x, v, theta, omega read and written (96 + 96), a, alpha read and written
+    (48 + 48), neighbour ids (4 k), bond/contact springs read and written
+    (144 c), the list rebuild amortised over its interval ((24 + 4 k)/50)."""
+    return 288 + 4 * kbar + 144 * cbar + (24 + 4 * kbar) / REBUILD
+
+
+def run_pool(L, precision, sizes, xs, vs, steps, warmup):
+    pool = L.Pool(params(L, precision), max_particles=int(sizes.sum()) + 1024, max_systems=len(xs))
+    slots = [pool.add(x, v) for x, v in zip(xs, vs)]
+    pool.step(DT, warmup, rebuild_every=REBUILD)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pool.step(DT, steps, rebuild_every=REBUILD)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = pool.last_step_profile()
+    # measured neighbour and spring counts on a sample of systems
+    sample = slots[:: max(1, len(slots) // 64)]
+    npart = sum(pool.info(s)[0] for s in sample)
+    kbar = sum(len(pool.neighbors(s)[1]) for s in sample) / npart
+    cbar = sum(len(pool.springs(s, 0)[0]) + len(pool.springs(s, 1)[0]) for s in sample) / npart
+    return pool, el, prof, kbar, cbar
+
+
+def plain_fp64_sample(L, sizes, xs, vs, steps, nsample=48):
+    """The plain FP64 path: one hg_system at a time, per-step launches, on a
+    stratified sample of the batch (every len/nsample-th aggregate by size)."""
+    import torch
+    order = np.argsort(sizes)
+    pick = order[np.linspace(0, len(order) - 1, nsample).astype(int)]
+    ps, tt = 0, 0.0
+    p = params(L, 0)
+    for k in pick:
+        s = L.System(p, xs[k], vs[k])
+        s.step(DT, REBUILD, rebuild_every=REBUILD)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(DT, steps, rebuild_every=REBUILD)
+        torch.cuda.synchronize()
+        tt += time.perf_counter() - t0
+        ps += int(sizes[k]) * steps
+        s.close()
+    return ps / tt, f"{nsample} aggregates sampled across the size distribution, {steps} steps each"
+
+
+def single_aggregate(L, precision, steps, warmup):
+    import torch
+    x = L.generate_aggregate(20000, R_PART, 1.8, 1.3, 1)
+    v0 = np.random.default_rng(1).normal(0.0, 1.0, x.shape)
+    s = L.System(params(L, precision), x, v0)
+    s.step(DT, warmup, rebuild_every=REBUILD)
+    torch.cuda.synchronize()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.step(DT, steps, rebuild_every=REBUILD)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    launches, ms = s.profile_read()
+    return 20000 * steps / el, x, v0, launches, ms
 
 
 def cpu_baseline(x, v0, steps):
     """The CPU oracle (a restatement of the reference, OpenMP over particles)
-    on the same workload, timed on this host."""
+    on the N=20,000 aggregate, timed on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-    contact, sinter, ham = aggregate_params()
-    p = O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R_PART, contact=contact,
-                        sinter=sinter, hamaker=ham)
+    c, s, h = aggregate_params()
+    p = O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R_PART, contact=c, sinter=s,
+                        hamaker=h)
     osys = O.OracleSystem(p, x, v0)
-    osys.step(DT, 20, rebuild_every=50)  # warm-up
+    osys.step(DT, 20, rebuild_every=REBUILD)
     t0 = time.perf_counter()
-    osys.step(DT, steps, rebuild_every=50)
+    osys.step(DT, steps, rebuild_every=REBUILD)
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": x.shape[0] * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"same N={x.shape[0]} aggregate and parameters, {steps} steps after 20 warm-up steps, "
-                      f"list rebuilt every 50 steps (cell list), OpenMP over particles, {el:.1f} s"}
+    return {"value": round(x.shape[0] * steps / el, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"N=20,000 aggregate of the batch's parameters (seed 1, Df 1.8), {steps} FP64 steps after 20 "
+                      f"warm-up steps, list rebuilt every 50 steps, OpenMP over particles: {el:.1f} s; the batch "
+                      "is independent aggregates, so its CPU rate per particle-step is the same"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--n", type=int, default=20000)
-    ap.add_argument("--cpu-steps", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--cpu-steps", type=int, default=6000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-events", action="store_true", help="time without per-launch HIP events")
+    ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (one aggregate per step "
-              "is a single-GPU workload; see the sharding notes of the design document)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the batch shards by aggregate "
+              "across GPUs with no communication; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % REBUILD or args.warmup % REBUILD:
+        print("--steps and --warmup must be multiples of 50 (the rebuild interval)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    # torch first: it bundles its own HIP runtime library with the same soname
+    # as ROCm's, and the first one loaded serves the whole process.
+    import torch  # noqa: F401  (torch.cuda.synchronize brackets the timed regions)
     import libgran_amd as L
 
-    x, v0 = workload(args.n)
-    contact, sinter, ham = aggregate_params()
-    params = L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST,
-                           r_verlet=3.2 * R_PART, contact=contact, sinter=sinter, hamaker=ham)
-    sysm = L.System(params, x, v0)
-    sysm.step(DT, args.warmup, rebuild_every=50)
-    torch.cuda.synchronize()
+    t_gen = time.perf_counter()
+    sizes, xs, vs = batch_workload(L, args.count)
+    t_gen = time.perf_counter() - t_gen
+    total = int(sizes.sum())
 
-    if not args.no_events:
-        L._abi.check(L._abi.lib().hg_profile_enable(sysm._h, 1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sysm.step(DT, args.steps, rebuild_every=50)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    value = args.n * args.steps / elapsed
-
-    launches = np.zeros(4, dtype=np.uint64)
-    dev_ms = np.zeros(4)
-    L._abi.check(L._abi.lib().hg_profile_read(sysm._h, launches.ctypes.data_as(L._abi._u64p),
-                                              dev_ms.ctypes.data_as(L._abi._dp)))
-    off, idx = sysm.neighbors()
-    kbar = len(idx) / args.n
-    cbar = (len(sysm.springs(0)[0]) + len(sysm.springs(1)[0])) / args.n
-    roofline = None
-    if launches[0] > 0:
-        avg_ms = dev_ms[0] / launches[0]
-        bytes_launch = force_bytes_per_particle(kbar, cbar) * args.n
-        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                    "kernel": "k_force_nl<SINTER,LIMITED> (acceleration pass)",
-                    "avg_launch_us": round(avg_ms * 1e3, 3), "launches": int(launches[0]),
-                    "alg_bytes_per_launch": int(bytes_launch), "kbar": round(kbar, 3), "cbar": round(cbar, 3),
-                    "phase_us_per_step": {k: round(float(dev_ms[i]) * 1e3 / args.steps, 3)
-                                          for i, k in enumerate(("force", "kick", "rebuild"))}}
+    pool, el_mx, prof_mx, kbar, cbar = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps, args.warmup)
+    pool.close()
+    value = total * args.steps / el_mx
+    chunk_us = prof_mx["chunk_ms"] * 1e3 / max(prof_mx["chunks"], 1)
+    bpps = state_bytes_per_particle_step(kbar, cbar)
+    bytes_launch = bpps * total * REBUILD
+    achieved = bytes_launch / (chunk_us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate per launch)",
+                "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
+                "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
+                "kbar": round(kbar, 3), "cbar": round(cbar, 3),
+                "rebuild_us_per_interval": round(prof_mx["rebuild_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1)}
+    extra = {"aggregates": len(xs), "particles": total, "generation_s": round(t_gen, 1)}
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x, v0, args.cpu_steps)
-
+    if not args.only_batched:
+        pool, el_64, prof_64, _, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup)
+        pool.close()
+        plain, plain_note = plain_fp64_sample(L, sizes, xs, vs, 200)
+        single64, x20, v20, launches, ms = single_aggregate(L, L.PRECISION_FP64, 5000, 200)
+        single_mx = single_aggregate(L, L.PRECISION_MIXED, 5000, 200)[0]
+        extra.update({
+            "batched_fp64_pool": round(total * args.steps / el_64, 1),
+            "plain_fp64_gpu": round(plain, 1), "plain_fp64_gpu_sample": plain_note,
+            "batched_mixed_vs_plain_fp64": round(value / plain, 1),
+            "batched_fp64_vs_plain_fp64": round(total * args.steps / el_64 / plain, 1),
+            "single_n20000_fp64": round(single64, 1), "single_n20000_mixed": round(single_mx, 1),
+            "single_n20000_force_kernel_us": round(float(ms[0]) * 1e3 / max(int(launches[0]), 1), 2),
+        })
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(x20, v20, args.cpu_steps)
     out = {"metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 5),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"single aggregate N={args.n} (seed 1, Df 1.8, kf 1.3), alt_sinter + contact + "
-                                  "limited-range Hamaker (3r), neighbour list 3.2r rebuilt every 50 steps, "
-                                  "dt 1e-13 s, FP64",
-                      "particles": args.n, "rebuild_every": 50, "precision": "fp64"},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_mx * 1e3 / args.steps, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"{len(xs)} independent soot aggregates ({total} particles; N log-normal median "
+                                  "400 sigma 1 clipped 50..20000, Df U(1.6,2.2), seeds 0..4095), alt_sinter + "
+                                  "contact + limited-range Hamaker, neighbour lists 3.2r rebuilt every 50 steps, "
+                                  "dt 1e-13 s, mixed precision (FP32 forces, FP64 state/springs), one pool",
+                      "rebuild_every": REBUILD, "precision": "mixed"},
+           "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
     print(json.dumps(out))
 
 

@@ -239,6 +239,12 @@ hg_status hg_profile_read(hg_system *sys, uint64_t *launches, double *device_ms)
  * radius r_part (fractal dimension df, prefactor kf). x_out = 3*n doubles.
  * Not in the reference: the input generator of the aggregate workloads. */
 hg_status hg_generate_aggregate(size_t n, double r_part, double df, double kf, uint64_t seed, double *x_out);
+/* count aggregates at once on host threads (threads <= 0: all cores);
+ * aggregate k has n[k] particles, dimension df[k], seed seeds[k]; positions
+ * are concatenated in x_out (3 * sum(n) doubles). Each equals
+ * hg_generate_aggregate with the same arguments. */
+hg_status hg_generate_aggregates(size_t count, const uint32_t *n, const double *df, double kf, const uint64_t *seeds,
+                                 double r_part, double *x_out, int threads);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,18 @@ def generate_aggregate(n, r_part, df, kf, seed):
     return x
 
 
+def generate_aggregates(sizes, dfs, kf, seeds, r_part, threads=0):
+    """Many seeded TD-CCA aggregates on host threads; returns a list of (n_k, 3)
+    arrays, each identical to generate_aggregate(n_k, r_part, df_k, kf, seed_k)."""
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+    dfs = np.ascontiguousarray(dfs, dtype=np.float64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    x = np.zeros((int(sizes.sum()), 3))
+    _abi.check(_abi.lib().hg_generate_aggregates(len(sizes), _abi.u32ptr(sizes), _abi.dptr(dfs), kf,
+                                                 seeds.ctypes.data_as(_abi._u64p), r_part, _abi.dptr(x), threads))
+    return np.split(x, np.cumsum(sizes)[:-1])
+
+
 def sinter_topology(x0, r_part, d_crit):
     """Bonds (i<j, reference insertion order) and skipped-cycle count (host only)."""
     x0 = _abi.c64(x0)

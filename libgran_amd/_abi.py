@@ -83,7 +83,8 @@ SIGNATURES = {
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
-    "hg_pool_create": (C.c_int, [C.POINTER(Params), C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(_sysp)]),
+    "hg_generate_aggregates": (C.c_int, [C.c_size_t, _u32p, _dp, C.c_double, _u64p, C.c_double, _dp, C.c_int]),
+    "hg_pool_create":(C.c_int, [C.POINTER(Params), C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(_sysp)]),
     "hg_pool_destroy": (C.c_int, [_sysp]),
     "hg_pool_add": (C.c_int, [_sysp, C.c_size_t, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]),
     "hg_pool_remove": (C.c_int, [_sysp, C.c_int32]),

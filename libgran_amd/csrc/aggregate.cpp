@@ -15,9 +15,11 @@
 // from mt19937_64 with explicit transforms, so a seed gives the same aggregate
 // on every platform.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <random>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -141,11 +143,47 @@ cluster merge(const cluster &A, const cluster &B, double r, double df, double kf
 }  // namespace
 }  // namespace hg
 
+namespace hg {
+namespace {
+void generate_one(size_t n, double r_part, double df, double kf, uint64_t seed, double *x_out);
+}
+}  // namespace hg
+
 extern "C" hg_status hg_generate_aggregate(size_t n, double r_part, double df, double kf, uint64_t seed,
                                            double *x_out) {
     using namespace hg;
     if (n == 0 || x_out == nullptr || !(r_part > 0.0) || !(df > 0.0) || !(kf > 0.0))
         return fail(HG_ERR_INVALID, "hg_generate_aggregate: need n > 0, r_part > 0, df > 0, kf > 0 and x_out");
+    generate_one(n, r_part, df, kf, seed, x_out);
+    return HG_OK;
+}
+
+extern "C" hg_status hg_generate_aggregates(size_t count, const uint32_t *n, const double *df, double kf,
+                                            const uint64_t *seeds, double r_part, double *x_out, int threads) {
+    using namespace hg;
+    if (count == 0) return HG_OK;
+    if (!n || !df || !seeds || !x_out || !(r_part > 0.0) || !(kf > 0.0))
+        return fail(HG_ERR_INVALID, "hg_generate_aggregates: invalid arguments");
+    std::vector<size_t> off(count + 1, 0);
+    for (size_t k = 0; k < count; k++) {
+        if (n[k] == 0 || !(df[k] > 0.0)) return fail(HG_ERR_INVALID, "hg_generate_aggregates: empty aggregate or df <= 0");
+        off[k + 1] = off[k] + n[k];
+    }
+    unsigned nt = threads > 0 ? unsigned(threads) : std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++)
+        pool.emplace_back([&] {
+            for (size_t k; (k = next.fetch_add(1)) < count;)
+                generate_one(n[k], r_part, df[k], kf, seeds[k], x_out + 3 * off[k]);
+        });
+    for (auto &th : pool) th.join();
+    return HG_OK;
+}
+
+namespace hg {
+namespace {
+void generate_one(size_t n, double r_part, double df, double kf, uint64_t seed, double *x_out) {
     rng R(seed);
     std::vector<cluster> cl(n);
     for (auto &c : cl) c.p.assign(1, zero3());
@@ -162,5 +200,6 @@ extern "C" hg_status hg_generate_aggregate(size_t n, double r_part, double df, d
         x_out[3 * i + 1] = cl[0].p[i].y;
         x_out[3 * i + 2] = cl[0].p[i].z;
     }
-    return HG_OK;
 }
+}  // namespace
+}  // namespace hg

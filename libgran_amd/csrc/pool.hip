@@ -330,6 +330,7 @@ struct hg_pool {
     uint32_t *order_d = nullptr, *counter_d = nullptr;
     std::vector<uint32_t> order;
     bool order_dirty = true;
+    bool meta_dirty = true;
     uint32_t hash_size = 0, *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr,
              *bucket_fill = nullptr, *cell_items = nullptr, *scan_tmp = nullptr;
     int *flags = nullptr, *flags_host = nullptr;
@@ -418,6 +419,7 @@ hg_status valid_slot(hg_pool *Q, int32_t slot) {
 hg_status upload_meta(hg_pool *Q) {
     HG_HIP(hipMemcpyAsync(Q->meta_d, Q->meta.data(), Q->S * sizeof(pool_meta), hipMemcpyHostToDevice, Q->stream));
     HG_HIP(hipStreamSynchronize(Q->stream));
+    Q->meta_dirty = false;
     return HG_OK;
 }
 
@@ -653,7 +655,7 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
     Q->bonds[slot] = std::move(bonds);
     Q->skipped[slot] = skipped;
     Q->order_dirty = true;
-    PG_TRY(upload_meta(Q));
+    Q->meta_dirty = true;
     *slot_out = slot;
     return HG_OK;
 }
@@ -688,7 +690,8 @@ hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
     // lower the high-water mark when the top range is free
     auto last = std::prev(Q->free_ranges.end());
     if (last->first + last->second == Q->P) Q->high_water = std::min<uint32_t>(Q->high_water, last->first);
-    return upload_meta(Q);
+    Q->meta_dirty = true;
+    return HG_OK;
 }
 
 hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *verts, const double *v_facet) {
@@ -715,7 +718,8 @@ hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *
     HG_HIP(hipStreamSynchronize(Q->stream));
     Q->meta[slot].nf = uint32_t(nf);
     Q->meta[slot].acc_valid = 0;
-    return upload_meta(Q);
+    Q->meta_dirty = true;
+    return HG_OK;
 }
 
 hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
