@@ -156,7 +156,18 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (known traffic for scaling HBM counters)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": 1 << 30, "calib_copy_bytes_written": 1 << 30, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the batch shards by aggregate "
               "across GPUs with no communication; see the design notes)", file=sys.stderr)

@@ -36,6 +36,9 @@ __device__ __forceinline__ void st_spring(double *b, size_t stride, const spring
 // partner loop, writing the new table. Entries not visited this pass (pairs
 // that are not listed) are copied unchanged: the reference leaves the dense
 // spring table untouched for pairs it does not evaluate.
+// The first kTagPrefetch tags of the old row are loaded together with the
+// count (the table capacity is at least that many slots), so walking the
+// usual one to four entries needs no dependent load.
 struct walker {
     const uint32_t *pj_old;
     const double *ps_old;
@@ -43,6 +46,7 @@ struct walker {
     double *ps_new;
     size_t stride;
     uint32_t i, cnt_old, p, q, cap, ej;
+    uint32_t pre[kTagPrefetch];
 
     __device__ void init(const pair_table &T, uint32_t i_) {
         pj_old = T.pj_old;
@@ -53,13 +57,21 @@ struct walker {
         i = i_;
         cap = T.cap;
         cnt_old = T.pcnt_old[i];
+#pragma unroll
+        for (uint32_t s = 0; s < kTagPrefetch; s++) pre[s] = pj_old[size_t(s) * stride + i];
         p = 0;
         q = 0;
-        ej = cnt_old > 0 ? pj_old[i] : kNoEntry;
+        ej = cnt_old > 0 ? pre[0] : kNoEntry;
+    }
+    __device__ uint32_t tag_at(uint32_t s) const {
+        uint32_t t = pre[0];
+#pragma unroll
+        for (uint32_t k = 1; k < kTagPrefetch; k++) t = s == k ? pre[k] : t;
+        return s < kTagPrefetch ? t : pj_old[size_t(s) * stride + i];
     }
     __device__ void next() {
         p++;
-        ej = p < cnt_old ? pj_old[size_t(p) * stride + i] : kNoEntry;
+        ej = p < cnt_old ? tag_at(p) : kNoEntry;
     }
     __device__ spring9 load() const { return ld_spring(ps_old + size_t(p) * 9 * stride + i, stride); }
     __device__ void store(uint32_t tag, const spring9 &s) {
@@ -89,6 +101,30 @@ struct walker {
             store(ej, load());
             next();
         }
+    }
+};
+
+// A particle's listed partners; the first kIdPrefetch slots are loaded
+// together with the count (slots below the list capacity only).
+struct id_list {
+    const uint32_t *nb;
+    size_t stride;
+    uint32_t i, n;
+    uint32_t pre[kIdPrefetch];
+
+    __device__ void init(const uint32_t *nb_, const uint32_t *ncnt, size_t stride_, uint32_t i_, uint32_t cap) {
+        nb = nb_;
+        stride = stride_;
+        i = i_;
+        n = ncnt[i];
+#pragma unroll
+        for (uint32_t k = 0; k < kIdPrefetch; k++) pre[k] = k < cap ? nb[size_t(k) * stride + i] : 0u;
+    }
+    __device__ uint32_t at(uint32_t k) const {
+        uint32_t t = pre[0];
+#pragma unroll
+        for (uint32_t s = 1; s < kIdPrefetch; s++) t = k == s ? pre[s] : t;
+        return k < kIdPrefetch ? t : nb[size_t(k) * stride + i];
     }
 };
 

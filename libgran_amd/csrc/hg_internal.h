@@ -23,6 +23,12 @@ constexpr uint32_t kNoEntry = 0xffffffffu;  // (kNoEntry & kIdxMask) exceeds eve
 enum : int { FLAG_NL_OVERFLOW = 0, FLAG_PAIR_OVERFLOW = 1, FLAG_NL_MAXCOUNT = 2, FLAG_PAIR_MAXCOUNT = 3, FLAG_COUNT = 4 };
 
 constexpr int kMaxFacets = 16;
+// Pair-state tags loaded up front per particle (the table capacity is never
+// below this) and neighbour ids loaded up front per particle. Deeper
+// prefetch arrays are indexed at run time and land in scratch memory, which
+// costs more than the dependent loads they save.
+constexpr uint32_t kTagPrefetch = 1;
+constexpr uint32_t kIdPrefetch = 1;
 
 // Pair-state table, old (read) and new (written) halves; slot s of particle i
 // at s*stride + i (spring component c at (s*9 + c)*stride + i).
@@ -45,6 +51,7 @@ struct sys_view {
     // neighbour list (transposed: slot k of particle i at k*n + i)
     const uint32_t *nb;
     const uint32_t *ncnt;
+    uint32_t cap_n;
     pair_table pt;
     // facets
     const facet_dev *facets;

@@ -33,10 +33,10 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     acc6 acc{zero3(), zero3()};
     walker W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
-    const uint32_t nn = S.ncnt[i];
-    const uint32_t *nbi = S.nb + i;
-    for (uint32_t k = 0; k < nn; k++) {
-        const uint32_t j = nbi[size_t(k) * S.n];
+    id_list ids;
+    ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
+    for (uint32_t k = 0; k < ids.n; k++) {
+        const uint32_t j = ids.at(k);
         pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(S.x, j), ld3(S.v, j), ld3(S.w, j), W, acc);
     }
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
@@ -54,10 +54,10 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     accf acc{zerof(), zerof()};
     walker W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
-    const uint32_t nn = S.ncnt[i];
-    const uint32_t *nbi = S.nb + i;
-    for (uint32_t k = 0; k < nn; k++) {
-        const uint32_t j = nbi[size_t(k) * S.n];
+    id_list ids;
+    ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
+    for (uint32_t k = 0; k < ids.n; k++) {
+        const uint32_t j = ids.at(k);
         pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(S.x, j), ld3(S.v, j), ld3(S.w, j), W, acc);
     }
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);

@@ -39,7 +39,7 @@ struct pool_view {
     const uint32_t *ncnt;
     uint32_t *pj0, *pj1, *pcnt0, *pcnt1;
     double *ps0, *ps1;
-    uint32_t cap_s;
+    uint32_t cap_s, cap_n;
     facet_dev *facets;
     uint32_t max_facets;
     double *ss;
@@ -50,7 +50,73 @@ struct pool_view {
 
 namespace {
 
-template <int BIN, int HAM, int PAIRS, int PREC>
+// Where a running system's particle state lives during a chunk (chosen per
+// system from its size and the workgroup's LDS; local index g - offset):
+//   STATE_FULL   all of x, v, omega, theta, a, alpha in LDS (18 doubles per
+//                particle), read from global memory at the start of the
+//                chunk and written back once at its end;
+//   STATE_MIRROR x, v, omega copied into LDS (9 doubles per particle) and
+//                refreshed by each owner lane in the kick/drift phase, so
+//                partner gathers read LDS; the state itself stays global;
+//   STATE_GLOBAL everything in global memory.
+enum { STATE_GLOBAL = 0, STATE_MIRROR = 1, STATE_FULL = 2 };
+extern __shared__ double pool_lds[];
+constexpr uint32_t kMirrorStride = 9, kFullStride = 18;
+
+template <int MODE>
+__device__ __forceinline__ double *lds_rec(uint32_t offset, uint32_t g) {
+    return pool_lds + size_t(g - offset) * (MODE == STATE_FULL ? kFullStride : kMirrorStride);
+}
+__device__ __forceinline__ d3 lds3(const double *q) { return d3{q[0], q[1], q[2]}; }
+__device__ __forceinline__ void sds3(double *q, d3 a) {
+    q[0] = a.x;
+    q[1] = a.y;
+    q[2] = a.z;
+}
+
+template <int MODE>
+__device__ __forceinline__ void load_state(const pool_view &V, uint32_t offset, uint32_t g, d3 &x, d3 &v, d3 &w) {
+    if constexpr (MODE == STATE_GLOBAL) {
+        x = ld3(V.x, g);
+        v = ld3(V.v, g);
+        w = ld3(V.w, g);
+    } else {
+        const double *q = lds_rec<MODE>(offset, g);
+        x = lds3(q);
+        v = lds3(q + 3);
+        w = lds3(q + 6);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void store_acc(const pool_view &V, uint32_t offset, uint32_t g, d3 a, d3 al) {
+    if constexpr (MODE == STATE_FULL) {
+        double *q = lds_rec<MODE>(offset, g);
+        sds3(q + 12, a);
+        sds3(q + 15, al);
+    } else {
+        st3(V.a, g, a);
+        st3(V.al, g, al);
+    }
+}
+
+// Fills the LDS records of a system at the start of its chunk.
+template <int MODE>
+__device__ __forceinline__ void lds_fill(const pool_view &V, const pool_meta &M) {
+    for (uint32_t g = M.offset + threadIdx.x; g < M.offset + M.n; g += blockDim.x) {
+        double *q = lds_rec<MODE>(M.offset, g);
+        sds3(q, ld3(V.x, g));
+        sds3(q + 3, ld3(V.v, g));
+        sds3(q + 6, ld3(V.w, g));
+        if constexpr (MODE == STATE_FULL) {
+            sds3(q + 9, ld3(V.th, g));
+            sds3(q + 12, ld3(V.a, g));
+            sds3(q + 15, ld3(V.al, g));
+        }
+    }
+}
+
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                                  const pool_meta &M, uint32_t cur, const facet_dev *F) {
     pair_table T;
@@ -64,85 +130,160 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
     T.stride = V.stride;
     const uint32_t end = M.offset + M.n;
     for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
-        const d3 xi = ld3(V.x, g), vi = ld3(V.v, g), wi = ld3(V.w, g);
+        d3 xi, vi, wi;
+        load_state<LDS>(V, M.offset, g, xi, vi, wi);
         walker W;
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
+        id_list ids;
+        if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                const uint32_t nn = V.ncnt[g];
-                for (uint32_t k = 0; k < nn; k++) {
-                    const uint32_t j = V.nb[size_t(k) * V.stride + g];
-                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                for (uint32_t k = 0; k < ids.n; k++) {
+                    const uint32_t j = ids.at(k);
+                    d3 xj, vj, wj;
+                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
+                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
-                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                    d3 xj, vj, wj;
+                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
+                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags);
             if (M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
-            st3(V.a, g, acc.lin);
-            st3(V.al, g, acc.ang);
+            store_acc<LDS>(V, M.offset, g, acc.lin, acc.ang);
         } else {
             accf acc{zerof(), zerof()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                const uint32_t nn = V.ncnt[g];
-                for (uint32_t k = 0; k < nn; k++) {
-                    const uint32_t j = V.nb[size_t(k) * V.stride + g];
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                for (uint32_t k = 0; k < ids.n; k++) {
+                    const uint32_t j = ids.at(k);
+                    d3 xj, vj, wj;
+                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, ld3(V.x, j), ld3(V.v, j), ld3(V.w, j), W, acc);
+                    d3 xj, vj, wj;
+                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags);
             if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
-            st3(V.a, g, tod(acc.lin));
-            st3(V.al, g, tod(acc.ang));
+            store_acc<LDS>(V, M.offset, g, tod(acc.lin), tod(acc.ang));
         }
     }
 }
 
 // Kick (completing the previous step when `pending`), kick, drift; the same
 // operations in the same order as k_kick_drift.
+template <int LDS>
 __device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_meta &M, bool pending, double half,
                                                 double dt) {
     const uint32_t end = M.offset + M.n;
     for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
-        d3 vi = ld3(V.v, g), wi = ld3(V.w, g);
-        const d3 ai = ld3(V.a, g), ali = ld3(V.al, g);
+        double *q = lds_rec<LDS>(M.offset, g);
+        d3 vi, wi, ai, ali, x0, th0;
+        if constexpr (LDS == STATE_FULL) {
+            vi = lds3(q + 3); wi = lds3(q + 6); ai = lds3(q + 12); ali = lds3(q + 15); x0 = lds3(q); th0 = lds3(q + 9);
+        } else {
+            vi = ld3(V.v, g); wi = ld3(V.w, g); ai = ld3(V.a, g); ali = ld3(V.al, g); x0 = ld3(V.x, g);
+            th0 = ld3(V.th, g);
+        }
         if (pending) {
             vi = vi + ai * half;
             wi = wi + ali * half;
         }
         vi = vi + ai * half;
         wi = wi + ali * half;
-        st3(V.x, g, ld3(V.x, g) + vi * dt);
-        st3(V.th, g, ld3(V.th, g) + wi * dt);
-        st3(V.v, g, vi);
-        st3(V.w, g, wi);
+        const d3 xi = x0 + vi * dt;
+        const d3 thi = th0 + wi * dt;
+        if constexpr (LDS != STATE_FULL) {
+            st3(V.x, g, xi);
+            st3(V.th, g, thi);
+            st3(V.v, g, vi);
+            st3(V.w, g, wi);
+        }
+        if constexpr (LDS != STATE_GLOBAL) {
+            sds3(q, xi);
+            sds3(q + 3, vi);
+            sds3(q + 6, wi);
+        }
+        if constexpr (LDS == STATE_FULL) sds3(q + 9, thi);
     }
 }
 
+// Final kick; a system kept wholly in LDS is written back here.
+template <int LDS>
 __device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_meta &M, double half) {
     const uint32_t end = M.offset + M.n;
     for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
-        st3(V.v, g, ld3(V.v, g) + ld3(V.a, g) * half);
-        st3(V.w, g, ld3(V.w, g) + ld3(V.al, g) * half);
+        if constexpr (LDS == STATE_FULL) {
+            const double *q = lds_rec<LDS>(M.offset, g);
+            const d3 a = lds3(q + 12), al = lds3(q + 15);
+            st3(V.x, g, lds3(q));
+            st3(V.v, g, lds3(q + 3) + a * half);
+            st3(V.w, g, lds3(q + 6) + al * half);
+            st3(V.th, g, lds3(q + 9));
+            st3(V.a, g, a);
+            st3(V.al, g, al);
+        } else {
+            st3(V.v, g, ld3(V.v, g) + ld3(V.a, g) * half);
+            st3(V.w, g, ld3(V.w, g) + ld3(V.al, g) * half);
+        }
     }
 }
 
+// All steps of one rebuild interval for one system, by one workgroup.
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS>
+__device__ __forceinline__ void run_system(const hg_params &P, const mixed_consts &MC, const pool_view &V,
+                                           const pool_meta &M, facet_dev *F, uint32_t nsteps, double dt) {
+    const double half = dt / 2.0;
+    uint32_t cur = M.cur;
+    if constexpr (LDS != STATE_GLOBAL) {
+        lds_fill<LDS>(V, M);
+        __syncthreads();
+    }
+    if (!M.acc_valid) {
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F);
+        cur ^= 1u;
+        __syncthreads();
+    }
+    for (uint32_t s = 0; s < nsteps; s++) {
+        pool_kick_drift<LDS>(V, M, s > 0, half, dt);
+        if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
+        __syncthreads();
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F);
+        cur ^= 1u;
+        __syncthreads();
+    }
+    if (nsteps > 0) {
+        pool_final_kick<LDS>(V, M, half);
+        if (threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
+    } else if constexpr (LDS == STATE_FULL) {
+        // only the initial pass ran: its accelerations go back to global
+        for (uint32_t g = M.offset + threadIdx.x; g < M.offset + M.n; g += blockDim.x) {
+            const double *q = lds_rec<LDS>(M.offset, g);
+            st3(V.a, g, lds3(q + 12));
+            st3(V.al, g, lds3(q + 15));
+        }
+    }
+    __syncthreads();
+}
+
 // One rebuild interval of every queued system. Exit: every workgroup leaves
-// once the shared counter passes `count`.
+// once the shared counter passes `count`. lds_bytes of dynamic LDS per
+// workgroup decide each system's state placement.
 template <int BIN, int HAM, int PAIRS, int PREC>
 __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
-                                                    uint32_t count, uint32_t *counter, uint32_t nsteps, double dt) {
+                                                    uint32_t count, uint32_t *counter, uint32_t nsteps, double dt,
+                                                    uint32_t lds_bytes) {
     __shared__ uint32_t s_item;
-    const double half = dt / 2.0;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -152,25 +293,13 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
         const uint32_t slot = order[item];
         const pool_meta M = V.meta[slot];
         facet_dev *F = V.facets + size_t(slot) * V.max_facets;
-        uint32_t cur = M.cur;
-        if (!M.acc_valid) {
-            pool_force_phase<BIN, HAM, PAIRS, PREC>(P, MC, V, M, cur, F);
-            cur ^= 1u;
-            __syncthreads();
-        }
-        for (uint32_t s = 0; s < nsteps; s++) {
-            pool_kick_drift(V, M, s > 0, half, dt);
-            if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
-            __syncthreads();
-            pool_force_phase<BIN, HAM, PAIRS, PREC>(P, MC, V, M, cur, F);
-            cur ^= 1u;
-            __syncthreads();
-        }
-        if (nsteps > 0) {
-            pool_final_kick(V, M, half);
-            if (threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
-        }
-        __syncthreads();
+        const size_t need = size_t(M.n) * sizeof(double);
+        if (need * kFullStride <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_FULL>(P, MC, V, M, F, nsteps, dt);
+        else if (need * kMirrorStride <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR>(P, MC, V, M, F, nsteps, dt);
+        else
+            run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL>(P, MC, V, M, F, nsteps, dt);
     }
 }
 
@@ -272,25 +401,39 @@ __global__ void k_zero_rows_f64(double *p, size_t stride, uint32_t offset, uint3
 
 inline unsigned nblk(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
+template <int BIN, int HAM, int PAIRS, int PREC>
+hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
+                          hipStream_t s) {
+    const size_t lds_bytes = lds_cap;
+    if (lds_bytes > 0) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pool_chunk<BIN, HAM, PAIRS, PREC>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+        if (e != hipSuccess) return e;
+    }
+    k_pool_chunk<BIN, HAM, PAIRS, PREC><<<grid, 256, lds_bytes, s>>>(p, make_mixed_consts(p), V, order, count, counter,
+                                                                      nsteps, dt, lds_cap);
+    return hipGetLastError();
+}
+
 template <int BIN, int HAM, int PAIRS>
 hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
-                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, hipStream_t s) {
-    const mixed_consts mc = make_mixed_consts(p);
+                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
+                          hipStream_t s) {
     if (p.precision == HG_PRECISION_MIXED)
-        k_pool_chunk<BIN, HAM, PAIRS, HG_PRECISION_MIXED><<<grid, 256, 0, s>>>(p, mc, V, order, count, counter, nsteps,
-                                                                              dt);
-    else
-        k_pool_chunk<BIN, HAM, PAIRS, HG_PRECISION_FP64><<<grid, 256, 0, s>>>(p, mc, V, order, count, counter, nsteps,
-                                                                             dt);
-    return hipGetLastError();
+        return launch_chunk_k<BIN, HAM, PAIRS, HG_PRECISION_MIXED>(p, V, order, count, counter, nsteps, dt, grid,
+                                                                   lds_cap, s);
+    return launch_chunk_k<BIN, HAM, PAIRS, HG_PRECISION_FP64>(p, V, order, count, counter, nsteps, dt, grid, lds_cap,
+                                                              s);
 }
 
 template <int PAIRS>
 hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
-                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, hipStream_t s) {
+                          uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
+                          hipStream_t s) {
 #define HG_PCASE(B, H) \
     if (p.binary_model == B && p.hamaker_model == H) \
-        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, s);
+        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, s);
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
@@ -341,6 +484,8 @@ struct hg_pool {
     double last_chunk_ms = 0.0, last_rebuild_ms = 0.0;
     uint64_t last_chunks = 0;
     int grid = 0;
+    uint32_t lds_cap = 0;
+    size_t lds_bytes_per_wg = 0;
 };
 
 namespace {
@@ -374,7 +519,7 @@ uint32_t pool_cap_n(const hg_params &p) {
 
 uint32_t pool_cap_s(const hg_params &p) {
     if (p.binary_model == HG_BINARY_NONE) return 0;
-    return p.pair_capacity ? p.pair_capacity : 24u;
+    return std::max<uint32_t>(p.pair_capacity ? p.pair_capacity : 24u, kTagPrefetch);
 }
 
 hg_status check_pool_flags(hg_pool *Q, const char *where) {
@@ -401,6 +546,7 @@ pool_view make_pool_view(hg_pool *Q) {
     V.pcnt0 = Q->pcnt[0]; V.pcnt1 = Q->pcnt[1];
     V.ps0 = Q->ps[0]; V.ps1 = Q->ps[1];
     V.cap_s = Q->cap_s;
+    V.cap_n = Q->cap_n;
     V.facets = Q->facets;
     V.max_facets = uint32_t(Q->MF);
     V.ss = Q->ss;
@@ -511,7 +657,15 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     Q->P = max_particles;
     Q->S = max_systems;
     Q->MF = max_facets;
+    // Two workgroups per CU (the chunk kernel's register use allows two waves
+    // per SIMD); each gets half the CU's LDS for its mirror.
     Q->grid = prop.multiProcessorCount * 2;
+    {
+        size_t per_wg = std::min<size_t>(prop.sharedMemPerBlock, prop.maxSharedMemoryPerMultiProcessor / 2);
+        per_wg = per_wg > 1024 ? per_wg - 1024 : 0;  // room for the static LDS words
+        Q->lds_cap = uint32_t(per_wg);  // bytes of dynamic LDS per workgroup
+        Q->lds_bytes_per_wg = per_wg;
+    }
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream, hipStreamNonBlocking));
     for (auto &e : Q->ev) PG_HIPC(hipEventCreate(&e));
     const size_t P = Q->P;
@@ -773,9 +927,9 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
         HG_HIP(launch_zero_u32(Q->counter_d, 1, Q->stream));
         HG_HIP(hipEventRecord(Q->ev[2], Q->stream));
         hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt,
-                                                                   grid, Q->stream)
+                                                                   grid, Q->lds_cap, Q->stream)
                           : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, grid,
-                                                         Q->stream);
+                                                         Q->lds_cap, Q->stream);
         HG_HIP(e);
         HG_HIP(hipEventRecord(Q->ev[3], Q->stream));
         PG_TRY(check_pool_flags(Q, "hg_pool_step"));

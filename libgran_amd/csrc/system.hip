@@ -163,6 +163,7 @@ sys_view make_view(hg_system *S) {
     v.al = S->al;
     v.nb = S->nb;
     v.ncnt = S->ncnt;
+    v.cap_n = S->cap_n;
     v.pt.pj_old = S->pj[S->cur];
     v.pt.ps_old = S->ps[S->cur];
     v.pt.pcnt_old = S->pcnt[S->cur];
@@ -274,6 +275,7 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
     for (auto &r : rows) max_bonds = std::max(max_bonds, r.size());
     if (S->p.binary_model != HG_BINARY_NONE) {
         S->cap_s = S->p.pair_capacity ? S->p.pair_capacity : uint32_t((max_bonds + 24 + 3) / 4 * 4);
+        S->cap_s = std::max<uint32_t>(S->cap_s, kTagPrefetch);
         if (S->cap_s < max_bonds)
             return cleanup_on(fail(HG_ERR_CAPACITY, "pair_capacity is below the largest bond count " +
                                                         std::to_string(max_bonds)));
