@@ -81,6 +81,10 @@ typedef struct hg_params {
     int32_t facet_hamaker;       /* facets carry a surface_hamaker_functor */
     uint32_t neighbor_capacity;  /* max listed neighbours per particle; 0 = derived from r_verlet */
     uint32_t pair_capacity;      /* max bonded + touching pairs per particle; 0 = default */
+    int32_t facet_advance;       /* 0: facets advance by v_facet*dt after every step;
+                                    1: only through hg_advance_facet (the driver moves them,
+                                    as triangular_facet::update_positions does) */
+    int32_t reserved;
     double r_verlet;             /* neighbour-list radius (granular_system_neighbor_list.h:54) */
     hg_contact_params contact;           /* contact functor (also the one alt_sinter embeds) */
     hg_sinter_params sinter;             /* alt_sinter bonded model + d_crit */
@@ -113,6 +117,10 @@ hg_status hg_destroy(hg_system *sys);
 hg_status hg_set_facets(hg_system *sys, size_t nf, const double *verts, const double *v_facet);
 /* Reads the facets' current vertices (nf*9) and velocities (nf*3). */
 hg_status hg_get_facets(hg_system *sys, double *verts, double *v_facet);
+/* Sets facet f's velocity to v_facet[3] and moves its vertices by
+ * v_facet*dt. Replaces: triangular_facet::update_positions(dt) with the
+ * driver's current v_facet (triangular_facet.h:62-69). */
+hg_status hg_advance_facet(hg_system *sys, size_t f, const double *v_facet, double dt);
 
 /* Rebuilds the neighbour list from the current positions.
  * Replaces: update_neighbor_list() of rotational_binary_system_neighbors_omp,
@@ -142,6 +150,10 @@ hg_status hg_set_state(hg_system *sys, const double *x, const double *v, const d
                        const double *omega);
 /* Last computed accelerations a (linear) and alpha (angular). */
 hg_status hg_get_accelerations(hg_system *sys, double *a, double *alpha);
+/* Sets both acceleration buffers to zero; the next step's first half kick
+ * uses these zeros. Replaces: reset_acceleration_buffers()
+ * (Granular-system.md:228-244). */
+hg_status hg_reset_accelerations(hg_system *sys);
 /* Simulation time and number of steps taken. */
 hg_status hg_get_time(hg_system *sys, double *t, uint64_t *step_index);
 

@@ -43,7 +43,8 @@ class HamakerParams(C.Structure):
 class Params(C.Structure):
     _fields_ = [("binary_model", C.c_int32), ("hamaker_model", C.c_int32), ("pair_mode", C.c_int32),
                 ("precision", C.c_int32), ("facet_contact", C.c_int32), ("facet_hamaker", C.c_int32),
-                ("neighbor_capacity", C.c_uint32), ("pair_capacity", C.c_uint32), ("r_verlet", C.c_double),
+                ("neighbor_capacity", C.c_uint32), ("pair_capacity", C.c_uint32), ("facet_advance", C.c_int32),
+                ("reserved", C.c_int32), ("r_verlet", C.c_double),
                 ("contact", ContactParams), ("sinter", SinterParams), ("hamaker", HamakerParams),
                 ("surface_contact", ContactParams), ("surface_hamaker", HamakerParams)]
 
@@ -68,6 +69,7 @@ SIGNATURES = {
     "hg_destroy": (C.c_int, [_sysp]),
     "hg_set_facets": (C.c_int, [_sysp, C.c_size_t, _dp, _dp]),
     "hg_get_facets": (C.c_int, [_sysp, _dp, _dp]),
+    "hg_advance_facet": (C.c_int, [_sysp, C.c_size_t, _dp, C.c_double]),
     "hg_rebuild_neighbors": (C.c_int, [_sysp]),
     "hg_compute_accelerations": (C.c_int, [_sysp]),
     "hg_step": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
@@ -75,6 +77,7 @@ SIGNATURES = {
     "hg_set_state": (C.c_int, [_sysp, _dp, _dp, _dp, _dp]),
     "hg_get_accelerations": (C.c_int, [_sysp, _dp, _dp]),
     "hg_get_time": (C.c_int, [_sysp, _dp, _u64p]),
+    "hg_reset_accelerations": (C.c_int, [_sysp]),
     "hg_get_neighbors": (C.c_int, [_sysp, _u64p, _u32p, C.c_size_t, _szp]),
     "hg_get_bonds": (C.c_int, [_sysp, _u32p, C.c_size_t, _szp, _szp]),
     "hg_get_springs": (C.c_int, [_sysp, C.c_int, _u32p, _dp, C.c_size_t, _szp]),

@@ -130,6 +130,7 @@ hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const 
 hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
                              facet_dev *facets, int nf, double facet_dt, hipStream_t s);
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
+hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
 // Exclusive scan of count words; tmp holds scan_tmp_words(count) words.
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t count, uint32_t *tmp, hipStream_t s);
 uint32_t scan_tmp_words(uint32_t count);

@@ -147,6 +147,14 @@ __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const 
     st3(w, i, ld3(w, i) + ld3(al, i) * half);
 }
 
+// triangular_facet::update_positions with the driver's velocity (one lane).
+__global__ void k_advance_facet(facet_dev *f, double vx, double vy, double vz, double dt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        f->v_facet = d3{vx, vy, vz};
+        for (int k = 0; k < 3; k++) f->vert[k] = f->vert[k] + f->v_facet * dt;
+    }
+}
+
 __global__ void k_zero_u32(uint32_t *p, size_t count) {
     for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < count; k += size_t(gridDim.x) * blockDim.x)
         p[k] = 0;
@@ -428,6 +436,11 @@ hipError_t launch_final_kick(double *v, double *w, const double *a, const double
                              facet_dev *facets, int nf, double facet_dt, hipStream_t s) {
     unsigned nb = blocks_for(n > 0 ? n : 1, 256);
     k_final_kick<<<nb, 256, 0, s>>>(v, w, a, al, n, half, facets, nf, facet_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s) {
+    k_advance_facet<<<1, 64, 0, s>>>(f, v[0], v[1], v[2], dt);
     return hipGetLastError();
 }
 

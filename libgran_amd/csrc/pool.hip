@@ -628,6 +628,8 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
         return fail(HG_ERR_INVALID, "hg_pool_create: unknown precision");
     if (params->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(params->r_verlet > 0.0))
         return fail(HG_ERR_INVALID, "hg_pool_create: neighbour-list mode needs r_verlet > 0");
+    if (params->facet_advance != 0)
+        return fail(HG_ERR_INVALID, "hg_pool_create: pools advance facets after every step (facet_advance = 0)");
     int dev = 0;
     HG_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;

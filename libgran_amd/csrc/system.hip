@@ -393,6 +393,13 @@ hg_status hg_get_facets(hg_system *S, double *verts, double *v_facet) {
     return HG_OK;
 }
 
+hg_status hg_advance_facet(hg_system *S, size_t f, const double *v_facet, double dt) {
+    if (!S || !v_facet) return fail(HG_ERR_INVALID, "system and v_facet are required");
+    if (f >= size_t(S->nf)) return fail(HG_ERR_INVALID, "no such facet");
+    HG_HIP(launch_advance_facet(S->facets + f, v_facet, dt, S->stream));
+    return HG_OK;
+}
+
 hg_status hg_rebuild_neighbors(hg_system *S) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
     if (S->p.pair_mode != HG_PAIRS_NEIGHBOR_LIST) return fail(HG_ERR_INVALID, "system has no neighbour list");
@@ -417,19 +424,20 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
     const double half = dt / 2.0;
     const uint32_t n = uint32_t(S->n);
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
+    const int nf_advance = S->p.facet_advance ? 0 : S->nf;  // facets the step moves itself
     for (uint64_t s = 0; s < nsteps; s++) {
         if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
             HG_HIP(timed(S, PHASE_REBUILD, [&] { return launch_neighbor_build(*S, S->stream); }));
         HG_HIP(timed(S, PHASE_KICK, [&] {
             return launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt, S->facets,
-                                     S->nf, dt, S->stream);
+                                     nf_advance, dt, S->stream);
         }));
         HG_TRY(force_pass(S));
         S->t = S->t + dt;
         S->step_index++;
     }
     HG_HIP(timed(S, PHASE_KICK, [&] {
-        return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, S->nf, dt, S->stream);
+        return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, nf_advance, dt, S->stream);
     }));
     return check_flags(S, "hg_step");
 }
@@ -457,6 +465,15 @@ hg_status hg_get_accelerations(hg_system *S, double *a, double *alpha) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
     HG_TRY(download3(S, S->a, a));
     HG_TRY(download3(S, S->al, alpha));
+    return HG_OK;
+}
+
+hg_status hg_reset_accelerations(hg_system *S) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_HIP(launch_zero_f64(S->a, 4 * S->n, S->stream));
+    HG_HIP(launch_zero_f64(S->al, 4 * S->n, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    S->acc_valid = true;
     return HG_OK;
 }
 

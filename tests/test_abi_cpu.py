@@ -25,9 +25,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_layout_matches_header():
-    # 8 int32 then a double: r_verlet at byte 32; 59 doubles of functor parameters.
-    assert _abi.Params.r_verlet.offset == 32
-    assert C.sizeof(_abi.Params) == 40 + 8 * (18 + 13 + 5 + 18 + 5)
+    # 10 int32 then a double: r_verlet at byte 40; 59 doubles of functor parameters.
+    assert _abi.Params.r_verlet.offset == 40
+    assert C.sizeof(_abi.Params) == 48 + 8 * (18 + 13 + 5 + 18 + 5)
 
 
 def test_topology_matches_reference_fixtures():
