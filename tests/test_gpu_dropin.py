@@ -43,8 +43,8 @@ def compare(osys, got, what):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{what}: field {k}")
 
 
-def oracle(binary, hamaker, x, v, **kw):
-    return O.OracleSystem(O.system_params(binary, hamaker, **kw), x, v)
+def oracle(binary_model, hamaker_model, x, v, **kw):
+    return O.OracleSystem(O.system_params(binary_model, hamaker_model, **kw), x, v)
 
 
 def test_dropin_contact(tmp_path):
