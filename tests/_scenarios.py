@@ -51,6 +51,20 @@ def surface_drop():
     return x, v, verts
 
 
+def surface_contacts():
+    """Two particles in contact with the facet of surface_drop() from the
+    start of a short run: one over the facet's hypotenuse edge (the
+    reference geometry's x, y, lowered to z = 0.71 r so it meets the edge
+    within a few dozen steps of 1e-13 s) and one overlapping the face, sliding
+    and spinning."""
+    a = 50.0 * R
+    verts = np.array([[[-a, -a, 0.0], [-a, a, 0.0], [a, -a, 0.0]]])
+    x = np.array([[0.5 * R, 0.5 * R, 0.71 * R], [-10.0 * R, -10.0 * R, 0.995 * R]])
+    v = np.array([[0.0, 0.0, -0.5], [0.3, 0.0, -0.5]])
+    w = np.array([[0.0, 0.0, 0.0], [0.0, 1.0e8, 0.0]])
+    return x, v, w, verts
+
+
 def aggregate_params(k=1.0e4, zeta=0.5, dt=1e-13, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.0 * R, d_crit=1e-3 * R):
     """Aggregate workload functor set: alt_sinter (bonds with k, damping ratio
     zeta) embedding a contact functor with friction mu, plus a limited-range

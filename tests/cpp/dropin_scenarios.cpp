@@ -158,7 +158,10 @@ int run_surface(size_t steps, const char *out, bool moving) {
     using sh_t = surface_hamaker_functor<V3, double>;
     using facet_t = triangular_facet<V3, double, sc_t, sh_t>;
     const double dt = 1e-13, a = 50.0 * r_part, mu_s = 0.5;
-    VV x0{V3(0.5 * r_part, 0.5 * r_part, 1.01 * r_part)}, v0{V3(0.0, 0.0, -0.5)}, th0(1), w0(1);
+    // test/surface.cpp's facet; one particle meets its hypotenuse edge, one
+    // overlaps its face while sliding and spinning
+    VV x0{V3(0.5 * r_part, 0.5 * r_part, 0.71 * r_part), V3(-10.0 * r_part, -10.0 * r_part, 0.995 * r_part)};
+    VV v0{V3(0.0, 0.0, -0.5), V3(0.3, 0.0, -0.5)}, th0(2), w0{V3(), V3(0.0, 1.0e8, 0.0)};
     rotational_step_handler<VV, V3> handler;
     contact_force_functor<V3, double> contact(x0.size(), k, gamma_n, k, gamma_t, mu_s, phi, k, gamma_r, mu_o, phi, k,
                                               gamma_o, mu_o, phi, r_part, mass, inertia, dt, V3(), 0.0);

@@ -100,12 +100,14 @@ def test_dropin_triangular_facet(tmp_path, scenario):
     switches on surface Hamaker."""
     steps = 3000
     got = run_driver(tmp_path, scenario, steps)
-    x, v, verts = S.surface_drop()
-    x[0, 2] = 1.01 * R
+    x, v, w, verts = S.surface_contacts()
     moving = scenario == "plate"
     cp = O.contact_params(mu=0.5)
-    o = oracle(O.BIN_CONTACT, O.HAM_NONE, x, v, contact=cp, facet_contact=1, facet_hamaker=1, surface_contact=cp,
-               surface_hamaker=O.hamaker_params(A=1e-20 if moving else 0.0, h0=1e-9))
+    o = O.OracleSystem(O.system_params(O.BIN_CONTACT, O.HAM_NONE, contact=cp, facet_contact=1, facet_hamaker=1,
+                                       surface_contact=cp,
+                                       surface_hamaker=O.hamaker_params(A=1e-20 if moving else 0.0, h0=1e-9)),
+                       x, v, w0=w)
     o.set_facets(verts, np.array([[0.5, -0.2, 1.0]]) if moving else np.zeros((1, 3)))
     o.step(1e-13, steps)
     compare(o, got, scenario)
+    assert got["v"][0, 2] > -0.5 and got["v"][1, 2] > -0.5  # both particles met the facet

@@ -134,16 +134,21 @@ def test_alt_sinter_lattice():
 
 
 def test_surface_facet_drop():
-    """test/surface.cpp: one particle on a static facet, surface contact (A=0)."""
+    """test/surface.cpp facet and functors (surface contact mu 0.5, surface
+    Hamaker with A = 0): one particle meeting the facet's edge, one sliding
+    and spinning on its face."""
     announce("surface facet")
-    x, v, verts = S.surface_drop()
-    x[0, 2] = 1.01 * R
+    x, v, w, verts = S.surface_contacts()
     cp = O.contact_params(mu=0.5)
-    o, g = both(O.BIN_CONTACT, O.HAM_NONE, x, v, contact=cp, facets=(verts, np.zeros((1, 3))), facet_contact=1,
-                facet_hamaker=1, surface_contact=cp, surface_hamaker=O.hamaker_params(A=0.0, h0=1e-9))
+    o, g = both(O.BIN_CONTACT, O.HAM_NONE, x, v, w0=w, contact=cp, facets=(verts, np.zeros((1, 3))),
+                facet_contact=1, facet_hamaker=1, surface_contact=cp,
+                surface_hamaker=O.hamaker_params(A=0.0, h0=1e-9))
     o.step(1e-13, 3000)
     g.step(1e-13, 3000)
     assert_same_state(o, g)
+    st = g.state()
+    assert st["v"][0, 2] > -0.5 and st["v"][1, 2] > -0.5  # both met the facet
+    assert abs(st["omega"][1, 1] - 1.0e8) > 1.0  # friction acted on the spin
 
 
 def test_moving_facets_and_tilted_plate():

@@ -106,10 +106,12 @@ def test_pool_with_facets_matches_oracle():
         x, v0 = agg_case(300, seed)
         zmin, zmax = x[:, 2].min(), x[:, 2].max()
         a = 80 * R
-        verts = np.array([[[-a, -a, zmin - 1.05 * R], [a, -a, zmin - 1.05 * R], [-a, a, zmin - 1.05 * R]],
-                          [[a, a, zmin - 1.05 * R], [-a, a, zmin - 1.05 * R], [a, -a, zmin - 1.05 * R]],
-                          [[-a, -a, zmax + 1.05 * R], [-a, a, zmax + 1.05 * R], [a, -a, zmax + 1.05 * R]],
-                          [[a, a, zmax + 1.05 * R], [a, -a, zmax + 1.05 * R], [-a, a, zmax + 1.05 * R]]])
+        # plates 0.97 r from the outermost centres: in contact from the start
+        lo, hi = zmin - 0.97 * R, zmax + 0.97 * R
+        verts = np.array([[[-a, -a, lo], [a, -a, lo], [-a, a, lo]],
+                          [[a, a, lo], [-a, a, lo], [a, -a, lo]],
+                          [[-a, -a, hi], [-a, a, hi], [a, -a, hi]],
+                          [[a, a, hi], [a, -a, hi], [-a, a, hi]]])
         vel = np.array([[0, 0, speed], [0, 0, speed], [0, 0, -speed], [0, 0, -speed]], dtype=float)
         slot = pool.add(x, v0)
         pool.set_facets(slot, verts, vel)
