@@ -132,15 +132,12 @@ def _nsteps(case):
 CASES = _cases()
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", list(CASES))
-def test_reference_kat_device(name):
+def _device_run(case, progress):
+    """Runs one scenario on its own pool (own stream) and returns the final
+    state; progress[0] counts the steps done."""
     import libgran_amd as L
-    case = CASES[name]
     x, v, facets = _init(case)
     n = _nsteps(case)
-    print(f"[kat] device {name}: {n} steps", flush=True)
-    sys.stdout.flush()
     p = L.make_params(case["binary"], case["hamaker"], pair_mode=case.get("pair_mode", 0),
                       r_verlet=case.get("r_verlet", 0.0),
                       contact=case.get("contact", O.contact_params()), sinter=case.get("sinter", O.sinter_params()),
@@ -153,12 +150,50 @@ def test_reference_kat_device(name):
     if facets:
         pool.set_facets(slot, *facets)
     every = case.get("rebuild_every", 0)
-    head = n - n % every if every else n
-    pool.step(case["dt"], head, every)
-    if n - head:
-        pool.step(case["dt"], n - head, n - head)
+    slice_len = 100000 - 100000 % every if every else 100000
+    done = 0
+    while done < n:
+        k = min(slice_len, n - done)
+        pool.step(case["dt"], k, every if every and k % every == 0 else (k if every else 0))
+        done += k
+        progress[0] = done
     assert pool.info(slot)[2] == n
-    _assert_checks(name, case, _evaluate(case, x, pool.state(slot)))
+    return x, pool.state(slot)
+
+
+@pytest.mark.gpu
+def test_reference_kats_device():
+    """All five scenarios at full length, concurrently (one pool and stream
+    each; a scenario of a few dozen particles occupies one workgroup)."""
+    import threading
+    import time
+
+    import libgran_amd  # noqa: F401  (load the library before the threads start)
+    names = list(CASES)
+    progress = {nm: [0] for nm in names}
+    results, errors = {}, {}
+
+    def work(nm):
+        try:
+            results[nm] = _device_run(CASES[nm], progress[nm])
+        except Exception as e:  # reported below
+            errors[nm] = e
+
+    threads = [threading.Thread(target=work, args=(nm,)) for nm in names]
+    for t in threads:
+        t.start()
+    t0 = time.time()
+    while any(t.is_alive() for t in threads):
+        time.sleep(15.0)
+        print(f"[kat] {time.time() - t0:6.0f} s: " +
+              ", ".join(f"{nm} {progress[nm][0] / _nsteps(CASES[nm]):.0%}" for nm in names), flush=True)
+        sys.stdout.flush()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for nm in names:
+        x0, st = results[nm]
+        _assert_checks(nm, CASES[nm], _evaluate(CASES[nm], x0, st))
 
 
 @pytest.mark.slow
