@@ -11,7 +11,10 @@ the reference tree, against numbers the reference's authors produced with it.
 
 One reference target is not reproduced: test/alt_sintered.cpp expects the
 bonded 3x3x3 lattice to keep its initial radius of gyration (3.9598e-08 m is
-2*sqrt(2)*r to six digits), while the restated model lets it unfold by 3.1 %
+2*sqrt(2)*r to six digits, the closed form of the undeformed lattice rather
+than a simulation output; the driver's check uses unqualified abs(), which
+under libstdc++ binds to abs(int) and passes whatever the value), while the
+restated model lets it unfold by 3.1 %
 over the run (4.08443e-08 m; the centre-of-mass drift target is met). The
 bond topology is a spanning tree (alt_sinter_bridge.h:64-86 skips the 28
 cycle-closing pairs of the 54 touching ones), and the spring rotation
