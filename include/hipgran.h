@@ -237,6 +237,61 @@ size_t hg_pool_bytes_per_particle(const hg_params *params, size_t max_facets);
  * events on the pool's stream) and the number of launches. */
 hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *rebuild_ms, uint64_t *chunks);
 
+/* ------------------------------------------------------------------------
+ * Server: continuous batching of independent simulation requests.
+ *
+ * Not in the reference, whose drivers run one granular_system to completion
+ * per process. A server owns one pool (arena sized by budget_bytes through
+ * hg_pool_bytes_per_particle: state, neighbour list, pair-state table and
+ * facet springs) and one worker thread. Requests (initial state, facets, step
+ * count) are submitted by one producer thread and queue in arrival order. The
+ * worker admits the head of the queue as soon as its particles fit a free
+ * range of the arena and a slot is free, advances all resident requests by
+ * `quantum` steps per round (one hg_pool_step), and retires each request when
+ * it has run its steps: the final state is copied to host memory and its
+ * range freed for the next one. A request's result does not depend on the
+ * other requests on the device. Callers are the serving stream's producers.
+ * ------------------------------------------------------------------------ */
+typedef struct hg_server hg_server;
+
+typedef struct {
+    uint64_t budget_bytes;  /* device memory for the pool arena */
+    uint32_t max_requests;  /* requests resident on the device at once */
+    uint32_t max_facets;    /* facets per request (0..16) */
+    uint32_t quantum;       /* steps per worker round; every request's nsteps is a multiple */
+    uint32_t rebuild_every; /* neighbour-list cadence; divides quantum */
+    double dt;              /* time step of every request */
+} hg_server_config;
+
+typedef struct {
+    uint64_t submitted, queued, running, done, failed;
+    uint64_t rounds;         /* worker rounds (hg_pool_step calls) */
+    uint64_t particle_steps; /* sum over rounds of resident particles x quantum */
+    double device_ms;        /* device time of the rounds' launches */
+    double elapsed_s;        /* seconds since hg_server_create */
+} hg_server_stats;
+
+enum { HG_REQ_QUEUED = 0, HG_REQ_RUNNING = 1, HG_REQ_DONE = 2, HG_REQ_FAILED = 3 };
+
+hg_status hg_server_create(const hg_params *params, const hg_server_config *cfg, hg_server **out);
+/* Stops the worker after its current round; unfinished requests are dropped. */
+hg_status hg_server_destroy(hg_server *srv);
+/* Queues a request (inputs are copied; v0, theta0, omega0 may be NULL for
+ * zeros; nf facets as hg_set_facets). *id identifies it in later calls. A
+ * request larger than the whole budget is refused with HG_ERR_CAPACITY. */
+hg_status hg_server_submit(hg_server *srv, size_t n, const double *x0, const double *v0, const double *theta0,
+                           const double *omega0, size_t nf, const double *verts, const double *v_facet,
+                           uint64_t nsteps, uint64_t *id);
+/* HG_REQ_* state of a request not yet released by hg_server_wait. */
+hg_status hg_server_status(hg_server *srv, uint64_t id, int *state);
+/* Blocks until the request has finished (timeout_s < 0: no limit; on time-out
+ * HG_ERR_CAPACITY and the request stays), copies its final x, v, theta, omega
+ * (any may be NULL) and times[3] = {submitted, admitted, done} in seconds
+ * since hg_server_create, and releases it. */
+hg_status hg_server_wait(hg_server *srv, uint64_t id, double timeout_s, double *x, double *v, double *theta,
+                         double *omega, double *times);
+hg_status hg_server_get_stats(hg_server *srv, hg_server_stats *out);
+
 /* Device-time profile of a system's launches, measured with HIP events
  * recorded on the system's stream around each launch (not in the reference,
  * which only times whole runs with std::chrono, main.cpp:146-164).

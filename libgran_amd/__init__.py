@@ -255,3 +255,62 @@ class Pool:
         c, r, k = C.c_double(0), C.c_double(0), C.c_uint64(0)
         _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k)))
         return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value)
+
+
+class Server:
+    """Continuous-batching request server (hg_server): requests queue in
+    arrival order, are admitted while they fit the device budget, run in one
+    shared pool and are retired with their final state on the host."""
+
+    def __init__(self, params, budget_bytes, max_requests, max_facets=0, quantum=1000, rebuild_every=50,
+                 dt=1e-13):
+        self.params = params
+        cfg = _abi.ServerConfig(int(budget_bytes), int(max_requests), int(max_facets), int(quantum),
+                                int(rebuild_every), float(dt))
+        self._h = C.c_void_p()
+        _abi.check(_abi.lib().hg_server_create(C.byref(params), C.byref(cfg), C.byref(self._h)))
+        self._sizes = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _abi.lib().hg_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def submit(self, x0, nsteps, v0=None, theta0=None, omega0=None, facets=None):
+        """Queues one request; facets = (verts (nf, 3, 3), v_facet (nf, 3)) or None. Returns its id."""
+        x0 = _abi.c64(x0)
+        nf, verts, vel = 0, None, None
+        if facets is not None:
+            verts, vel = _abi.c64(facets[0]), _abi.c64(facets[1])
+            nf = verts.shape[0]
+        rid = C.c_uint64(0)
+        _abi.check(_abi.lib().hg_server_submit(self._h, x0.shape[0], _abi.dptr(x0), _abi.dptr(_abi.c64(v0)),
+                                               _abi.dptr(_abi.c64(theta0)), _abi.dptr(_abi.c64(omega0)), nf,
+                                               _abi.dptr(verts), _abi.dptr(vel), int(nsteps), C.byref(rid)))
+        self._sizes[rid.value] = x0.shape[0]
+        return rid.value
+
+    def status(self, rid):
+        st = C.c_int(0)
+        _abi.check(_abi.lib().hg_server_status(self._h, rid, C.byref(st)))
+        return st.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Final state and times (submitted, admitted, done; seconds since the
+        server started) of a request; releases it."""
+        n = self._sizes[rid]
+        arrs = [np.zeros((n, 3)) for _ in range(4)]
+        times = np.zeros(3)
+        _abi.check(_abi.lib().hg_server_wait(self._h, rid, float(timeout), *[_abi.dptr(a) for a in arrs],
+                                             _abi.dptr(times)))
+        del self._sizes[rid]
+        out = dict(zip(("x", "v", "theta", "omega"), arrs))
+        out["times"] = times
+        return out
+
+    def stats(self):
+        s = _abi.ServerStats()
+        _abi.check(_abi.lib().hg_server_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}

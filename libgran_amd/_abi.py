@@ -49,6 +49,19 @@ class Params(C.Structure):
                 ("surface_contact", ContactParams), ("surface_hamaker", HamakerParams)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("max_requests", C.c_uint32), ("max_facets", C.c_uint32),
+                ("quantum", C.c_uint32), ("rebuild_every", C.c_uint32), ("dt", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "queued", "running", "done", "failed", "rounds",
+                                           "particle_steps")] + [("device_ms", C.c_double), ("elapsed_s", C.c_double)]
+
+
+HG_REQ_QUEUED, HG_REQ_RUNNING, HG_REQ_DONE, HG_REQ_FAILED = range(4)
+
+
 def fill(struct, values):
     """Sets a params sub-struct from a sequence in ctor-argument order."""
     for (name, _), val in zip(struct._fields_, values):
@@ -101,6 +114,12 @@ SIGNATURES = {
     "hg_pool_info": (C.c_int, [_sysp, C.c_int32, _szp, _dp, _u64p]),
     "hg_pool_bytes_per_particle": (C.c_size_t, [C.POINTER(Params), C.c_size_t]),
     "hg_pool_last_step_profile": (C.c_int, [_sysp, _dp, _dp, _u64p]),
+    "hg_server_create": (C.c_int, [C.POINTER(Params), C.POINTER(ServerConfig), C.POINTER(_sysp)]),
+    "hg_server_destroy": (C.c_int, [_sysp]),
+    "hg_server_submit": (C.c_int, [_sysp, C.c_size_t, _dp, _dp, _dp, _dp, C.c_size_t, _dp, _dp, C.c_uint64, _u64p]),
+    "hg_server_status": (C.c_int, [_sysp, C.c_uint64, C.POINTER(C.c_int)]),
+    "hg_server_wait": (C.c_int, [_sysp, C.c_uint64, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+    "hg_server_get_stats": (C.c_int, [_sysp, C.POINTER(ServerStats)]),
 }
 
 _lib = None
