@@ -236,6 +236,21 @@ size_t hg_pool_bytes_per_particle(const hg_params *params, size_t max_facets);
 /* Device time of the last hg_pool_step's chunk launches (milliseconds, HIP
  * events on the pool's stream) and the number of launches. */
 hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *rebuild_ms, uint64_t *chunks);
+/* Systems of at least wide_min particles (default 2048; 0: none) are stepped
+ * by per-step launches over all their particles, one thread each, on a second
+ * stream beside the chunk kernel, instead of by one workgroup: a large system
+ * then no longer sets the critical path of the interval. The per-particle
+ * operations are the same, so results do not change. Takes effect at the next
+ * hg_pool_step. */
+hg_status hg_pool_set_wide_threshold(hg_pool *pool, size_t wide_min);
+/* While both kinds of systems are present, the chunk kernel's persistent grid
+ * shrinks from two workgroups per CU to 2 * (CUs - wide_cus), leaving room
+ * worth wide_cus CUs (default a quarter) for the wide launches that run
+ * beside it. 0 < wide_cus < CU count. */
+hg_status hg_pool_set_wide_cus(hg_pool *pool, int wide_cus);
+/* Device time of the last hg_pool_step's wide launches (second stream, HIP
+ * events; milliseconds) and their number. */
+hg_status hg_pool_last_wide_profile(hg_pool *pool, double *wide_ms, uint64_t *launches);
 
 /* ------------------------------------------------------------------------
  * Server: continuous batching of independent simulation requests.

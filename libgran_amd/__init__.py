@@ -254,7 +254,17 @@ class Pool:
     def last_step_profile(self):
         c, r, k = C.c_double(0), C.c_double(0), C.c_uint64(0)
         _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k)))
-        return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value)
+        w, wl = C.c_double(0), C.c_uint64(0)
+        _abi.check(_abi.lib().hg_pool_last_wide_profile(self._h, C.byref(w), C.byref(wl)))
+        return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value, wide_ms=w.value, wide_launches=wl.value)
+
+    def set_wide_threshold(self, wide_min):
+        """Systems of at least wide_min particles (0: none) use the wide launches."""
+        _abi.check(_abi.lib().hg_pool_set_wide_threshold(self._h, int(wide_min)))
+
+    def set_wide_cus(self, wide_cus):
+        """CUs given to the wide launches while both kinds of systems run."""
+        _abi.check(_abi.lib().hg_pool_set_wide_cus(self._h, int(wide_cus)))
 
 
 class Server:

@@ -116,9 +116,13 @@ __device__ __forceinline__ void lds_fill(const pool_view &V, const pool_meta &M)
     }
 }
 
+// Acceleration pass for particles g = g_first, g_first + g_step, ... < g_end
+// of system M (a workgroup strides over the whole system; a wide launch gives
+// each thread one particle).
 template <int BIN, int HAM, int PAIRS, int PREC, int LDS>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
-                                                 const pool_meta &M, uint32_t cur, const facet_dev *F) {
+                                                 const pool_meta &M, uint32_t cur, const facet_dev *F,
+                                                 uint32_t g_first, uint32_t g_end, uint32_t g_step) {
     pair_table T;
     T.pj_old = cur ? V.pj1 : V.pj0;
     T.ps_old = cur ? V.ps1 : V.ps0;
@@ -129,7 +133,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
     T.cap = V.cap_s;
     T.stride = V.stride;
     const uint32_t end = M.offset + M.n;
-    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+    for (uint32_t g = g_first; g < g_end; g += g_step) {
         d3 xi, vi, wi;
         load_state<LDS>(V, M.offset, g, xi, vi, wi);
         walker W;
@@ -184,9 +188,8 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
 // operations in the same order as k_kick_drift.
 template <int LDS>
 __device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_meta &M, bool pending, double half,
-                                                double dt) {
-    const uint32_t end = M.offset + M.n;
-    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+                                                double dt, uint32_t g_first, uint32_t g_end, uint32_t g_step) {
+    for (uint32_t g = g_first; g < g_end; g += g_step) {
         double *q = lds_rec<LDS>(M.offset, g);
         d3 vi, wi, ai, ali, x0, th0;
         if constexpr (LDS == STATE_FULL) {
@@ -220,9 +223,9 @@ __device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_m
 
 // Final kick; a system kept wholly in LDS is written back here.
 template <int LDS>
-__device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_meta &M, double half) {
-    const uint32_t end = M.offset + M.n;
-    for (uint32_t g = M.offset + threadIdx.x; g < end; g += blockDim.x) {
+__device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_meta &M, double half,
+                                                uint32_t g_first, uint32_t g_end, uint32_t g_step) {
+    for (uint32_t g = g_first; g < g_end; g += g_step) {
         if constexpr (LDS == STATE_FULL) {
             const double *q = lds_rec<LDS>(M.offset, g);
             const d3 a = lds3(q + 12), al = lds3(q + 15);
@@ -245,25 +248,26 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
                                            const pool_meta &M, facet_dev *F, uint32_t nsteps, double dt) {
     const double half = dt / 2.0;
     uint32_t cur = M.cur;
+    const uint32_t g0 = M.offset + threadIdx.x, g_end = M.offset + M.n, gs = blockDim.x;
     if constexpr (LDS != STATE_GLOBAL) {
         lds_fill<LDS>(V, M);
         __syncthreads();
     }
     if (!M.acc_valid) {
-        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F);
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F, g0, g_end, gs);
         cur ^= 1u;
         __syncthreads();
     }
     for (uint32_t s = 0; s < nsteps; s++) {
-        pool_kick_drift<LDS>(V, M, s > 0, half, dt);
+        pool_kick_drift<LDS>(V, M, s > 0, half, dt, g0, g_end, gs);
         if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
         __syncthreads();
-        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F);
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F, g0, g_end, gs);
         cur ^= 1u;
         __syncthreads();
     }
     if (nsteps > 0) {
-        pool_final_kick<LDS>(V, M, half);
+        pool_final_kick<LDS>(V, M, half, g0, g_end, gs);
         if (threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
     } else if constexpr (LDS == STATE_FULL) {
         // only the initial pass ran: its accelerations go back to global
@@ -301,6 +305,63 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
         else
             run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL>(P, MC, V, M, F, nsteps, dt);
     }
+}
+
+// ------------------------------------------------- wide (large) systems -----
+// Systems of at least wide_min particles skip the chunk kernel, where one
+// workgroup would carry the whole system and set the interval's critical
+// path. They are stepped by per-step launches over all their particles at
+// once, one thread per particle: wide_block b covers particles
+// [start, start + kWideBlock) of system `slot`. The operations and their order
+// per particle are those of run_system, so results are the same either way.
+constexpr uint32_t kWideBlock = 128;
+constexpr uint32_t kDefaultWideMin = 2048;
+constexpr double kDefaultWideCuFraction = 0.25;
+struct wide_block {
+    uint32_t slot, start;
+};
+
+// pass < 0: the initial acceleration pass of the systems without valid
+// accelerations (others return); pass s >= 0: the pass of step s.
+template <int BIN, int HAM, int PAIRS, int PREC>
+__global__ void __launch_bounds__(kWideBlock) k_pool_wide_force(hg_params P, mixed_consts MC, pool_view V,
+                                                                const wide_block *map, int pass) {
+    const wide_block b = map[blockIdx.x];
+    const pool_meta M = V.meta[b.slot];
+    uint32_t cur = M.cur;
+    if (pass < 0) {
+        if (M.acc_valid) return;
+    } else {
+        cur ^= (uint32_t(pass) + (M.acc_valid ? 0u : 1u)) & 1u;
+    }
+    const uint32_t g = M.offset + b.start + threadIdx.x;
+    const uint32_t end = M.offset + M.n;
+    if (g >= end) return;
+    const facet_dev *F = V.facets + size_t(b.slot) * V.max_facets;
+    pool_force_phase<BIN, HAM, PAIRS, PREC, STATE_GLOBAL>(P, MC, V, M, cur, F, g, g + 1, 1);
+}
+
+// Kick/drift of step s (pending = s > 0 completes step s-1 and moves the
+// facets by one step first, as run_system does).
+__global__ void __launch_bounds__(kWideBlock) k_pool_wide_kick_drift(pool_view V, const wide_block *map, int pending,
+                                                                     double half, double dt) {
+    const wide_block b = map[blockIdx.x];
+    const pool_meta M = V.meta[b.slot];
+    if (pending && b.start == 0 && threadIdx.x == 0)
+        advance_facets(V.facets + size_t(b.slot) * V.max_facets, int(M.nf), dt);
+    const uint32_t g = M.offset + b.start + threadIdx.x;
+    if (g >= M.offset + M.n) return;
+    pool_kick_drift<STATE_GLOBAL>(V, M, pending != 0, half, dt, g, g + 1, 1);
+}
+
+__global__ void __launch_bounds__(kWideBlock) k_pool_wide_final_kick(pool_view V, const wide_block *map, double half,
+                                                                     double dt) {
+    const wide_block b = map[blockIdx.x];
+    const pool_meta M = V.meta[b.slot];
+    if (b.start == 0 && threadIdx.x == 0) advance_facets(V.facets + size_t(b.slot) * V.max_facets, int(M.nf), dt);
+    const uint32_t g = M.offset + b.start + threadIdx.x;
+    if (g >= M.offset + M.n) return;
+    pool_final_kick<STATE_GLOBAL>(V, M, half, g, g + 1, 1);
 }
 
 // ------------------------------------------------------ batched rebuild -----
@@ -447,6 +508,36 @@ hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t
     return hipErrorInvalidValue;
 }
 
+template <int BIN, int HAM, int PAIRS>
+hipError_t launch_wide_force_t(const hg_params &p, const pool_view &V, const wide_block *map, unsigned blocks,
+                               int pass, hipStream_t s) {
+    if (p.precision == HG_PRECISION_MIXED)
+        k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_MIXED>
+            <<<blocks, kWideBlock, 0, s>>>(p, make_mixed_consts(p), V, map, pass);
+    else
+        k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_FP64>
+            <<<blocks, kWideBlock, 0, s>>>(p, make_mixed_consts(p), V, map, pass);
+    return hipGetLastError();
+}
+
+template <int PAIRS>
+hipError_t launch_wide_force_p(const hg_params &p, const pool_view &V, const wide_block *map, unsigned blocks,
+                               int pass, hipStream_t s) {
+#define HG_WCASE(B, H) \
+    if (p.binary_model == B && p.hamaker_model == H) return launch_wide_force_t<B, H, PAIRS>(p, V, map, blocks, pass, s);
+    HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
+    HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
+    HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
+    HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
+    HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
+    HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
+    HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+#undef HG_WCASE
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
 }  // namespace hg
 
@@ -471,7 +562,20 @@ struct hg_pool {
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> bonds;
     std::vector<size_t> skipped;
     uint32_t *order_d = nullptr, *counter_d = nullptr;
-    std::vector<uint32_t> order;
+    std::vector<uint32_t> order;   // every active system, largest first
+    uint32_t narrow_count = 0;     // systems below wide_min, uploaded to order_d
+    uint32_t wide_min = 0;         // systems with n >= wide_min use the wide launches (0: none)
+    wide_block *wide_map_d = nullptr;
+    size_t wide_map_cap = 0;
+    unsigned wide_blocks = 0;
+    bool wide_pending_initial = false;
+    // with wide systems present the wide launches run on stream_w beside the
+    // chunk kernel, whose grid then leaves room worth wide_cus CUs
+    hipStream_t stream_w = nullptr;
+    int n_cus = 0, wide_cus = 0;
+    hipEvent_t wev[3] = {nullptr, nullptr, nullptr};
+    double last_wide_ms = 0.0;
+    uint64_t last_wide_launches = 0;
     bool order_dirty = true;
     bool meta_dirty = true;
     uint32_t hash_size = 0, *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr,
@@ -670,6 +774,11 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     }
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream, hipStreamNonBlocking));
     for (auto &e : Q->ev) PG_HIPC(hipEventCreate(&e));
+    for (auto &e : Q->wev) PG_HIPC(hipEventCreate(&e));
+    Q->n_cus = prop.multiProcessorCount;
+    Q->wide_min = kDefaultWideMin;
+    Q->wide_cus = int(std::lround(kDefaultWideCuFraction * Q->n_cus));
+    PG_HIPC(hipStreamCreateWithFlags(&Q->stream_w, hipStreamNonBlocking));
     const size_t P = Q->P;
     double **fields[6] = {&Q->x, &Q->v, &Q->th, &Q->w, &Q->a, &Q->al};
     for (auto f : fields) PG_TRYC(palloc(Q, f, 4 * P));
@@ -705,6 +814,8 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     if (Q->MF) PG_TRYC(palloc(Q, &Q->ss, Q->MF * 9 * P));
     PG_TRYC(palloc(Q, &Q->meta_d, Q->S));
     PG_TRYC(palloc(Q, &Q->order_d, Q->S));
+    Q->wide_map_cap = Q->P / kWideBlock + Q->S + 1;
+    PG_TRYC(palloc(Q, &Q->wide_map_d, Q->wide_map_cap));
     PG_TRYC(palloc(Q, &Q->counter_d, 1));
     PG_TRYC(palloc(Q, &Q->flags, FLAG_COUNT));
     PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->flags_host), FLAG_COUNT * sizeof(int)));
@@ -726,9 +837,13 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
 hg_status hg_pool_destroy(hg_pool *Q) {
     if (!Q) return HG_OK;
     if (Q->stream) (void)hipStreamSynchronize(Q->stream);
+    if (Q->stream_w) (void)hipStreamSynchronize(Q->stream_w);
     for (void *p : Q->allocations) (void)hipFree(p);
     for (auto e : Q->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : Q->wev)
+        if (e) (void)hipEventDestroy(e);
+    if (Q->stream_w) (void)hipStreamDestroy(Q->stream_w);
     if (Q->flags_host) (void)hipHostFree(Q->flags_host);
     if (Q->stream) (void)hipStreamDestroy(Q->stream);
     delete Q;
@@ -880,8 +995,8 @@ hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *
 
 hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
-    Q->last_chunk_ms = Q->last_rebuild_ms = 0.0;
-    Q->last_chunks = 0;
+    Q->last_chunk_ms = Q->last_rebuild_ms = Q->last_wide_ms = 0.0;
+    Q->last_chunks = Q->last_wide_launches = 0;
     if (nsteps == 0) return HG_OK;
     const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     if (nl && rebuild_every && nsteps % rebuild_every)
@@ -892,18 +1007,36 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
             if (Q->active[s]) Q->order.push_back(uint32_t(s));
         std::stable_sort(Q->order.begin(), Q->order.end(),
                          [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
-        if (!Q->order.empty())
-            HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data(), Q->order.size() * 4, hipMemcpyHostToDevice, Q->stream));
+        // largest first: the wide systems lead, the chunk kernel takes the rest
+        std::vector<wide_block> map;
+        size_t first_narrow = 0;
+        while (first_narrow < Q->order.size() && Q->wide_min &&
+               Q->meta[Q->order[first_narrow]].n >= Q->wide_min) {
+            const uint32_t s = Q->order[first_narrow++];
+            for (uint32_t st = 0; st < Q->meta[s].n; st += kWideBlock) map.push_back(wide_block{s, st});
+        }
+        if (map.size() > Q->wide_map_cap) return fail(HG_ERR_INVALID, "hg_pool_step: wide map overflow");
+        Q->narrow_count = uint32_t(Q->order.size() - first_narrow);
+        Q->wide_blocks = unsigned(map.size());
+        if (Q->narrow_count)
+            HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data() + first_narrow, size_t(Q->narrow_count) * 4,
+                                  hipMemcpyHostToDevice, Q->stream));
+        if (!map.empty())
+            HG_HIP(hipMemcpyAsync(Q->wide_map_d, map.data(), map.size() * sizeof(wide_block), hipMemcpyHostToDevice,
+                                  Q->stream));
         HG_HIP(hipStreamSynchronize(Q->stream));
         Q->order_dirty = false;
     }
-    const uint32_t count = uint32_t(Q->order.size());
-    if (count == 0) return HG_OK;
+    const uint32_t count = Q->narrow_count;
+    if (Q->order.empty()) return HG_OK;
+    bool any_wide_initial = false;
+    for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
     // chunks: one rebuild interval each in neighbour-list mode; bounded
     // lengths otherwise so a launch stays short
     const uint64_t chunk = (nl && rebuild_every) ? rebuild_every : std::min<uint64_t>(nsteps, 2000);
     const pool_view V = make_pool_view(Q);
     const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
+    const double half = dt / 2.0;
     for (uint64_t done = 0; done < nsteps;) {
         const uint32_t len = uint32_t(std::min<uint64_t>(chunk, nsteps - done));
         if (nl) {
@@ -926,20 +1059,57 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
             }
         }
         PG_TRY(upload_meta(Q));
-        HG_HIP(launch_zero_u32(Q->counter_d, 1, Q->stream));
-        HG_HIP(hipEventRecord(Q->ev[2], Q->stream));
-        hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt,
-                                                                   grid, Q->lds_cap, Q->stream)
-                          : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, grid,
-                                                         Q->lds_cap, Q->stream);
-        HG_HIP(e);
-        HG_HIP(hipEventRecord(Q->ev[3], Q->stream));
+        // wide systems: per-step launches on their own CUs, beside the chunk
+        // kernel on the others (all CUs go to whichever kind is alone)
+        const bool split = Q->wide_blocks && count;
+        hipStream_t sc = Q->stream;
+        if (Q->wide_blocks) {
+            hipStream_t s2 = split ? Q->stream_w : Q->stream;
+            HG_HIP(hipEventRecord(Q->wev[0], s2));
+            auto force = [&](int pass) {
+                return nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2)
+                          : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2);
+            };
+            if (any_wide_initial) {
+                HG_HIP(force(-1));
+                Q->last_wide_launches++;
+            }
+            for (uint32_t st = 0; st < len; st++) {
+                k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, s2>>>(V, Q->wide_map_d, st > 0 ? 1 : 0, half,
+                                                                               dt);
+                HG_HIP(hipGetLastError());
+                HG_HIP(force(int(st)));
+            }
+            k_pool_wide_final_kick<<<Q->wide_blocks, kWideBlock, 0, s2>>>(V, Q->wide_map_d, half, dt);
+            HG_HIP(hipGetLastError());
+            Q->last_wide_launches += 2 * len + 1;
+            HG_HIP(hipEventRecord(Q->wev[1], s2));
+        }
+        if (count) {
+            const unsigned g = split ? std::min(grid, unsigned(2 * (Q->n_cus - Q->wide_cus))) : grid;
+            HG_HIP(launch_zero_u32(Q->counter_d, 1, sc));
+            HG_HIP(hipEventRecord(Q->ev[2], sc));
+            hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len,
+                                                                       dt, g, Q->lds_cap, sc)
+                              : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, g,
+                                                             Q->lds_cap, sc);
+            HG_HIP(e);
+            HG_HIP(hipEventRecord(Q->ev[3], sc));
+        }
+        if (split) HG_HIP(hipStreamWaitEvent(Q->stream, Q->wev[1], 0));
         PG_TRY(check_pool_flags(Q, "hg_pool_step"));
         float ms = 0.f;
-        HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
-        Q->last_chunk_ms += ms;
-        Q->last_chunks++;
-        // host mirror of what the kernel did to each system
+        if (count) {
+            HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
+            Q->last_chunk_ms += ms;
+            Q->last_chunks++;
+        }
+        if (Q->wide_blocks) {
+            HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
+            Q->last_wide_ms += ms;
+        }
+        any_wide_initial = false;
+        // host mirror of what the kernels did to each system
         for (uint32_t s : Q->order) {
             pool_meta &M = Q->meta[s];
             if (Q->cap_s) {
@@ -953,6 +1123,29 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
         done += len;
     }
     return upload_meta(Q);
+}
+
+hg_status hg_pool_set_wide_threshold(hg_pool *Q, size_t wide_min) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (wide_min > 0xffffffffu) wide_min = 0;
+    Q->wide_min = uint32_t(wide_min);
+    Q->order_dirty = true;
+    return HG_OK;
+}
+
+hg_status hg_pool_set_wide_cus(hg_pool *Q, int wide_cus) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (wide_cus < 1 || wide_cus >= Q->n_cus)
+        return fail(HG_ERR_INVALID, "hg_pool_set_wide_cus: need 0 < wide_cus < " + std::to_string(Q->n_cus));
+    Q->wide_cus = wide_cus;
+    return HG_OK;
+}
+
+hg_status hg_pool_last_wide_profile(hg_pool *Q, double *wide_ms, uint64_t *launches) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (wide_ms) *wide_ms = Q->last_wide_ms;
+    if (launches) *launches = Q->last_wide_launches;
+    return HG_OK;
 }
 
 hg_status hg_pool_get_state(hg_pool *Q, int32_t slot, double *x, double *v, double *theta, double *omega) {

@@ -225,3 +225,54 @@ def test_mixed_precision_short_trajectory():
               flush=True)
     assert worst < 1e-5
     assert dx < 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pool_wide_launches_equal_chunk_kernel(precision):
+    """The same systems stepped by the chunk kernel (wide threshold 0) and by
+    the wide per-step launches (threshold 1): identical results, with facets,
+    a system added mid-run (initial acceleration pass on the wide path) and a
+    mixed pool where both paths run side by side."""
+    announce(f"pool wide vs chunk, precision {precision}")
+    L = _L()
+    c, s, h = S.aggregate_params()
+    p = L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R,
+                      precision=precision, contact=c, sinter=s, hamaker=h, facet_contact=1, facet_hamaker=1,
+                      surface_contact=c, surface_hamaker=O.hamaker_params(A=1e-20, h0=1e-10))
+    from libgran_amd import workloads as W
+    cases = [agg_case(n, 70 + k) for k, n in enumerate((90, 700, 2600, 300))]
+    late = agg_case(1500, 79)
+    results = []
+    for wide_min in (0, 1, 1000):
+        pool = L.Pool(p, max_particles=8000, max_systems=8, max_facets=4)
+        pool.set_wide_threshold(wide_min)
+        slots = []
+        for k, (x, v0) in enumerate(cases):
+            slot = pool.add(x, v0)
+            pool.set_facets(slot, *W.plates(x, R, 6.0, compress=k % 2 == 0))
+            slots.append(slot)
+        pool.step(DT, 150, rebuild_every=50)
+        slots.append(pool.add(*late))
+        pool.step(DT, 200, rebuild_every=50)
+        results.append([pool.state(sl) for sl in slots])
+        pool.close()
+    for other in results[1:]:
+        for k, (a, b) in enumerate(zip(results[0], other)):
+            for f in ("x", "v", "theta", "omega", "a", "alpha"):
+                np.testing.assert_array_equal(b[f], a[f], err_msg=f"system {k} field {f}")
+
+
+def test_pool_wide_all_pairs_matches_oracle():
+    """All-pairs mode on the wide path (threshold 1) against the oracle."""
+    announce("pool wide all pairs vs oracle")
+    L = _L()
+    x, v = S.lattice_with_impactor()
+    x[-1, 2] = 2.05 * R
+    p = L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=O.contact_params(), hamaker=O.hamaker_params())
+    pool = L.Pool(p, max_particles=64, max_systems=2)
+    pool.set_wide_threshold(1)
+    slot = pool.add(x, v)
+    pool.step(DT, 3000)
+    o = O.OracleSystem(O.system_params(O.BIN_CONTACT, O.HAM_FULL), x, v)
+    o.step(DT, 3000)
+    same(o.state(), pool.state(slot), "wide all pairs")
