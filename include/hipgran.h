@@ -248,6 +248,19 @@ hg_status hg_pool_set_wide_threshold(hg_pool *pool, size_t wide_min);
  * worth wide_cus CUs (default a quarter) for the wide launches that run
  * beside it. 0 < wide_cus < CU count. */
 hg_status hg_pool_set_wide_cus(hg_pool *pool, int wide_cus);
+/* The wide launches of one interval (2 per step + 1) are recorded once into a
+ * HIP graph per (interval length, dt, wide set) and replayed; enable = 0
+ * launches them one by one instead. Default on. */
+hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
+/* Online autotuner of the launch split: from the next hg_pool_step on, each
+ * candidate (wide threshold 0, or 1024..8192 up to the largest system, with
+ * room for CUs/16, /8 or /4) runs two rebuild intervals of live steps, scored
+ * by the second one's host-measured time per step; the fastest is kept. The
+ * split never changes results, so no step is wasted. enable = 0 stops it and
+ * keeps the current setting. */
+hg_status hg_pool_autotune(hg_pool *pool, int enable);
+/* Current split and whether the autotuner has finished (tuned = 1). */
+hg_status hg_pool_get_tuning(hg_pool *pool, size_t *wide_min, int *wide_cus, int *tuned);
 /* Device time of the last hg_pool_step's wide launches (second stream, HIP
  * events; milliseconds) and their number. */
 hg_status hg_pool_last_wide_profile(hg_pool *pool, double *wide_ms, uint64_t *launches);

@@ -84,7 +84,8 @@ class System:
                                _abi.dptr(_abi.c64(theta0)), _abi.dptr(_abi.c64(omega0)), C.byref(self._h)))
 
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "_h", None) and self._h.value and _abi is not None and C is not None:
             _abi.lib().hg_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -175,7 +176,8 @@ class Pool:
                                              C.byref(self._h)))
 
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "_h", None) and self._h.value and _abi is not None and C is not None:
             _abi.lib().hg_pool_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -266,6 +268,19 @@ class Pool:
         """CUs given to the wide launches while both kinds of systems run."""
         _abi.check(_abi.lib().hg_pool_set_wide_cus(self._h, int(wide_cus)))
 
+    def set_wide_graph(self, enable):
+        """Replay each interval's wide launches as a HIP graph (default) or not."""
+        _abi.check(_abi.lib().hg_pool_set_wide_graph(self._h, 1 if enable else 0))
+
+    def autotune(self, enable=True):
+        """Tune the wide threshold and room on the next live intervals."""
+        _abi.check(_abi.lib().hg_pool_autotune(self._h, 1 if enable else 0))
+
+    def tuning(self):
+        wm, wc, done = C.c_size_t(0), C.c_int(0), C.c_int(0)
+        _abi.check(_abi.lib().hg_pool_get_tuning(self._h, C.byref(wm), C.byref(wc), C.byref(done)))
+        return dict(wide_min=wm.value, wide_cus=wc.value, tuned=bool(done.value))
+
 
 class Server:
     """Continuous-batching request server (hg_server): requests queue in
@@ -282,7 +297,8 @@ class Server:
         self._sizes = {}
 
     def close(self):
-        if getattr(self, "_h", None) and self._h.value:
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "_h", None) and self._h.value and _abi is not None and C is not None:
             _abi.lib().hg_server_destroy(self._h)
             self._h = C.c_void_p()
 

@@ -117,6 +117,9 @@ SIGNATURES = {
     "hg_pool_set_wide_threshold": (C.c_int, [_sysp, C.c_size_t]),
     "hg_pool_last_wide_profile": (C.c_int, [_sysp, _dp, _u64p]),
     "hg_pool_set_wide_cus": (C.c_int, [_sysp, C.c_int]),
+    "hg_pool_set_wide_graph": (C.c_int, [_sysp, C.c_int]),
+    "hg_pool_autotune": (C.c_int, [_sysp, C.c_int]),
+    "hg_pool_get_tuning": (C.c_int, [_sysp, _szp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hg_server_create": (C.c_int, [C.POINTER(Params), C.POINTER(ServerConfig), C.POINTER(_sysp)]),
     "hg_server_destroy": (C.c_int, [_sysp]),
     "hg_server_submit": (C.c_int, [_sysp, C.c_size_t, _dp, _dp, _dp, _dp, C.c_size_t, _dp, _dp, C.c_uint64, _u64p]),

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -317,6 +318,7 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
 constexpr uint32_t kWideBlock = 128;
 constexpr uint32_t kDefaultWideMin = 2048;
 constexpr double kDefaultWideCuFraction = 0.25;
+enum { TUNE_OFF = 0, TUNE_START = 1, TUNE_RUNNING = 2, TUNE_DONE = 3 };
 struct wide_block {
     uint32_t slot, start;
 };
@@ -573,6 +575,15 @@ struct hg_pool {
     // chunk kernel, whose grid then leaves room worth wide_cus CUs
     hipStream_t stream_w = nullptr;
     int n_cus = 0, wide_cus = 0;
+    bool wide_graph = true;  // replay the wide launches of an interval as a HIP graph
+    int tune_phase = 0;      // TUNE_*
+    std::vector<std::pair<uint32_t, int>> tune_cands;  // (wide_min, wide_cus)
+    std::vector<double> tune_scores;                   // seconds per step
+    size_t tune_pos = 0;
+    hipGraphExec_t wg_exec = nullptr;
+    uint32_t wg_len = 0;
+    unsigned wg_blocks = 0;
+    double wg_dt = 0.0;
     hipEvent_t wev[3] = {nullptr, nullptr, nullptr};
     double last_wide_ms = 0.0;
     uint64_t last_wide_launches = 0;
@@ -843,6 +854,7 @@ hg_status hg_pool_destroy(hg_pool *Q) {
         if (e) (void)hipEventDestroy(e);
     for (auto e : Q->wev)
         if (e) (void)hipEventDestroy(e);
+    if (Q->wg_exec) (void)hipGraphExecDestroy(Q->wg_exec);
     if (Q->stream_w) (void)hipStreamDestroy(Q->stream_w);
     if (Q->flags_host) (void)hipHostFree(Q->flags_host);
     if (Q->stream) (void)hipStreamDestroy(Q->stream);
@@ -993,6 +1005,81 @@ hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *
     return HG_OK;
 }
 
+namespace {
+
+// Sorts the active systems largest first, splits off those that take the
+// wide launches and uploads the chunk kernel's queue and the wide block map.
+hg_status refresh_order(hg_pool *Q) {
+    Q->order.clear();
+    for (size_t s = 0; s < Q->S; s++)
+        if (Q->active[s]) Q->order.push_back(uint32_t(s));
+    std::stable_sort(Q->order.begin(), Q->order.end(),
+                     [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
+    std::vector<wide_block> map;
+    size_t first_narrow = 0;
+    while (first_narrow < Q->order.size() && Q->wide_min && Q->meta[Q->order[first_narrow]].n >= Q->wide_min) {
+        const uint32_t s = Q->order[first_narrow++];
+        for (uint32_t st = 0; st < Q->meta[s].n; st += kWideBlock) map.push_back(wide_block{s, st});
+    }
+    if (map.size() > Q->wide_map_cap) return fail(HG_ERR_INVALID, "hg_pool_step: wide map overflow");
+    Q->narrow_count = uint32_t(Q->order.size() - first_narrow);
+    Q->wide_blocks = unsigned(map.size());
+    if (Q->narrow_count)
+        HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data() + first_narrow, size_t(Q->narrow_count) * 4,
+                              hipMemcpyHostToDevice, Q->stream));
+    if (!map.empty())
+        HG_HIP(hipMemcpyAsync(Q->wide_map_d, map.data(), map.size() * sizeof(wide_block), hipMemcpyHostToDevice,
+                              Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    Q->order_dirty = false;
+    return HG_OK;
+}
+
+// Online tuner of the wide threshold and the room left for wide launches.
+// Every candidate runs two intervals (the first may record a graph); the
+// second one's host-measured time per step is its score. The split does not
+// change any result, so tuning happens on live steps.
+void tune_begin(hg_pool *Q) {
+    Q->tune_cands.clear();
+    size_t largest = 0;
+    for (size_t s = 0; s < Q->S; s++)
+        if (Q->active[s]) largest = std::max<size_t>(largest, Q->meta[s].n);
+    Q->tune_cands.push_back({0u, Q->wide_cus});
+    for (uint32_t wm : {1024u, 2048u, 4096u, 8192u}) {
+        if (wm > largest) break;  // no system would go wide
+        for (int frac : {16, 8, 4}) Q->tune_cands.push_back({wm, std::max(1, Q->n_cus / frac)});
+    }
+    Q->tune_scores.assign(Q->tune_cands.size(), 0.0);
+    Q->tune_pos = 0;
+    Q->tune_phase = Q->tune_cands.size() > 1 ? TUNE_RUNNING : TUNE_DONE;
+    if (Q->tune_phase == TUNE_DONE) {
+        Q->wide_min = 0;
+        Q->order_dirty = true;
+    }
+}
+
+void tune_select(hg_pool *Q) {
+    const auto &c = Q->tune_cands[Q->tune_pos / 2];
+    if (Q->wide_min != c.first) {
+        Q->wide_min = c.first;
+        Q->order_dirty = true;
+    }
+    Q->wide_cus = c.second;
+}
+
+void tune_record(hg_pool *Q, double seconds_per_step) {
+    if (Q->tune_pos % 2 == 1) Q->tune_scores[Q->tune_pos / 2] = seconds_per_step;
+    if (++Q->tune_pos < 2 * Q->tune_cands.size()) return;
+    size_t best = 0;
+    for (size_t k = 1; k < Q->tune_cands.size(); k++)
+        if (Q->tune_scores[k] < Q->tune_scores[best]) best = k;
+    Q->wide_min = Q->tune_cands[best].first;
+    Q->wide_cus = Q->tune_cands[best].second;
+    Q->order_dirty = true;
+    Q->tune_phase = TUNE_DONE;
+}
+}  // namespace
+
 hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
     Q->last_chunk_ms = Q->last_rebuild_ms = Q->last_wide_ms = 0.0;
@@ -1001,44 +1088,20 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
     const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     if (nl && rebuild_every && nsteps % rebuild_every)
         return fail(HG_ERR_INVALID, "hg_pool_step: nsteps must be a multiple of rebuild_every");
-    if (Q->order_dirty) {
-        Q->order.clear();
-        for (size_t s = 0; s < Q->S; s++)
-            if (Q->active[s]) Q->order.push_back(uint32_t(s));
-        std::stable_sort(Q->order.begin(), Q->order.end(),
-                         [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
-        // largest first: the wide systems lead, the chunk kernel takes the rest
-        std::vector<wide_block> map;
-        size_t first_narrow = 0;
-        while (first_narrow < Q->order.size() && Q->wide_min &&
-               Q->meta[Q->order[first_narrow]].n >= Q->wide_min) {
-            const uint32_t s = Q->order[first_narrow++];
-            for (uint32_t st = 0; st < Q->meta[s].n; st += kWideBlock) map.push_back(wide_block{s, st});
-        }
-        if (map.size() > Q->wide_map_cap) return fail(HG_ERR_INVALID, "hg_pool_step: wide map overflow");
-        Q->narrow_count = uint32_t(Q->order.size() - first_narrow);
-        Q->wide_blocks = unsigned(map.size());
-        if (Q->narrow_count)
-            HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data() + first_narrow, size_t(Q->narrow_count) * 4,
-                                  hipMemcpyHostToDevice, Q->stream));
-        if (!map.empty())
-            HG_HIP(hipMemcpyAsync(Q->wide_map_d, map.data(), map.size() * sizeof(wide_block), hipMemcpyHostToDevice,
-                                  Q->stream));
-        HG_HIP(hipStreamSynchronize(Q->stream));
-        Q->order_dirty = false;
-    }
-    const uint32_t count = Q->narrow_count;
-    if (Q->order.empty()) return HG_OK;
-    bool any_wide_initial = false;
-    for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
-    // chunks: one rebuild interval each in neighbour-list mode; bounded
-    // lengths otherwise so a launch stays short
     const uint64_t chunk = (nl && rebuild_every) ? rebuild_every : std::min<uint64_t>(nsteps, 2000);
     const pool_view V = make_pool_view(Q);
-    const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
     const double half = dt / 2.0;
+    if (Q->tune_phase == TUNE_START) tune_begin(Q);
     for (uint64_t done = 0; done < nsteps;) {
         const uint32_t len = uint32_t(std::min<uint64_t>(chunk, nsteps - done));
+        if (Q->tune_phase == TUNE_RUNNING) tune_select(Q);
+        if (Q->order_dirty) PG_TRY(refresh_order(Q));
+        if (Q->order.empty()) return HG_OK;
+        const uint32_t count = Q->narrow_count;
+        const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
+        bool any_wide_initial = false;
+        for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
+        const auto t_interval = std::chrono::steady_clock::now();
         if (nl) {
             bool any = false;
             for (uint32_t s : Q->order) {
@@ -1074,14 +1137,44 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
                 HG_HIP(force(-1));
                 Q->last_wide_launches++;
             }
-            for (uint32_t st = 0; st < len; st++) {
-                k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, s2>>>(V, Q->wide_map_d, st > 0 ? 1 : 0, half,
-                                                                               dt);
-                HG_HIP(hipGetLastError());
-                HG_HIP(force(int(st)));
+            // the interval's 2*len+1 launches: recorded once into a HIP graph
+            // per (interval length, dt, wide set) and replayed; launched one
+            // by one when graphs are off
+            auto enqueue = [&](hipStream_t st_) -> hipError_t {
+                for (uint32_t st = 0; st < len; st++) {
+                    k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, st > 0 ? 1 : 0,
+                                                                                    half, dt);
+                    hipError_t e = hipGetLastError();
+                    if (e != hipSuccess) return e;
+                    e = nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks,
+                                                                         int(st), st_)
+                           : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, int(st), st_);
+                    if (e != hipSuccess) return e;
+                }
+                k_pool_wide_final_kick<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, half, dt);
+                return hipGetLastError();
+            };
+            if (Q->wide_graph) {
+                if (!Q->wg_exec || Q->wg_len != len || Q->wg_dt != dt || Q->wg_blocks != Q->wide_blocks) {
+                    if (Q->wg_exec) HG_HIP(hipGraphExecDestroy(Q->wg_exec));
+                    Q->wg_exec = nullptr;
+                    hipGraph_t graph = nullptr;
+                    HG_HIP(hipStreamBeginCapture(Q->stream_w, hipStreamCaptureModeThreadLocal));
+                    const hipError_t ec = enqueue(Q->stream_w);
+                    const hipError_t ee = hipStreamEndCapture(Q->stream_w, &graph);
+                    HG_HIP(ec);
+                    HG_HIP(ee);
+                    const hipError_t ei = hipGraphInstantiate(&Q->wg_exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    HG_HIP(ei);
+                    Q->wg_len = len;
+                    Q->wg_dt = dt;
+                    Q->wg_blocks = Q->wide_blocks;
+                }
+                HG_HIP(hipGraphLaunch(Q->wg_exec, s2));
+            } else {
+                HG_HIP(enqueue(s2));
             }
-            k_pool_wide_final_kick<<<Q->wide_blocks, kWideBlock, 0, s2>>>(V, Q->wide_map_d, half, dt);
-            HG_HIP(hipGetLastError());
             Q->last_wide_launches += 2 * len + 1;
             HG_HIP(hipEventRecord(Q->wev[1], s2));
         }
@@ -1108,7 +1201,8 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
             HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
             Q->last_wide_ms += ms;
         }
-        any_wide_initial = false;
+        if (Q->tune_phase == TUNE_RUNNING)
+            tune_record(Q, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_interval).count() / len);
         // host mirror of what the kernels did to each system
         for (uint32_t s : Q->order) {
             pool_meta &M = Q->meta[s];
@@ -1130,6 +1224,26 @@ hg_status hg_pool_set_wide_threshold(hg_pool *Q, size_t wide_min) {
     if (wide_min > 0xffffffffu) wide_min = 0;
     Q->wide_min = uint32_t(wide_min);
     Q->order_dirty = true;
+    return HG_OK;
+}
+
+hg_status hg_pool_autotune(hg_pool *Q, int enable) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    Q->tune_phase = enable ? TUNE_START : TUNE_OFF;
+    return HG_OK;
+}
+
+hg_status hg_pool_get_tuning(hg_pool *Q, size_t *wide_min, int *wide_cus, int *tuned) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (wide_min) *wide_min = Q->wide_min;
+    if (wide_cus) *wide_cus = Q->wide_cus;
+    if (tuned) *tuned = Q->tune_phase == TUNE_DONE ? 1 : 0;
+    return HG_OK;
+}
+
+hg_status hg_pool_set_wide_graph(hg_pool *Q, int enable) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    Q->wide_graph = enable != 0;
     return HG_OK;
 }
 

@@ -276,3 +276,30 @@ def test_pool_wide_all_pairs_matches_oracle():
     o = O.OracleSystem(O.system_params(O.BIN_CONTACT, O.HAM_FULL), x, v)
     o.step(DT, 3000)
     same(o.state(), pool.state(slot), "wide all pairs")
+
+
+def test_pool_autotune_and_graph_do_not_change_results():
+    """Results with the online autotuner running (its candidates switch the
+    split every other interval) and with graphs off equal the plain run."""
+    announce("pool autotune / graph invariance")
+    L = _L()
+    cases = [agg_case(n, 90 + k) for k, n in enumerate((3000, 1300, 150, 2200, 90))]
+    outs = []
+    for mode in ("plain", "tuned", "nograph"):
+        pool = L.Pool(nl_params(1), max_particles=8000, max_systems=8)
+        slots = [pool.add(x, v) for x, v in cases]
+        if mode == "tuned":
+            pool.autotune(True)
+        if mode == "nograph":
+            pool.set_wide_threshold(1000)
+            pool.set_wide_graph(False)
+        pool.step(DT, 1200, rebuild_every=50)
+        if mode == "tuned":
+            t = pool.tuning()
+            assert t["tuned"], t
+        outs.append([pool.state(s) for s in slots])
+        pool.close()
+    for other in outs[1:]:
+        for k, (a, b) in enumerate(zip(outs[0], other)):
+            for f in ("x", "v", "omega", "a"):
+                np.testing.assert_array_equal(b[f], a[f], err_msg=f"system {k} field {f}")
