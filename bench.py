@@ -8,15 +8,19 @@ aggregates with seeds 0..4095; r = 14 nm, rho = 1700 kg/m^3; alt_sinter bonds
 (mu = 1), limited-range Hamaker (A = 1e-20 J, cutoff 3r, h0 = 0.1 nm),
 neighbour lists of radius 3.2r rebuilt every 50 steps, dt = 1e-13 s, Gaussian
 initial velocities (sigma 1 m/s). All aggregates are stepped together in one
-pool, mixed precision (FP32 forces, FP64 state and springs), 2,000 steps.
+pool, mixed precision (FP32 forces, FP64 state and springs), 2,000 steps. The
+pool's launch split (which aggregates take the wide per-step launches) is
+tuned online during the warm-up steps.
 
 Beside it on the same line: the same batch in an FP64 pool, the plain FP64
 GPU path (one system at a time, per-step launches) on a stratified sample of
-the batch, the single N=20,000 aggregate (FP64 and mixed), the roofline of the
+the batch, the single N=20,000 aggregate (FP64 and mixed), the serving stream
+(Poisson arrivals of plate compression/tension tests through the
+continuous-batching server, p50/p99 completion latency), the roofline of the
 dominant kernel from HIP events on its own stream, and the CPU oracle on this
 host's cores on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 1500]
 """
 import argparse
 import json
@@ -47,17 +51,19 @@ def aggregate_params(k=1.0e4, zeta=0.5, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.
     return contact, sinter, ham
 
 
-def params(L, precision):
+def params(L, precision, facets=False):
     c, s, h = aggregate_params()
+    extra = {}
+    if facets:
+        extra = dict(facet_contact=1, facet_hamaker=1, surface_contact=c,
+                     surface_hamaker=[1.0e-20, 1.0e-10, 3.0 * R_PART, R_PART, MASS])
     return L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R_PART,
-                         contact=c, sinter=s, hamaker=h, precision=precision)
+                         contact=c, sinter=s, hamaker=h, precision=precision, **extra)
 
 
 def batch_workload(L, count=4096):
-    rng = np.random.default_rng(2024)
-    z = rng.standard_normal(count)
-    sizes = np.clip(np.round(np.exp(np.log(400.0) + z)), 50, 20000).astype(np.uint32)
-    dfs = rng.uniform(1.6, 2.2, count)
+    from libgran_amd import workloads as W
+    sizes, dfs = W.batch_sizes(count)
     xs = L.generate_aggregates(sizes, dfs, 1.3, np.arange(count, dtype=np.uint64), R_PART)
     vs = [np.random.default_rng(1000 + k).normal(0.0, 1.0, x.shape) for k, x in enumerate(xs)]
     return sizes, xs, vs
@@ -71,23 +77,33 @@ def state_bytes_per_particle_step(kbar, cbar):
     return 288 + 4 * kbar + 144 * cbar + (24 + 4 * kbar) / REBUILD
 
 
-def run_pool(L, precision, sizes, xs, vs, steps, warmup):
+def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None):
+    import torch
     pool = L.Pool(params(L, precision), max_particles=int(sizes.sum()) + 1024, max_systems=len(xs))
     slots = [pool.add(x, v) for x, v in zip(xs, vs)]
+    if split is None:
+        pool.autotune(True)  # tunes the launch split on the warm-up steps; results do not depend on it
+    else:
+        pool.set_wide_threshold(split[0])
+        pool.set_wide_cus(split[1])
     pool.step(DT, warmup, rebuild_every=REBUILD)
-    import torch
+    pool.autotune(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pool.step(DT, steps, rebuild_every=REBUILD)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     prof = pool.last_step_profile()
-    # measured neighbour and spring counts on a sample of systems
-    sample = slots[:: max(1, len(slots) // 64)]
+    tuning = pool.tuning()
+    # measured neighbour and spring counts on a sample of the chunk kernel's systems
+    wm = tuning["wide_min"]
+    narrow = [s for s, n in zip(slots, sizes) if not wm or n < wm]
+    sample = narrow[:: max(1, len(narrow) // 64)]
     npart = sum(pool.info(s)[0] for s in sample)
     kbar = sum(len(pool.neighbors(s)[1]) for s in sample) / npart
     cbar = sum(len(pool.springs(s, 0)[0]) + len(pool.springs(s, 1)[0]) for s in sample) / npart
-    return pool, el, prof, kbar, cbar
+    narrow_particles = int(sum(int(n) for n in sizes if not wm or n < wm))
+    return pool, el, prof, kbar, cbar, tuning, narrow_particles
 
 
 def plain_fp64_sample(L, sizes, xs, vs, steps, nsample=48):
@@ -127,6 +143,45 @@ def single_aggregate(L, precision, steps, warmup):
     return 20000 * steps / el, x, v0, launches, ms
 
 
+def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0, seed=7):
+    """The serving stream: `count` plate tests (N log-uniform 100..20000, Df
+    U(1.6, 2.2), plates at U(1, 10) m/s, compression or tension) arriving as
+    a Poisson stream at `rate` per second, each run for `steps` steps through
+    the continuous-batching server with an HBM budget of `budget_gib` GiB for
+    resident requests. Latency = result ready - arrival."""
+    from libgran_amd import workloads as W
+    reqs = W.serving_stream(count, rate, R_PART, seed=seed)
+    p = params(L, L.PRECISION_MIXED, facets=True)
+    srv = L.Server(p, budget_bytes=int(budget_gib * (1 << 30)), max_requests=count, max_facets=4, quantum=quantum,
+                   rebuild_every=REBUILD, dt=DT)
+    ids = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        wait = r["arrival"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(srv.submit(r["x"], steps, facets=r["facets"]))
+    res = [srv.wait(i) for i in ids]
+    stats = srv.stats()
+    srv.close()
+    times = np.array([q["times"] for q in res])
+    lat = times[:, 2] - times[:, 0]
+    queue = times[:, 1] - times[:, 0]
+    span = times[:, 2].max() - times[:, 0].min()
+    ps = sum(r["n"] for r in reqs) * steps
+    return {"requests": count, "arrival_rate_per_s": rate, "steps_per_request": steps,
+            "particles_mean": round(float(np.mean([r["n"] for r in reqs])), 1),
+            "budget_gib": budget_gib, "quantum_steps": quantum,
+            "throughput": round(ps / span, 1), "unit": "particle-steps/s", "makespan_s": round(float(span), 3),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+            "queue_wait_p50_s": round(float(np.percentile(queue, 50)), 3),
+            "queue_wait_p99_s": round(float(np.percentile(queue, 99)), 3),
+            "offered_load": round(rate * float(np.mean([r["n"] for r in reqs])) * steps, 1),
+            "device_ms": round(stats["device_ms"], 1), "rounds": int(stats["rounds"]),
+            "precision": "mixed (FP32 forces, FP64 state/springs)"}
+
+
 def cpu_baseline(x, v0, steps):
     """The CPU oracle (a restatement of the reference, OpenMP over particles)
     on the N=20,000 aggregate, timed on this host."""
@@ -151,10 +206,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=1500)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=6000)
+    ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB device-to-device copy (known traffic for scaling HBM counters)")
@@ -186,24 +243,31 @@ def main():
     t_gen = time.perf_counter() - t_gen
     total = int(sizes.sum())
 
-    pool, el_mx, prof_mx, kbar, cbar = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps, args.warmup)
+    split = tuple(int(q) for q in args.split.split(":")) if args.split else None
+    pool, el_mx, prof_mx, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps,
+                                                                 args.warmup, split)
     pool.close()
     value = total * args.steps / el_mx
     chunk_us = prof_mx["chunk_ms"] * 1e3 / max(prof_mx["chunks"], 1)
     bpps = state_bytes_per_particle_step(kbar, cbar)
-    bytes_launch = bpps * total * REBUILD
+    bytes_launch = bpps * narrow * REBUILD
     achieved = bytes_launch / (chunk_us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate per launch)",
+                "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate below the "
+                          "wide threshold per launch)",
                 "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
+                "particles_per_launch": narrow,
                 "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
                 "kbar": round(kbar, 3), "cbar": round(cbar, 3),
-                "rebuild_us_per_interval": round(prof_mx["rebuild_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1)}
-    extra = {"aggregates": len(xs), "particles": total, "generation_s": round(t_gen, 1)}
+                "rebuild_us_per_interval": round(prof_mx["rebuild_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1),
+                "wide_us_per_interval": round(prof_mx["wide_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1)}
+    extra = {"aggregates": len(xs), "particles": total, "generation_s": round(t_gen, 1),
+             "launch_split": {"wide_min": tuning["wide_min"], "wide_cus": tuning["wide_cus"],
+                              "chunk_particles": narrow, "wide_particles": total - narrow}}
     cpu = None
     if not args.only_batched:
-        pool, el_64, prof_64, _, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup)
+        pool, el_64, _, _, _, tuning64, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup)
         pool.close()
         plain, plain_note = plain_fp64_sample(L, sizes, xs, vs, 200)
         single64, x20, v20, launches, ms = single_aggregate(L, L.PRECISION_FP64, 5000, 200)
@@ -216,6 +280,8 @@ def main():
             "single_n20000_fp64": round(single64, 1), "single_n20000_mixed": round(single_mx, 1),
             "single_n20000_force_kernel_us": round(float(ms[0]) * 1e3 / max(int(launches[0]), 1), 2),
         })
+        if args.serving_requests > 0:
+            extra["serving"] = serving_leg(L, args.serving_requests)
         if not args.no_cpu_baseline:
             cpu = cpu_baseline(x20, v20, args.cpu_steps)
     out = {"metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
