@@ -26,6 +26,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -182,6 +183,25 @@ def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0,
             "precision": "mixed (FP32 forces, FP64 state/springs)"}
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counters.txt")
+
+
+def measured_traffic(tuning, count):
+    """HBM bytes per chunk launch from the committed FETCH_SIZE/WRITE_SIZE
+    profile (copy-calibrated), valid for the split it was measured at."""
+    try:
+        txt = open(COUNTERS).read()
+    except OSError:
+        return None, "no counter profile"
+    m = re.search(r"traffic per launch\s*=\s*([\d,]+) B", txt)
+    split = re.search(r"launch split fixed at\s*(\d+):(\d+)", txt)
+    if not m or not split:
+        return None, "counter profile unreadable"
+    if count != 4096 or (tuning["wide_min"], tuning["wide_cus"]) != (int(split.group(1)), int(split.group(2))):
+        return None, f"counter profile was measured at split {split.group(1)}:{split.group(2)}, this run differs"
+    return int(m.group(1).replace(",", "")), "profiles/pool_chunk_mixed_batch4096_hbm_counters.txt"
+
+
 def cpu_baseline(x, v0, steps):
     """The CPU oracle (a restatement of the reference, OpenMP over particles)
     on the N=20,000 aggregate, timed on this host."""
@@ -213,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
+    ap.add_argument("--only-single", action="store_true", help="only the N=20,000 FP64 per-step path (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB device-to-device copy (known traffic for scaling HBM counters)")
     args = ap.parse_args()
@@ -238,6 +259,16 @@ def main():
     import torch  # noqa: F401  (torch.cuda.synchronize brackets the timed regions)
     import libgran_amd as L
 
+    if args.only_single:
+        # the plain FP64 per-step path on the N=20,000 aggregate alone (profiling runs)
+        rate, _, _, launches, ms = single_aggregate(L, L.PRECISION_FP64, args.steps, args.warmup)
+        print(json.dumps({"workload": "single aggregate N=20000 (seed 1, Df 1.8), FP64, per-step launches",
+                          "value": round(rate, 1), "unit": "particle-steps/s", "steps": args.steps,
+                          "phase_launches": [int(q) for q in launches],
+                          "phase_avg_us": [round(float(m) * 1e3 / max(int(q), 1), 2) for q, m in zip(launches, ms)],
+                          "phases": ["force", "kick", "rebuild", "reserved"]}))
+        return
+
     t_gen = time.perf_counter()
     sizes, xs, vs = batch_workload(L, args.count)
     t_gen = time.perf_counter() - t_gen
@@ -252,8 +283,9 @@ def main():
     bpps = state_bytes_per_particle_step(kbar, cbar)
     bytes_launch = bpps * narrow * REBUILD
     achieved = bytes_launch / (chunk_us * 1e-6) / 1e9
+    traffic, traffic_note = measured_traffic(tuning, args.count)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                 "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate below the "
                           "wide threshold per launch)",
                 "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
