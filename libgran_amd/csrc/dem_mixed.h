@@ -33,6 +33,21 @@ HGD f3 tof(d3 a) { return {float(a.x), float(a.y), float(a.z)}; }
 HGD d3 tod(f3 a) { return {double(a.x), double(a.y), double(a.z)}; }
 HGD f3 zerof() { return {0.f, 0.f, 0.f}; }
 
+// Reciprocal square roots without the correctly rounded division/sqrt
+// expansions (this path is not bit-exact by design): the hardware estimates
+// refined by Newton steps. FP64: v_rsq_f64 then two steps
+// y <- y (3 - a y^2) / 2, each squaring the relative error, to ~1e-16.
+// FP32: v_rsq_f32 / v_rcp_f32 (1 ulp). Arguments must be > 0.
+__device__ __forceinline__ double rsqrt64(double a) {
+#pragma clang fp contract(fast)
+    double y = __builtin_amdgcn_rsq(a);
+    y = y * (1.5 - 0.5 * a * y * y);
+    y = y * (1.5 - 0.5 * a * y * y);
+    return y;
+}
+__device__ __forceinline__ float rsqrt32(float a) { return __builtin_amdgcn_rsqf(a); }
+__device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
+
 struct accf {
     f3 lin, ang;
 };
@@ -89,9 +104,13 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
 // full length: the result's direction is that residual's, and an FP32 normal
 // (error ~6e-8 rad) would perturb it by ~1%.
 __device__ __forceinline__ d3 rotate_spring(d3 xi, d3 n) {
+#pragma clang fp contract(fast)
     d3 xr = xi - dot(xi, n) * n;
     const double l2 = dot(xr, xr);
-    if (l2 > 0.0) xr = xr * sqrt(dot(xi, xi) / l2);
+    if (l2 > 0.0) {
+        const double x2 = dot(xi, xi);  // > 0 whenever l2 > 0
+        xr = xr * (x2 * rsqrt64(x2) * rsqrt64(l2));
+    }
     return xr;
 }
 
@@ -105,8 +124,10 @@ __device__ __forceinline__ f3 shear_contact_mx(d3 &xi, d3 n64, float k, float g,
     xi = rotate_spring(xi, n64);
     const f3 f0 = (-k) * tof(xi) - g * rv;
     const float sf = mus * fn, df = mud * fn;
-    const float f0n = sqrtf(dotf(f0, f0));
-    const f3 t = f0n > 0.f ? f0 * (1.0f / f0n) : zerof();
+    const float q = dotf(f0, f0);
+    const float iq = q > 0.f ? rsqrt32(q) : 0.f;
+    const float f0n = q * iq;
+    const f3 t = f0 * iq;
     float fsel;
     if (f0n <= sf) {
         xi = xi + rv64 * dt;
@@ -114,8 +135,8 @@ __device__ __forceinline__ f3 shear_contact_mx(d3 &xi, d3 n64, float k, float g,
     } else {
         const double kd = double(k), gd = double(g);
         const d3 f064 = (-kd) * xi - gd * rv64;
-        const double l = sqrt(dot(f064, f064));
-        const d3 t64 = l > 0.0 ? f064 * (1.0 / l) : zero3();
+        const double l2 = dot(f064, f064);
+        const d3 t64 = l2 > 0.0 ? f064 * rsqrt64(l2) : zero3();
         xi = (-1.0 / kd) * (double(df) * t64 + gd * rv64);
         fsel = df;
     }
@@ -133,9 +154,11 @@ __device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, d3 n64, float k, float g, 
 
 // Shared kinematics of contact and bonded pairs: n from the FP64 distance.
 __device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, d3 &n64, f3 &n) {
+#pragma clang fp contract(fast)
     const d3 d = xi - xj;
-    dist = sqrt(dot(d, d));
-    const double inv = dist > 0.0 ? 1.0 / dist : 0.0;
+    const double d2 = dot(d, d);
+    const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
+    dist = d2 * inv;
     n64 = d * inv;
     n = tof(n64);
 }
@@ -193,7 +216,7 @@ __device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 
     if (hd < m.h_h0) hd = m.h_h0;
     const float h = float(hd), rp = m.h_r;
     const float a = 4.0f * rp + h, b = 2.0f * rp + h;
-    const float ia = 1.0f / a, ib = 1.0f / b, ih = 1.0f / h;
+    const float ia = rcp32(a), ib = rcp32(b), ih = rcp32(h);
     const float s = m.h_a6 * ((a + h) * ia * ih - 2.0f * ib - 4.0f * rp * rp * ib * ib * ib -
                               2.0f * rp * rp * (a + h) * ia * ia * ih * ih);
     // the reference's direction is normalized(x_j - x_i) = -n
@@ -204,13 +227,15 @@ __device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf,
                                                    spring9 &sp) {
 #pragma clang fp contract(fast)
     const d3 d = xi - xf;
-    const double dist = sqrt(dot(d, d));
+    const double d2 = dot(d, d);
+    const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
+    const double dist = d2 * inv;
     const double ovd = m.sc_rd - dist;
     if (ovd <= 0.0) {
         sp = zero_spring();
         return {zerof(), zerof()};
     }
-    const d3 n64 = d * (1.0 / dist);
+    const d3 n64 = d * inv;
     const f3 n = tof(n64);
     const float ov = float(ovd), r = m.sc_r;
     const double rppd = m.sc_rd - ovd;
@@ -233,13 +258,15 @@ __device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf,
 __device__ __forceinline__ f3 surface_hamaker_mx(const mixed_consts &m, d3 xf, d3 xi) {
 #pragma clang fp contract(fast)
     const d3 r = xf - xi;
-    const double dist = sqrt(dot(r, r));
+    const double d2 = dot(r, r);
+    const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
+    const double dist = d2 * inv;
     double hd = dist - m.sh_r;
     if (hd < m.sh_h0) hd = m.sh_h0;
     const float h = float(hd), rp = float(m.sh_r);
     const float q = h + 2.0f * rp;
-    const float s = m.sh_a23 * (rp * rp * rp) / (h * h * q * q);
-    return (s * m.sh_im) * tof(r * (1.0 / dist));
+    const float s = m.sh_a23 * (rp * rp * rp) * rcp32(h * h * q * q);
+    return (s * m.sh_im) * tof(r * inv);
 }
 
 }  // namespace hg

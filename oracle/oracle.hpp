@@ -195,8 +195,10 @@ inline v3 shear_sinter(v3 &xi, v3 n, double stiffness, double damping, v3 rel_v,
 }
 
 // alt_sinter_bridge.h:92-129, bonded branch (no overlap gate).
+// alt_breaking_sinter_bridge.h:85-125 evaluates the same expressions and also
+// needs the force before the division by the mass: *ftot receives f_n n + f_t.
 inline acc2 sinter_bonded_eval(const sinter_params &p, v3 xi_, v3 xj, v3 vi, v3 vj, v3 wi, v3 wj,
-                               spring3 &sp) {
+                               spring3 &sp, v3 *ftot = nullptr) {
     v3 n = normalized(xi_ - xj);
     double overlap = 2.0 * p.r_part - dot(xi_ - xj, n);
     double rpp = p.r_part - overlap / 2.0;
@@ -212,15 +214,21 @@ inline acc2 sinter_bonded_eval(const sinter_params &p, v3 xi_, v3 xj, v3 vi, v3 
     v3 tau_t = rpp * cross(n, f_t);
     v3 tau_r = p.r_part * cross(n, f_r);
     v3 tau_o = p.r_part * f_o;
-    return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+    const v3 f_tot = f_n * n + f_t;
+    if (ftot) *ftot = f_tot;
+    return {f_tot / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
 }
 
 // alt_sinter_bridge.h:56-90: lexicographic scan over i<j; a pair whose gap is
 // within d_crit is bonded unless both ends already share a component (cycle),
 // which is skipped (the reference prints a warning). Components merge by
 // relabelling every member of j's subset with i's label, as the reference does.
+// With prevent_cycles = false it is the bond search of
+// alt_breaking_sinter_functor (alt_breaking_sinter_bridge.h:63-75): every pair
+// within d_crit is bonded.
 inline void sinter_topology(size_t n, const std::vector<v3> &x0, double r_part, double d_crit,
-                            std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped) {
+                            std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped,
+                            bool prevent_cycles = true) {
     std::vector<size_t> subset(n);
     for (size_t i = 0; i < n; i++) subset[i] = i;
     bonds.clear();
@@ -229,7 +237,7 @@ inline void sinter_topology(size_t n, const std::vector<v3> &x0, double r_part, 
     for (size_t i = 0; i + 1 < n; i++) {
         for (size_t j = i + 1; j < n; j++) {
             if (std::fabs(norm(x0[i] - x0[j]) - 2.0 * r_part) < d_crit) {
-                if (subset[i] == subset[j]) {
+                if (prevent_cycles && subset[i] == subset[j]) {
                     skipped++;
                     continue;
                 }
@@ -361,6 +369,7 @@ struct system_params {
     int neighbor_mode;  // 0: all pairs (granular_system), 1: list (granular_system_neighbor_list)
     int facet_contact;  // facets carry a surface contact functor
     int facet_hamaker;  // facets carry a surface hamaker functor
+    int sinter_breaking;  // BIN_SINTER as alt_breaking_sinter_functor
     double r_verlet;
     contact_params contact;
     sinter_params sinter;
@@ -399,12 +408,18 @@ public:
         sinter_springs.resize(n);
         bonded_to.assign(n, {});
         if (p.binary_model == BIN_SINTER) {
-            sinter_topology(n, x, p.sinter.r_part, p.sinter.d_crit, bonds, bonds_skipped);
+            sinter_topology(n, x, p.sinter.r_part, p.sinter.d_crit, bonds, bonds_skipped, !p.sinter_breaking);
             for (auto &b : bonds) {
                 bonded_to[b.first].push_back(b.second);
                 bonded_to[b.second].push_back(b.first);
             }
             for (auto &r : bonded_to) std::sort(r.begin(), r.end());
+            neck.assign(n, {});
+            for (auto &b : bonds) {
+                neck[b.first].push_back({b.second, INFINITY});
+                neck[b.second].push_back({b.first, INFINITY});
+            }
+            for (auto &r : neck) std::sort(r.begin(), r.end());
         }
         if (p.neighbor_mode == 1) rebuild_neighbors(true);
     }
@@ -414,13 +429,43 @@ public:
         return std::binary_search(r.begin(), r.end(), j);
     }
 
+    // Neck strength of bond {i, j} (both ordered pairs share it). The
+    // reference stores one draw per bond at neck_strengths[i] and
+    // neck_strengths[j] but reads neck_strengths[i*n + j]
+    // (alt_breaking_sinter_bridge.h:71-72, 118); here each bond keeps its own
+    // draw, which is what the read intends.
+    std::vector<std::vector<std::pair<uint32_t, double>>> neck;
+    double neck_strength(size_t i, uint32_t j) const {
+        const auto &r = neck[i];
+        auto it = std::lower_bound(r.begin(), r.end(), j,
+                                   [](const std::pair<uint32_t, double> &e, uint32_t key) { return e.first < key; });
+        return (it != r.end() && it->first == j) ? it->second : INFINITY;
+    }
+    void set_neck_strengths(const double *s) {
+        for (auto &r : neck) r.clear();
+        for (size_t b = 0; b < bonds.size(); b++) {
+            neck[bonds[b].first].push_back({bonds[b].second, s[b]});
+            neck[bonds[b].second].push_back({bonds[b].first, s[b]});
+        }
+        for (auto &r : neck) std::sort(r.begin(), r.end());
+    }
+
     // One ordered pair through the binary container fold.
     acc2 pair(size_t i, uint32_t j) {
         bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
         acc2 b{vzero, vzero}, h{vzero, vzero};
         if (have_b) {
             if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
-                b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], sinter_springs.at(i, j));
+                v3 ftot;
+                b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], sinter_springs.at(i, j), &ftot);
+                // alt_breaking_sinter_bridge.h:117-119: the ordered pair (i, j)
+                // stops being bonded once its force exceeds the neck strength;
+                // from then on the embedded contact functor takes it, with
+                // contact springs that were never touched (zero).
+                if (p.sinter_breaking && norm(ftot) > neck_strength(i, j)) {
+                    auto &r = bonded_to[i];
+                    r.erase(std::lower_bound(r.begin(), r.end(), j));
+                }
             } else {
                 spring3 tmp{{vzero, vzero, vzero}};
                 auto &row = contact_springs.rows[i];

@@ -103,6 +103,42 @@ void or_sinter_seq(const double *sinter13, const double *contact18, size_t nstep
     }
 }
 
+// alt_breaking_sinter on the ordered pair (0, 1) only, with neck strength
+// `strength`: out = 6 doubles per step (a, alpha of particle 0); broken_out[k]
+// = 1 once the pair has broken (after step k's evaluation). Bonding as the
+// reference decides it on the first state (every pair within d_crit).
+void or_breaking_seq(const double *sinter13, const double *contact18, double strength, size_t nsteps,
+                     const double *states, double *out, int *broken_out) {
+    sinter_params sp = sp_from(sinter13);
+    contact_params cp = cp_from(contact18);
+    std::vector<v3> x0 = {ld(states), ld(states + 3)};
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped;
+    sinter_topology(2, x0, sp.r_part, sp.d_crit, bonds, skipped, false);
+    bool bonded = !bonds.empty();
+    spring3 s01{{vzero, vzero, vzero}}, c01{{vzero, vzero, vzero}};
+    for (size_t k = 0; k < nsteps; k++) {
+        const double *s = states + 18 * k;
+        v3 x0_ = ld(s), x1 = ld(s + 3), v0 = ld(s + 6), v1 = ld(s + 9), w0 = ld(s + 12), w1 = ld(s + 15);
+        acc2 e01;
+        if (bonded) {
+            v3 ftot;
+            e01 = sinter_bonded_eval(sp, x0_, x1, v0, v1, w0, w1, s01, &ftot);
+            if (norm(ftot) > strength) bonded = false;
+        } else {
+            bool r;
+            e01 = contact_eval(cp, x0_, x1, v0, v1, w0, w1, c01, r);
+        }
+        st(out + 6 * k, e01.lin);
+        st(out + 6 * k + 3, e01.ang);
+        broken_out[k] = bonded ? 0 : 1;
+    }
+}
+
+void or_sys_set_neck_strengths(void *s, const double *strengths) {
+    static_cast<dem_system *>(s)->set_neck_strengths(strengths);
+}
+
 // mode 1: hamaker_functor, mode 2: limited_range_hamaker_functor.
 void or_hamaker_seq(int mode, const double *ham5, size_t nsteps, const double *states, double *out) {
     hamaker_params hp = hp_from(ham5);

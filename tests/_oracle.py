@@ -73,7 +73,8 @@ class HamakerP(C.Structure):
 
 class SystemP(C.Structure):
     _fields_ = [("binary_model", C.c_int), ("hamaker_model", C.c_int), ("neighbor_mode", C.c_int),
-                ("facet_contact", C.c_int), ("facet_hamaker", C.c_int), ("r_verlet", C.c_double),
+                ("facet_contact", C.c_int), ("facet_hamaker", C.c_int), ("sinter_breaking", C.c_int),
+                ("r_verlet", C.c_double),
                 ("contact", ContactP), ("sinter", SinterP), ("hamaker", HamakerP),
                 ("surface_contact", ContactP), ("surface_hamaker", HamakerP)]
 
@@ -89,8 +90,10 @@ HAM_NONE, HAM_FULL, HAM_LIMITED = 0, 1, 2
 
 
 def system_params(binary_model, hamaker_model, neighbor_mode=0, r_verlet=0.0, contact=None, sinter=None,
-                  hamaker=None, facet_contact=0, facet_hamaker=0, surface_contact=None, surface_hamaker=None):
+                  hamaker=None, facet_contact=0, facet_hamaker=0, surface_contact=None, surface_hamaker=None,
+                  sinter_breaking=0):
     p = SystemP()
+    p.sinter_breaking = sinter_breaking
     p.binary_model, p.hamaker_model, p.neighbor_mode = binary_model, hamaker_model, neighbor_mode
     p.facet_contact, p.facet_hamaker, p.r_verlet = facet_contact, facet_hamaker, r_verlet
     _fill(p.contact, contact if contact is not None else contact_params())
@@ -137,6 +140,8 @@ def lib():
         L.or_fold_seq.argtypes = [_dp, _dp, _dp, C.c_size_t, _dp, _dp]
         L.or_facet_seq.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_size_t, _dp, _dp]
         L.or_closest_point.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        L.or_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
+        L.or_sys_set_neck_strengths.argtypes = [C.c_void_p, _dp]
         L.or_sinter_topology.restype = C.c_size_t
         L.or_sinter_topology.argtypes = [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t,
                                          C.POINTER(C.c_size_t)]
@@ -155,6 +160,7 @@ def ref():
         R.ref_lr_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
         R.ref_fold_seq.argtypes = [_dp, _dp, _dp, C.c_size_t, _dp, _dp]
         R.ref_facet_seq.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_size_t, _dp, _dp]
+        R.ref_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
         R.ref_sinter_topology.restype = C.c_size_t
         R.ref_sinter_topology.argtypes = [C.c_size_t, _dp, _dp, _dp, _u32p, C.c_size_t]
         _ref = R
@@ -184,6 +190,26 @@ def oracle_sinter_seq(sinter, contact, states):
     lib().or_sinter_seq(dptr(c64(sinter)), dptr(c64(contact)), n, dptr(states), dptr(out), dptr(springs),
                         C.byref(b))
     return out, springs, b.value
+
+
+def _breaking(fn, sinter, contact, strength, states):
+    states = c64(states)
+    n = states.shape[0]
+    out = np.zeros((n, 6))
+    broken = np.zeros(n, dtype=np.int32)
+    fn(dptr(c64(sinter)), dptr(c64(contact)), float(strength), n, dptr(states), dptr(out),
+       broken.ctypes.data_as(C.POINTER(C.c_int)))
+    return out, broken
+
+
+def oracle_breaking_seq(sinter, contact, strength, states):
+    """alt_breaking_sinter on the ordered pair (0, 1): (a, alpha) per step and
+    whether the bond has broken after that step."""
+    return _breaking(lib().or_breaking_seq, sinter, contact, strength, states)
+
+
+def ref_breaking_seq(sinter, contact, strength, states):
+    return _breaking(ref().ref_breaking_seq, sinter, contact, strength, states)
 
 
 def oracle_hamaker_seq(mode, ham, states):
@@ -297,6 +323,10 @@ class OracleSystem:
         if getattr(self, "_h", None):
             lib().or_sys_destroy(self._h)
             self._h = None
+
+    def set_neck_strengths(self, strengths):
+        """alt_breaking_sinter neck strengths, one per bond in bonds() order."""
+        lib().or_sys_set_neck_strengths(self._h, dptr(c64(strengths)))
 
     def set_facets(self, verts, vels):
         verts, vels = c64(verts), c64(vels)

@@ -64,6 +64,18 @@ def main():
         out[f"sinter_{seed}_states"] = s
         out[f"sinter_{seed}_out"] = o
         out[f"sinter_{seed}_springs"] = sp
+    # alt_breaking_sinter: the ordered pair (0, 1) with a neck strength that
+    # the sequence's force exceeds part-way (0.9 of the unbroken run's peak)
+    for seed in range(3):
+        s = _seq.bonded_sequence(250 + seed)
+        o_inf, _ = O.ref_breaking_seq(sinter, contact_sets["std"], float("inf"), s)
+        strength = 0.9 * float(np.max(np.linalg.norm(o_inf[:, :3], axis=1))) * O.MASS
+        o, broken = O.ref_breaking_seq(sinter, contact_sets["std"], strength, s)
+        assert 0 < broken.sum() < len(broken)
+        out[f"breaking_{seed}_states"] = s
+        out[f"breaking_{seed}_strength"] = np.array(strength)
+        out[f"breaking_{seed}_out"] = o
+        out[f"breaking_{seed}_broken"] = broken
     out["ham_params"] = ham
     for seed in range(2):
         s = _seq.far_sequence(300 + seed)

@@ -59,6 +59,37 @@ def test_golden_sinter_sequences_bit_exact():
         np.testing.assert_array_equal(springs, GOLD[f"sinter_{seed}_springs"])
 
 
+def test_golden_breaking_sinter_bit_exact():
+    """alt_breaking_sinter_bridge.h on the ordered pair (0, 1): bonded force,
+    the step at which |f_n n + f_t| first exceeds the neck strength, and the
+    contact evaluation that takes over (with untouched, zero springs)."""
+    sinter, contact = GOLD["sinter_params"], GOLD["contact_std_params"]
+    for seed in range(3):
+        s = GOLD[f"breaking_{seed}_states"]
+        out, broken = O.oracle_breaking_seq(sinter, contact, float(GOLD[f"breaking_{seed}_strength"]), s)
+        np.testing.assert_array_equal(out, GOLD[f"breaking_{seed}_out"], err_msg=f"seed {seed}")
+        np.testing.assert_array_equal(broken, GOLD[f"breaking_{seed}_broken"], err_msg=f"seed {seed}")
+        assert 0 < broken.sum() < len(broken)
+        # never breaking reduces to alt_sinter's bonded evaluation
+        out_inf, broken_inf = O.oracle_breaking_seq(sinter, contact, float("inf"), s)
+        o_sinter, _, _ = O.oracle_sinter_seq(sinter, contact, s)
+        assert broken_inf.sum() == 0
+        np.testing.assert_array_equal(out_inf, o_sinter[:, :6])
+
+
+def test_breaking_topology_bonds_every_touching_pair():
+    """alt_breaking_sinter_bridge.h:63-75 bonds every pair within d_crit (no
+    cycle check): the 3x3x3 lattice gets all 54 touching pairs."""
+    import _scenarios as S
+    x, v = S.sinter_lattice()
+    sp = O.sinter_params(k_bond=1000.0, d_crit=1.0e-9, dt=5e-14)
+    o = O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_NONE, sinter=sp, sinter_breaking=1), x, v)
+    pairs, skipped = o.bonds()
+    assert len(pairs) == 54 and skipped == 0
+    o2 = O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_NONE, sinter=sp), x, v)
+    assert len(o2.bonds()[0]) == 26
+
+
 def test_golden_hamaker_bit_exact():
     hp = GOLD["ham_params"]
     for seed in range(2):
