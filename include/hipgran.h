@@ -84,7 +84,12 @@ typedef struct hg_params {
     int32_t facet_advance;       /* 0: facets advance by v_facet*dt after every step;
                                     1: only through hg_advance_facet (the driver moves them,
                                     as triangular_facet::update_positions does) */
-    int32_t reserved;
+    int32_t sinter_breaking;     /* with HG_BINARY_SINTER: alt_breaking_sinter_functor instead of
+                                    alt_sinter_functor (alt_breaking_sinter_bridge.h:15-166): every
+                                    pair within d_crit is bonded (no cycle check), and an ordered
+                                    bonded pair (i, j) becomes a contact pair once |f_n n + f_t|
+                                    exceeds its bond's neck strength (hg_set_neck_strengths;
+                                    infinite until set) */
     double r_verlet;             /* neighbour-list radius (granular_system_neighbor_list.h:54) */
     hg_contact_params contact;           /* contact functor (also the one alt_sinter embeds) */
     hg_sinter_params sinter;             /* alt_sinter bonded model + d_crit */
@@ -184,6 +189,20 @@ hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *ou
 hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs, size_t cap,
                              size_t *nbonds, size_t *nskipped);
 
+/* alt_breaking_sinter neck strengths, one per bond in hg_get_bonds order
+ * (pairs i < j in lexicographic order, the order in which the reference's
+ * constructor draws them, alt_breaking_sinter_bridge.h:63-75). The reference
+ * stores each draw at neck_strengths[i] and [j] but reads [i*n + j]
+ * (:71-72, :118); each bond keeps its own draw here, which that read intends.
+ * Needs params.sinter_breaking. Replaces: the neck_strength_distribution draws
+ * of the alt_breaking_sinter_functor ctor. */
+hg_status hg_set_neck_strengths(hg_system *sys, const double *strengths);
+/* The bond search of alt_breaking_sinter_functor on its own (host only): every
+ * pair i < j with |norm(x_i - x_j) - 2 r| < d_crit, lexicographic order.
+ * Replaces: alt_breaking_sinter_bridge.h:63-75. pairs = 2*cap uint32. */
+hg_status hg_breaking_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs,
+                                      size_t cap, size_t *nbonds);
+
 /* ------------------------------------------------------------------------
  * Pools: many independent systems stepped together.
  *
@@ -218,6 +237,8 @@ hg_status hg_pool_set_facets(hg_pool *pool, int32_t slot, size_t nf, const doubl
  * system's list is rebuilt before every step whose index (the system's own
  * step count) is a multiple of rebuild_every. */
 hg_status hg_pool_step(hg_pool *pool, double dt, uint64_t nsteps, uint32_t rebuild_every);
+/* Neck strengths of one system, as hg_set_neck_strengths. */
+hg_status hg_pool_set_neck_strengths(hg_pool *pool, int32_t slot, const double *strengths);
 /* Per-system copies, as the hg_system getters. */
 hg_status hg_pool_get_state(hg_pool *pool, int32_t slot, double *x, double *v, double *theta, double *omega);
 hg_status hg_pool_get_accelerations(hg_pool *pool, int32_t slot, double *a, double *alpha);

@@ -25,11 +25,13 @@ __all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sin
 
 def make_params(binary_model, hamaker_model, pair_mode=PAIRS_ALL, r_verlet=0.0, contact=None, sinter=None,
                 hamaker=None, facet_contact=0, facet_hamaker=0, surface_contact=None, surface_hamaker=None,
-                precision=PRECISION_FP64, neighbor_capacity=0, pair_capacity=0):
+                precision=PRECISION_FP64, neighbor_capacity=0, pair_capacity=0, sinter_breaking=0):
     """Builds hg_params; functor parameter sequences are in the reference ctor
     order (contact_force.h:10-30, alt_sinter_bridge.h:19-36 + d_crit,
-    limited_range_hamaker_force.h:20-27 as A, h0, r_cutoff, r_part, mass)."""
+    limited_range_hamaker_force.h:20-27 as A, h0, r_cutoff, r_part, mass).
+    sinter_breaking=1 with BINARY_SINTER selects alt_breaking_sinter."""
     p = Params()
+    p.sinter_breaking = sinter_breaking
     p.binary_model, p.hamaker_model, p.pair_mode, p.precision = binary_model, hamaker_model, pair_mode, precision
     p.facet_contact, p.facet_hamaker = facet_contact, facet_hamaker
     p.neighbor_capacity, p.pair_capacity, p.r_verlet = neighbor_capacity, pair_capacity, r_verlet
@@ -71,6 +73,18 @@ def sinter_topology(x0, r_part, d_crit):
     return pairs[:nb.value], sk.value
 
 
+def breaking_sinter_topology(x0, r_part, d_crit):
+    """alt_breaking_sinter bonds: every pair i<j within d_crit (host only)."""
+    x0 = _abi.c64(x0)
+    n = x0.shape[0]
+    nb = C.c_size_t(0)
+    _abi.check(_abi.lib().hg_breaking_sinter_topology(n, _abi.dptr(x0), r_part, d_crit, None, 0, C.byref(nb)))
+    pairs = np.zeros((max(nb.value, 1), 2), dtype=np.uint32)
+    _abi.check(_abi.lib().hg_breaking_sinter_topology(n, _abi.dptr(x0), r_part, d_crit, _abi.u32ptr(pairs),
+                                                      nb.value, C.byref(nb)))
+    return pairs[:nb.value]
+
+
 class System:
     """One DEM system on the current GPU (hg_system)."""
 
@@ -94,6 +108,10 @@ class System:
     def set_facets(self, verts, v_facet):
         verts, v_facet = _abi.c64(verts), _abi.c64(v_facet)
         _abi.check(_abi.lib().hg_set_facets(self._h, verts.shape[0], _abi.dptr(verts), _abi.dptr(v_facet)))
+
+    def set_neck_strengths(self, strengths):
+        """alt_breaking_sinter neck strengths, one per bond in bonds() order."""
+        _abi.check(_abi.lib().hg_set_neck_strengths(self._h, _abi.dptr(_abi.c64(strengths))))
 
     def facets(self, nf):
         verts, vel = np.zeros((nf, 3, 3)), np.zeros((nf, 3))
@@ -205,6 +223,10 @@ class Pool:
 
     def remove(self, slot):
         _abi.check(_abi.lib().hg_pool_remove(self._h, slot))
+
+    def set_neck_strengths(self, slot, strengths):
+        """alt_breaking_sinter neck strengths of one system, in bonds(slot) order."""
+        _abi.check(_abi.lib().hg_pool_set_neck_strengths(self._h, slot, _abi.dptr(_abi.c64(strengths))))
 
     def set_facets(self, slot, verts, v_facet):
         verts, v_facet = _abi.c64(verts), _abi.c64(v_facet)

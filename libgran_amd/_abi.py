@@ -44,7 +44,7 @@ class Params(C.Structure):
     _fields_ = [("binary_model", C.c_int32), ("hamaker_model", C.c_int32), ("pair_mode", C.c_int32),
                 ("precision", C.c_int32), ("facet_contact", C.c_int32), ("facet_hamaker", C.c_int32),
                 ("neighbor_capacity", C.c_uint32), ("pair_capacity", C.c_uint32), ("facet_advance", C.c_int32),
-                ("reserved", C.c_int32), ("r_verlet", C.c_double),
+                ("sinter_breaking", C.c_int32), ("r_verlet", C.c_double),
                 ("contact", ContactParams), ("sinter", SinterParams), ("hamaker", HamakerParams),
                 ("surface_contact", ContactParams), ("surface_hamaker", HamakerParams)]
 
@@ -96,6 +96,9 @@ SIGNATURES = {
     "hg_get_springs": (C.c_int, [_sysp, C.c_int, _u32p, _dp, C.c_size_t, _szp]),
     "hg_diagnostics": (C.c_int, [_sysp, C.c_double, C.c_double, _dp]),
     "hg_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp, _szp]),
+    "hg_set_neck_strengths": (C.c_int, [_sysp, _dp]),
+    "hg_breaking_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp]),
+    "hg_pool_set_neck_strengths": (C.c_int, [_sysp, C.c_int32, _dp]),
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),

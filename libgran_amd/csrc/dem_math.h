@@ -126,7 +126,9 @@ HGD d3 shear_sinter(d3 &xi, d3 n, double stiffness, double damping, d3 rel_v, do
 }
 
 // ---- alt_sinter_bridge.h:92-129 (bonded branch; no overlap gate) ----------
-HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp) {
+// ftot receives f_n n + f_t, the force alt_breaking_sinter_bridge.h:115-118
+// compares with the neck strength.
+HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp, d3 &ftot) {
     d3 d = xi - xj;
     d3 n = normalized(d);
     double overlap = 2.0 * p.r_part - dot(d, n);
@@ -144,7 +146,8 @@ HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 w
     d3 tau_t = rpp * cross(n, f_t);
     d3 tau_r = p.r_part * cross(n, f_r);
     d3 tau_o = p.r_part * f_o;
-    return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+    ftot = f_n * n + f_t;
+    return {ftot / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
 }
 
 // ---- hamaker_force.h:36-39: the scalar factor of -dU/dh --------------------

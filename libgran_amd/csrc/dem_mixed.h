@@ -56,7 +56,8 @@ HGD accf operator+(accf a, accf b) { return {a.lin + b.lin, a.ang + b.ang}; }
 // Inverse masses/inertias and FP32 model constants, prepared once per launch.
 struct mixed_consts {
     float c_k, c_gn, c_kt, c_gt, c_mus, c_mud, c_kr, c_gr, c_mur, c_mudr, c_ko, c_go, c_muo, c_mudo, c_r, c_im, c_ii;
-    float s_k, s_gn, s_kt, s_gt, s_kr, s_gr, s_ko, s_go, s_r, s_im, s_ii;
+    float s_k, s_gn, s_kt, s_gt, s_kr, s_gr, s_ko, s_go, s_r, s_im, s_ii, s_m;
+    int s_breaking;
     float h_a6, h_r, h_im;   // -A/6, r, 1/m
     double c_r2, s_r2, h_r2, h_h0, h_cut;
     double c_dt, s_dt;
@@ -81,6 +82,7 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
     m.s_kr = float(s.k_r); m.s_gr = float(s.gamma_r); m.s_ko = float(s.k_o); m.s_go = float(s.gamma_o);
     m.s_r = float(s.r_part); m.s_im = float(1.0 / s.mass); m.s_ii = float(1.0 / s.inertia);
     m.s_r2 = 2.0 * s.r_part; m.s_dt = s.dt;
+    m.s_m = float(s.mass); m.s_breaking = p.sinter_breaking;
     const hg_hamaker_params &h = p.hamaker;
     m.h_a6 = float(-h.A / 6.0); m.h_r = float(h.r_part); m.h_im = float(1.0 / h.mass);
     m.h_r2 = 2.0 * h.r_part; m.h_h0 = h.h0; m.h_cut = h.r_cutoff;

@@ -44,6 +44,9 @@ struct walker {
     const double *ps_old;
     uint32_t *pj_new;
     double *ps_new;
+    const uint32_t *bj;
+    const double *bstr;
+    const uint32_t *bcnt;
     size_t stride;
     uint32_t i, cnt_old, p, q, cap, ej;
     uint32_t pre[kTagPrefetch];
@@ -53,6 +56,9 @@ struct walker {
         ps_old = T.ps_old;
         pj_new = T.pj_new;
         ps_new = T.ps_new;
+        bj = T.bj;
+        bstr = T.bstr;
+        bcnt = T.bcnt;
         stride = T.stride;
         i = i_;
         cap = T.cap;
@@ -95,6 +101,14 @@ struct walker {
             return true;
         }
         return false;
+    }
+    // Neck strength of the bond between i and j (infinite when none is set).
+    __device__ double neck(uint32_t j) const {
+        if (!bcnt) return __builtin_huge_val();
+        const uint32_t c = bcnt[i];
+        for (uint32_t s = 0; s < c; s++)
+            if (bj[size_t(s) * stride + i] == j) return bstr[size_t(s) * stride + i];
+        return __builtin_huge_val();
     }
     __device__ void finish() {
         while (p < cnt_old) {
@@ -141,8 +155,13 @@ __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi,
         W.take(j, sp, bonded);
         bool keep;
         if (BIN == HG_BINARY_SINTER && bonded) {
-            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp);
-            keep = true;
+            d3 ftot;
+            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot);
+            // alt_breaking_sinter_bridge.h:117-119: (i, j) stops being bonded
+            // once its force exceeds the neck strength; dropping the entry
+            // leaves the contact functor a pair with zero springs, as the
+            // reference's untouched contact table holds.
+            keep = !(P.sinter_breaking && norm(ftot) > W.neck(j));
         } else {
             b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
         }
@@ -198,9 +217,10 @@ __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, 
         bool bonded = false;
         W.take(j, sp, bonded);
         bool keep;
-        if (BIN == HG_BINARY_SINTER && bonded)
+        if (BIN == HG_BINARY_SINTER && bonded) {
             b = bonded_or_contact_mx<true>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
-        else
+            if (m.s_breaking && sqrtf(dotf(b.lin, b.lin)) * m.s_m > float(W.neck(j))) keep = false;
+        } else
             b = bonded_or_contact_mx<false>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
         if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
     }

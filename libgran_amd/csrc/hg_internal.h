@@ -23,6 +23,9 @@ constexpr uint32_t kNoEntry = 0xffffffffu;  // (kNoEntry & kIdxMask) exceeds eve
 enum : int { FLAG_NL_OVERFLOW = 0, FLAG_PAIR_OVERFLOW = 1, FLAG_NL_MAXCOUNT = 2, FLAG_PAIR_MAXCOUNT = 3, FLAG_COUNT = 4 };
 
 constexpr int kMaxFacets = 16;
+// Bonded partners per particle in the neck-strength lists (touching equal
+// spheres: at most 12).
+constexpr uint32_t kMaxBondsPerParticle = 16;
 // Pair-state tags loaded up front per particle (the table capacity is never
 // below this) and neighbour ids loaded up front per particle. Deeper
 // prefetch arrays are indexed at run time and land in scratch memory, which
@@ -41,6 +44,12 @@ struct pair_table {
     uint32_t *pcnt_new;
     uint32_t cap;
     size_t stride;
+    // alt_breaking_sinter neck strengths: particle i's bonded partners at
+    // bj[s*stride + i], their strengths at bstr[s*stride + i], s < bcnt[i]
+    // (null when bonds do not break)
+    const uint32_t *bj;
+    const double *bstr;
+    const uint32_t *bcnt;
 };
 
 // Everything a force/integrate kernel needs, passed by value.
@@ -76,6 +85,10 @@ struct hg_system_impl {
     double *ps[2] = {nullptr, nullptr};
     uint32_t *pcnt[2] = {nullptr, nullptr};
     int cur = 0;
+    // neck strengths (sinter_breaking): bonded partners and strengths per particle
+    uint32_t bcap = 0;
+    uint32_t *bj = nullptr, *bcnt = nullptr;
+    double *bstr = nullptr;
     // neighbour list
     uint32_t cap_n = 0;
     uint32_t *nb = nullptr, *ncnt = nullptr;
@@ -140,8 +153,11 @@ hipError_t launch_diagnostics(const double *x, const double *v, const double *w,
                               double inertia, double *out, hipStream_t s);
 
 // host-side set-up (topology.cpp)
+// prevent_cycles: alt_sinter's spanning forest; false: alt_breaking_sinter's
+// every-touching-pair rule.
 void sinter_topology_host(size_t n, const double *x0, double r_part, double d_crit,
-                          std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped);
+                          std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped,
+                          bool prevent_cycles = true);
 
 }  // namespace hg
 

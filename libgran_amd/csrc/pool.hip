@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -41,6 +42,9 @@ struct pool_view {
     uint32_t *pj0, *pj1, *pcnt0, *pcnt1;
     double *ps0, *ps1;
     uint32_t cap_s, cap_n;
+    const uint32_t *bj;  // neck-strength lists (see pair_table), null unless bonds break
+    const double *bstr;
+    const uint32_t *bcnt;
     facet_dev *facets;
     uint32_t max_facets;
     double *ss;
@@ -133,6 +137,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
     T.pcnt_new = cur ? V.pcnt0 : V.pcnt1;
     T.cap = V.cap_s;
     T.stride = V.stride;
+    T.bj = V.bj;
+    T.bstr = V.bstr;
+    T.bcnt = V.bcnt;
     const uint32_t end = M.offset + M.n;
     for (uint32_t g = g_first; g < g_end; g += g_step) {
         d3 xi, vi, wi;
@@ -552,6 +559,8 @@ struct hg_pool {
     double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
     uint32_t cap_n = 0, *nb = nullptr, *ncnt = nullptr;
     uint32_t cap_s = 0, *pj[2] = {nullptr, nullptr}, *pcnt[2] = {nullptr, nullptr};
+    uint32_t *bj = nullptr, *bcnt = nullptr;  // neck strengths (sinter_breaking)
+    double *bstr = nullptr;
     double *ps[2] = {nullptr, nullptr};
     uint32_t *owner = nullptr;
     facet_dev *facets = nullptr;
@@ -661,6 +670,9 @@ pool_view make_pool_view(hg_pool *Q) {
     V.pcnt0 = Q->pcnt[0]; V.pcnt1 = Q->pcnt[1];
     V.ps0 = Q->ps[0]; V.ps1 = Q->ps[1];
     V.cap_s = Q->cap_s;
+    V.bj = Q->bj;
+    V.bstr = Q->bstr;
+    V.bcnt = Q->bcnt;
     V.cap_n = Q->cap_n;
     V.facets = Q->facets;
     V.max_facets = uint32_t(Q->MF);
@@ -728,6 +740,8 @@ size_t hg_pool_bytes_per_particle(const hg_params *p, size_t max_facets) {
     b += 2 * (cap_s * (4 + 72) + 4);     // pair-state table, double buffered
     b += max_facets * 72;                // facet springs
     b += 4;                              // owner
+    if (p->binary_model == HG_BINARY_SINTER && p->sinter_breaking)
+        b += kMaxBondsPerParticle * 12 + 4;  // neck-strength lists
     if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST) b += 4 + 4 + 3 * 2 * 4 + 4;  // cell list scratch
     return b;
 }
@@ -741,6 +755,8 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     if (max_facets > size_t(kMaxFacets)) return fail(HG_ERR_INVALID, "hg_pool_create: at most 16 facets per system");
     if (params->precision != HG_PRECISION_FP64 && params->precision != HG_PRECISION_MIXED)
         return fail(HG_ERR_INVALID, "hg_pool_create: unknown precision");
+    if (params->sinter_breaking != 0 && (params->sinter_breaking != 1 || params->binary_model != HG_BINARY_SINTER))
+        return fail(HG_ERR_INVALID, "hg_pool_create: sinter_breaking is 0, or 1 with HG_BINARY_SINTER");
     if (params->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(params->r_verlet > 0.0))
         return fail(HG_ERR_INVALID, "hg_pool_create: neighbour-list mode needs r_verlet > 0");
     if (params->facet_advance != 0)
@@ -816,6 +832,12 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
             PG_TRYC(palloc(Q, &Q->pcnt[b], P));
         }
     }
+    if (Q->p.binary_model == HG_BINARY_SINTER && Q->p.sinter_breaking) {
+        PG_TRYC(palloc(Q, &Q->bj, size_t(kMaxBondsPerParticle) * P));
+        PG_TRYC(palloc(Q, &Q->bstr, size_t(kMaxBondsPerParticle) * P));
+        PG_TRYC(palloc(Q, &Q->bcnt, P));
+        PG_HIPC(launch_zero_u32(Q->bcnt, P, Q->stream));
+    }
     PG_TRYC(palloc(Q, &Q->owner, P));
     {
         std::vector<uint32_t> none(P, kNoEntry);
@@ -883,14 +905,14 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
     size_t skipped = 0;
     std::vector<std::vector<uint32_t>> rows(n);
     if (Q->p.binary_model == HG_BINARY_SINTER) {
-        sinter_topology_host(n, x0, Q->p.sinter.r_part, Q->p.sinter.d_crit, bonds, skipped);
+        sinter_topology_host(n, x0, Q->p.sinter.r_part, Q->p.sinter.d_crit, bonds, skipped, !Q->p.sinter_breaking);
         for (auto &b : bonds) {
             rows[b.first].push_back((offset + b.second) | kBondBit);
             rows[b.second].push_back((offset + b.first) | kBondBit);
         }
         for (auto &r : rows) std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return (a & kIdxMask) < (b & kIdxMask); });
         for (auto &r : rows)
-            if (r.size() > Q->cap_s)
+            if (r.size() > Q->cap_s || (Q->bcnt && r.size() > kMaxBondsPerParticle))
                 return fail(HG_ERR_CAPACITY, "hg_pool_add: a particle has " + std::to_string(r.size()) +
                                                  " bonds, pair capacity is " + std::to_string(Q->cap_s));
     }
@@ -940,6 +962,36 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
     Q->order_dirty = true;
     Q->meta_dirty = true;
     *slot_out = slot;
+    if (Q->bcnt) {  // neck strengths start infinite
+        std::vector<double> inf(Q->bonds[slot].size(), std::numeric_limits<double>::infinity());
+        PG_TRY(hg_pool_set_neck_strengths(Q, slot, inf.data()));
+    }
+    return HG_OK;
+}
+
+hg_status hg_pool_set_neck_strengths(hg_pool *Q, int32_t slot, const double *strengths) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!strengths) return fail(HG_ERR_INVALID, "strengths is null");
+    if (!Q->bcnt) return fail(HG_ERR_INVALID, "hg_pool_set_neck_strengths: the pool's bonds do not break "
+                                              "(params.sinter_breaking)");
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    std::vector<uint32_t> bj(size_t(kMaxBondsPerParticle) * n, kNoEntry), cnt(n, 0);
+    std::vector<double> bs(size_t(kMaxBondsPerParticle) * n, 0.0);
+    for (size_t b = 0; b < Q->bonds[slot].size(); b++) {
+        for (int end = 0; end < 2; end++) {
+            const uint32_t i = end ? Q->bonds[slot][b].second : Q->bonds[slot][b].first;
+            const uint32_t j = end ? Q->bonds[slot][b].first : Q->bonds[slot][b].second;
+            bj[size_t(cnt[i]) * n + i] = off + j;  // arena index, as the pair-state tags
+            bs[size_t(cnt[i]) * n + i] = strengths[b];
+            cnt[i]++;
+        }
+    }
+    HG_HIP(hipMemcpy2DAsync(Q->bj + off, Q->P * 4, bj.data(), n * 4, n * 4, kMaxBondsPerParticle,
+                            hipMemcpyHostToDevice, Q->stream));
+    HG_HIP(hipMemcpy2DAsync(Q->bstr + off, Q->P * 8, bs.data(), n * 8, n * 8, kMaxBondsPerParticle,
+                            hipMemcpyHostToDevice, Q->stream));
+    HG_HIP(hipMemcpyAsync(Q->bcnt + off, cnt.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
     return HG_OK;
 }
 

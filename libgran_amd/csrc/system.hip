@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -172,6 +173,9 @@ sys_view make_view(hg_system *S) {
     v.pt.pcnt_new = S->pcnt[S->cur ^ 1];
     v.pt.cap = S->cap_s;
     v.pt.stride = S->n;
+    v.pt.bj = S->bj;
+    v.pt.bstr = S->bstr;
+    v.pt.bcnt = S->bcnt;
     v.facets = S->facets;
     v.ss = S->ss;
     v.nf = S->nf;
@@ -203,6 +207,8 @@ hg_status validate(const hg_params *p, size_t n) {
     if (p->precision == HG_PRECISION_MIXED && p->pair_mode != HG_PAIRS_NEIGHBOR_LIST)
         return fail(HG_ERR_INVALID, "a single system runs HG_PRECISION_MIXED in neighbour-list mode only; "
                                     "all-pairs systems run it through a pool");
+    if (p->sinter_breaking != 0 && (p->sinter_breaking != 1 || p->binary_model != HG_BINARY_SINTER))
+        return fail(HG_ERR_INVALID, "sinter_breaking is 0, or 1 with HG_BINARY_SINTER");
     return HG_OK;
 }
 
@@ -263,7 +269,8 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
     // Bond topology (alt_sinter ctor) and the initial pair-state table.
     std::vector<std::vector<uint32_t>> rows(n);
     if (S->p.binary_model == HG_BINARY_SINTER) {
-        sinter_topology_host(n, x0, S->p.sinter.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped);
+        sinter_topology_host(n, x0, S->p.sinter.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped,
+                             !S->p.sinter_breaking);
         for (auto &b : S->bonds) {
             rows[b.first].push_back(b.second | kBondBit);
             rows[b.second].push_back(b.first | kBondBit);
@@ -293,6 +300,18 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
         HG_HIPC(hipMemcpyAsync(S->pcnt[0], cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, S->stream));
         HG_HIPC(launch_zero_f64(S->ps[0], size_t(S->cap_s) * 9 * n, S->stream));
         HG_HIPC(hipStreamSynchronize(S->stream));
+    }
+    if (S->p.binary_model == HG_BINARY_SINTER && S->p.sinter_breaking) {
+        if (max_bonds > kMaxBondsPerParticle)
+            return cleanup_on(fail(HG_ERR_CAPACITY, "a particle has " + std::to_string(max_bonds) +
+                                                        " bonds; neck strengths hold at most " +
+                                                        std::to_string(kMaxBondsPerParticle)));
+        S->bcap = kMaxBondsPerParticle;
+        HG_TRYC(dalloc(S, &S->bj, size_t(S->bcap) * n));
+        HG_TRYC(dalloc(S, &S->bstr, size_t(S->bcap) * n));
+        HG_TRYC(dalloc(S, &S->bcnt, n));
+        std::vector<double> inf(S->bonds.size(), std::numeric_limits<double>::infinity());
+        HG_TRYC(hg_set_neck_strengths(S, inf.data()));
     }
 
     if (S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
@@ -582,6 +601,43 @@ hg_status hg_sinter_topology(size_t n, const double *x0, double r_part, double d
     }
     if (nbonds) *nbonds = bonds.size();
     if (nskipped) *nskipped = skipped;
+    return HG_OK;
+}
+
+hg_status hg_breaking_sinter_topology(size_t n, const double *x0, double r_part, double d_crit, uint32_t *pairs,
+                                      size_t cap, size_t *nbonds) {
+    if (n > 0 && !x0) return fail(HG_ERR_INVALID, "x0 is null");
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;
+    size_t skipped = 0;
+    sinter_topology_host(n, x0, r_part, d_crit, bonds, skipped, false);
+    for (size_t b = 0; pairs && b < bonds.size() && b < cap; b++) {
+        pairs[2 * b] = bonds[b].first;
+        pairs[2 * b + 1] = bonds[b].second;
+    }
+    if (nbonds) *nbonds = bonds.size();
+    return HG_OK;
+}
+
+hg_status hg_set_neck_strengths(hg_system *S, const double *strengths) {
+    if (!S || !strengths) return fail(HG_ERR_INVALID, "system and strengths are required");
+    if (!S->bcnt) return fail(HG_ERR_INVALID, "hg_set_neck_strengths: the system's bonds do not break "
+                                              "(params.sinter_breaking)");
+    const size_t n = S->n;
+    std::vector<uint32_t> bj(size_t(S->bcap) * n, kNoEntry), cnt(n, 0);
+    std::vector<double> bs(size_t(S->bcap) * n, 0.0);
+    for (size_t b = 0; b < S->bonds.size(); b++) {
+        for (int end = 0; end < 2; end++) {
+            const uint32_t i = end ? S->bonds[b].second : S->bonds[b].first;
+            const uint32_t j = end ? S->bonds[b].first : S->bonds[b].second;
+            bj[size_t(cnt[i]) * n + i] = j;
+            bs[size_t(cnt[i]) * n + i] = strengths[b];
+            cnt[i]++;
+        }
+    }
+    HG_HIP(hipMemcpyAsync(S->bj, bj.data(), bj.size() * 4, hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipMemcpyAsync(S->bstr, bs.data(), bs.size() * 8, hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipMemcpyAsync(S->bcnt, cnt.data(), n * 4, hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
     return HG_OK;
 }
 

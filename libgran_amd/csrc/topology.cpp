@@ -35,7 +35,7 @@ struct uf {
 }  // namespace
 
 void sinter_topology_host(size_t n, const double *x0, double r_part, double d_crit,
-                          std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped) {
+                          std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped, bool prevent_cycles) {
     bonds.clear();
     skipped = 0;
     if (n < 2 || !(d_crit > 0.0)) return;
@@ -81,6 +81,10 @@ void sinter_topology_host(size_t n, const double *x0, double r_part, double d_cr
         }
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    }
+    if (!prevent_cycles) {  // alt_breaking_sinter_bridge.h:63-75: every candidate
+        bonds = std::move(cand);
+        return;
     }
     uf comps(n);
     for (auto &c : cand) {

@@ -6,7 +6,7 @@
 // with the CPU oracle.
 //
 //   dropin_scenarios <scenario> <steps> <output-file>
-//   scenarios: contact hamaker hamaker_neighbor alt_sintered surface plate
+//   scenarios: contact hamaker hamaker_neighbor alt_sintered breaking surface plate
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +20,7 @@
 #include <libgran/granular_system/granular_system_neighbor_list.h>
 #include <libgran/hamaker_force/hamaker_force.h>
 #include <libgran/hamaker_force/surface_hamaker_force.h>
+#include <libgran/sinter_bridge/alt_breaking_sinter_bridge.h>
 #include <libgran/sinter_bridge/alt_sinter_bridge.h>
 #include <libgran/surface_force/triangular_facet.h>
 #include <libgran/vec3.h>
@@ -153,6 +154,50 @@ int run_alt_sintered(size_t steps, const char *out) {
     return 0;
 }
 
+// Neck strengths 1e-8 N * (1 + b % 7) for the b-th draw.
+struct cycling_strength {
+    size_t b = 0;
+    double operator()() { return 1.0e-8 * double(1 + (b++ % 7)); }
+};
+
+int run_breaking(size_t steps, const char *out) {
+    VV x0, v0;
+    for (int i = 0; i < 3; i++) {
+        double y = -1.0 * r_part + double(i) * 2.0 * r_part;
+        for (int j = 0; j < 3; j++) {
+            double xx = -1.0 * r_part + double(j) * 2.0 * r_part;
+            x0.emplace_back(xx, y, 0.0);
+            x0.emplace_back(xx, y, -2.0 * r_part);
+            x0.emplace_back(xx, y, -4.0 * r_part);
+            V3 vel = j == 0 ? V3(0.0, 0.0, 5.0) : (j == 2 ? V3(0.0, 5.0, 0.0) : V3());
+            for (int q = 0; q < 3; q++) v0.push_back(vel);
+        }
+    }
+    VV th0(x0.size()), w0(x0.size());
+    const double dt = 5e-14;
+    const double k_bond = 1000.0, g_bond = 2.0 * std::sqrt(2.0 * mass * k_bond);
+    rotational_step_handler<VV, V3> handler;
+    contact_force_functor<V3, double> contact(x0.size(), k, gamma_n, k, gamma_t, mu, phi, k, gamma_r, mu_o, phi, k,
+                                              gamma_o, mu_o, phi, r_part, mass, inertia, dt, V3(), 0.0);
+    alt_breaking_sinter_functor<V3, double, cycling_strength> sinter(
+        x0.size(), x0, k_bond, g_bond, k_bond, 0.2 * g_bond, k_bond, 0.05 * g_bond, k_bond, 0.05 * g_bond, r_part,
+        mass, inertia, dt, V3(), 0.0, 1.0e-9, cycling_strength{}, contact);
+    binary_force_functor_container<V3, double, decltype(sinter)> binary{sinter};
+    unary_force_functor_container<V3, double> unary;
+    granular_system<V3, double, rotational_velocity_verlet_half, rotational_step_handler, decltype(binary),
+                    decltype(unary)>
+        system(x0, v0, th0, w0, 0.0, V3(), 0.0, handler, binary, unary);
+    size_t bonded_before = 0;
+    for (size_t idx = 0; idx < x0.size() * x0.size(); idx++) bonded_before += sinter.bonded_contacts[idx] ? 1 : 0;
+    for (size_t n = 0; n < steps; n++) system.do_step(dt);
+    size_t bonded_after = 0;
+    for (size_t idx = 0; idx < x0.size() * x0.size(); idx++) bonded_after += sinter.bonded_contacts[idx] ? 1 : 0;
+    std::fprintf(stderr, "bonded ordered pairs: %zu -> %zu\n", bonded_before, bonded_after);
+    if (bonded_before != 2 * 54 || !(bonded_after < bonded_before)) return 6;
+    finish(system, out);
+    return 0;
+}
+
 int run_surface(size_t steps, const char *out, bool moving) {
     using sc_t = surface_contact_force_functor<V3, double>;
     using sh_t = surface_hamaker_functor<V3, double>;
@@ -197,6 +242,7 @@ int main(int argc, char **argv) {
         if (s == "hamaker") return run_contact(steps, argv[3], true, false);
         if (s == "hamaker_neighbor") return run_contact(steps, argv[3], true, true);
         if (s == "alt_sintered") return run_alt_sintered(steps, argv[3]);
+        if (s == "breaking") return run_breaking(steps, argv[3]);
         if (s == "surface") return run_surface(steps, argv[3], false);
         if (s == "plate") return run_surface(steps, argv[3], true);
     } catch (const std::exception &e) {

@@ -93,6 +93,21 @@ def test_dropin_alt_sintered(tmp_path):
     compare(o, got, "alt_sintered")
 
 
+def test_dropin_alt_breaking_sintered(tmp_path):
+    """The same lattice through alt_breaking_sinter_functor (every touching
+    pair bonded, neck strengths 1e-8 N * (1 + b % 7) in draw order); the driver
+    also checks that bonded_contacts follows the breaks on the device."""
+    steps = 6000
+    got = run_driver(tmp_path, "breaking", steps)
+    x, v = S.sinter_lattice()
+    o = oracle(O.BIN_SINTER, O.HAM_NONE, x, v, contact=O.contact_params(dt=5e-14),
+               sinter=O.sinter_params(k_bond=1000.0, d_crit=1.0e-9, dt=5e-14), sinter_breaking=1)
+    nb = len(o.bonds()[0])
+    o.set_neck_strengths(1.0e-8 * (1.0 + np.arange(nb) % 7))
+    o.step(5e-14, steps)
+    compare(o, got, "breaking")
+
+
 @pytest.mark.parametrize("scenario", ["surface", "plate"])
 def test_dropin_triangular_facet(tmp_path, scenario):
     """test/surface.cpp set-up: a triangular_facet in the unary container; the
