@@ -58,6 +58,7 @@ struct hg_server {
     bool stop = false;
     std::string fatal;  // set when the worker stopped on an error
     hg_server_stats stats{};
+    size_t tuned_particles = 0;  // resident particles when the pool was last tuned
     std::thread worker;
 
     double now() const { return std::chrono::duration<double>(clock_type::now() - t0).count(); }
@@ -102,6 +103,15 @@ void hg_server::run() {
             stats.queued = queue.size();
             stats.running = running.size();
             if (running.empty()) continue;
+            // re-tune the pool's launch split when the resident particle count
+            // has moved by half since the last tuning (the split never changes
+            // results, only speed)
+            size_t resident = 0;
+            for (uint64_t id : running) resident += reqs[id].n;
+            if (resident > tuned_particles + tuned_particles / 2 || 2 * resident < tuned_particles) {
+                (void)hg_pool_autotune(pool, 1);
+                tuned_particles = resident;
+            }
         }
         // one quantum for every resident request (the lock is not held, so
         // submissions proceed meanwhile)
