@@ -292,6 +292,9 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
 // once the shared counter passes `count`. lds_bytes of dynamic LDS per
 // workgroup decide each system's state placement.
 template <int BIN, int HAM, int PAIRS, int PREC>
+// The model constants come by value: the scalar registers they spill to are
+// re-read with v_readlane, which measured faster than reloading them through
+// a pointer inside the pair loop (scalar loads add memory latency there).
 __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
                                                     uint32_t count, uint32_t *counter, uint32_t nsteps, double dt,
                                                     uint32_t lds_bytes) {
