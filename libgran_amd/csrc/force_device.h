@@ -32,13 +32,32 @@ __device__ __forceinline__ void st_spring(double *b, size_t stride, const spring
     b[6 * stride] = s.s[2].x; b[7 * stride] = s.s[2].y; b[8 * stride] = s.s[2].z;
 }
 
+// A pair-state record (kSpringRec doubles, 16-byte aligned) in five 16-byte
+// accesses; the pad word is written as zero.
+__device__ __forceinline__ spring9 ld_rec(const double *r) {
+    const double2 *q = reinterpret_cast<const double2 *>(r);
+    const double2 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    return spring9{{d3{a.x, a.y, b.x}, d3{b.y, c.x, c.y}, d3{d.x, d.y, e.x}}};
+}
+__device__ __forceinline__ void st_rec(double *r, const spring9 &s) {
+    double2 *q = reinterpret_cast<double2 *>(r);
+    q[0] = double2{s.s[0].x, s.s[0].y};
+    q[1] = double2{s.s[0].z, s.s[1].x};
+    q[2] = double2{s.s[1].y, s.s[1].z};
+    q[3] = double2{s.s[2].x, s.s[2].y};
+    q[4] = double2{s.s[2].z, 0.0};
+}
+static_assert(kSpringRec == 10, "ld_rec/st_rec move 10 doubles");
+
 // Walks one particle's old pair-state table in ascending j alongside the
 // partner loop, writing the new table. Entries not visited this pass (pairs
 // that are not listed) are copied unchanged: the reference leaves the dense
 // spring table untouched for pairs it does not evaluate.
 // The first kTagPrefetch tags of the old row are loaded together with the
 // count (the table capacity is at least that many slots), so walking the
-// usual one to four entries needs no dependent load.
+// usual one to four entries needs no dependent load. The first kNewTagBuf
+// tags of the new row are held in registers and written by finish(), when
+// every lane writes its tag row s in the same instruction.
 struct walker {
     const uint32_t *pj_old;
     const double *ps_old;
@@ -50,6 +69,7 @@ struct walker {
     size_t stride;
     uint32_t i, cnt_old, p, q, cap, ej;
     uint32_t pre[kTagPrefetch];
+    uint32_t nt[kNewTagBuf];
 
     __device__ void init(const pair_table &T, uint32_t i_) {
         pj_old = T.pj_old;
@@ -68,6 +88,8 @@ struct walker {
         p = 0;
         q = 0;
         ej = cnt_old > 0 ? pre[0] : kNoEntry;
+#pragma unroll
+        for (uint32_t s = 0; s < kNewTagBuf; s++) nt[s] = kNoEntry;
     }
     __device__ uint32_t tag_at(uint32_t s) const {
         uint32_t t = pre[0];
@@ -79,11 +101,13 @@ struct walker {
         p++;
         ej = p < cnt_old ? tag_at(p) : kNoEntry;
     }
-    __device__ spring9 load() const { return ld_spring(ps_old + size_t(p) * 9 * stride + i, stride); }
+    __device__ spring9 load() const { return ld_rec(ps_old + (size_t(p) * stride + i) * kSpringRec); }
     __device__ void store(uint32_t tag, const spring9 &s) {
         if (q < cap) {
-            pj_new[size_t(q) * stride + i] = tag;
-            st_spring(ps_new + size_t(q) * 9 * stride + i, stride, s);
+            if (q >= kNewTagBuf) pj_new[size_t(q) * stride + i] = tag;
+#pragma unroll
+            for (uint32_t k = 0; k < kNewTagBuf; k++) nt[k] = q == k ? tag : nt[k];
+            st_rec(ps_new + (size_t(q) * stride + i) * kSpringRec, s);
         }
         q++;
     }
@@ -115,6 +139,10 @@ struct walker {
             store(ej, load());
             next();
         }
+        const uint32_t m = q < cap ? q : cap;
+#pragma unroll
+        for (uint32_t s = 0; s < kNewTagBuf; s++)
+            if (s < m) pj_new[size_t(s) * stride + i] = nt[s];
     }
 };
 

@@ -32,9 +32,19 @@ constexpr uint32_t kMaxBondsPerParticle = 16;
 // costs more than the dependent loads they save.
 constexpr uint32_t kTagPrefetch = 1;
 constexpr uint32_t kIdPrefetch = 1;
+// Doubles per pair-state record: the 9 spring components and one pad word,
+// so that records stay 16-byte aligned.
+constexpr uint32_t kSpringRec = 10;
+// New-table tags a particle keeps in registers while it walks its partners
+// and writes at the end of the walk, all lanes together.
+constexpr uint32_t kNewTagBuf = 2;
 
-// Pair-state table, old (read) and new (written) halves; slot s of particle i
-// at s*stride + i (spring component c at (s*9 + c)*stride + i).
+// Pair-state table, old (read) and new (written) halves; the tag of slot s of
+// particle i at s*stride + i, its springs in the kSpringRec-double record
+// starting at (s*stride + i)*kSpringRec. A particle's record is contiguous:
+// the walker stores a record when it reaches that pair, at a different
+// iteration for each lane, and a record shared with no other lane fills whole
+// 32-byte sectors on its own instead of leaving partly written lines in L2.
 struct pair_table {
     const uint32_t *pj_old;
     const double *ps_old;

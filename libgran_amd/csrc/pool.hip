@@ -486,8 +486,8 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
 }
 
 // Zeroes `rows` strided rows [offset, offset+n) of an array with row stride.
-__global__ void k_zero_rows_f64(double *p, size_t stride, uint32_t offset, uint32_t n, uint32_t rows) {
-    const size_t total = size_t(rows) * n;
+__global__ void k_zero_rows_f64(double *p, size_t stride, size_t offset, size_t n, uint32_t rows) {
+    const size_t total = rows * n;
     for (size_t k = blockIdx.x * size_t(blockDim.x) + threadIdx.x; k < total; k += size_t(gridDim.x) * blockDim.x)
         p[(k / n) * stride + offset + (k % n)] = 0.0;
 }
@@ -720,11 +720,14 @@ hg_status upload_meta(hg_pool *Q) {
 }
 
 // Copies strided rows [offset, offset+n) of a device array into a host array
-// with row stride n.
+// with row stride n; `width` elements per particle (a row of the device array
+// holds P*width elements).
 template <typename T>
-hg_status download_rows(hg_pool *Q, const T *dev, uint32_t offset, uint32_t n, uint32_t rows, T *host) {
+hg_status download_rows(hg_pool *Q, const T *dev, uint32_t offset, uint32_t n, uint32_t rows, T *host,
+                        size_t width = 1) {
     if (rows == 0 || n == 0) return HG_OK;
-    HG_HIP(hipMemcpy2DAsync(host, n * sizeof(T), dev + offset, Q->P * sizeof(T), n * sizeof(T), rows,
+    const size_t row = n * width * sizeof(T);
+    HG_HIP(hipMemcpy2DAsync(host, row, dev + offset * width, Q->P * width * sizeof(T), row, rows,
                             hipMemcpyDeviceToHost, Q->stream));
     HG_HIP(hipStreamSynchronize(Q->stream));
     return HG_OK;
@@ -760,7 +763,7 @@ size_t hg_pool_bytes_per_particle(const hg_params *p, size_t max_facets) {
     const size_t cap_n = pool_cap_n(*p), cap_s = pool_cap_s(*p);
     size_t b = 6 * 32;                   // x, v, theta, omega, a, alpha
     b += cap_n * 4 + 4;                  // neighbour slots + count
-    b += 2 * (cap_s * (4 + 72) + 4);     // pair-state table, double buffered
+    b += 2 * (cap_s * (4 + 8 * kSpringRec) + 4);  // pair-state table, double buffered
     b += max_facets * 72;                // facet springs
     b += 4;                              // owner
     if (p->binary_model == HG_BINARY_SINTER && p->sinter_breaking)
@@ -851,7 +854,7 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     if (Q->cap_s) {
         for (int b = 0; b < 2; b++) {
             PG_TRYC(palloc(Q, &Q->pj[b], size_t(Q->cap_s) * P));
-            PG_TRYC(palloc(Q, &Q->ps[b], size_t(Q->cap_s) * 9 * P));
+            PG_TRYC(palloc(Q, &Q->ps[b], size_t(Q->cap_s) * kSpringRec * P));
             PG_TRYC(palloc(Q, &Q->pcnt[b], P));
         }
     }
@@ -971,7 +974,8 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
         HG_HIP(hipMemcpy2DAsync(Q->pj[0] + offset, Q->P * 4, pj.data(), n * 4, n * 4, Q->cap_s,
                                 hipMemcpyHostToDevice, Q->stream));
         HG_HIP(hipMemcpyAsync(Q->pcnt[0] + offset, cnt.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
-        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ps[0], Q->P, offset, uint32_t(n), Q->cap_s * 9);
+        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ps[0], size_t(Q->P) * kSpringRec, size_t(offset) * kSpringRec,
+                                                     n * kSpringRec, Q->cap_s);
         HG_HIP(hipGetLastError());
         HG_HIP(hipStreamSynchronize(Q->stream));
     }
@@ -1398,9 +1402,9 @@ hg_status hg_pool_get_springs(hg_pool *Q, int32_t slot, int which, uint32_t *ij,
     if (!Q->cap_s) return HG_OK;
     const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n, cur = Q->meta[slot].cur;
     std::vector<uint32_t> pj(size_t(Q->cap_s) * n), cnt(n);
-    std::vector<double> ps(size_t(Q->cap_s) * 9 * n);
+    std::vector<double> ps(size_t(Q->cap_s) * kSpringRec * n);
     PG_TRY(download_rows(Q, Q->pj[cur], off, n, Q->cap_s, pj.data()));
-    PG_TRY(download_rows(Q, Q->ps[cur], off, n, Q->cap_s * 9, ps.data()));
+    PG_TRY(download_rows(Q, Q->ps[cur], off, n, Q->cap_s, ps.data(), kSpringRec));
     PG_TRY(download_rows(Q, Q->pcnt[cur], off, n, 1, cnt.data()));
     size_t tot = 0;
     for (uint32_t i = 0; i < n; i++)
@@ -1413,7 +1417,7 @@ hg_status hg_pool_get_springs(hg_pool *Q, int32_t slot, int which, uint32_t *ij,
                     ij[2 * tot + 1] = (tag & kIdxMask) - off;
                 }
                 if (springs)
-                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * 9 + c) * n + i];
+                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * n + i) * kSpringRec + c];
             }
             tot++;
         }

@@ -288,7 +288,7 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
                                                         std::to_string(max_bonds)));
         for (int b = 0; b < 2; b++) {
             HG_TRYC(dalloc(S, &S->pj[b], size_t(S->cap_s) * n));
-            HG_TRYC(dalloc(S, &S->ps[b], size_t(S->cap_s) * 9 * n));
+            HG_TRYC(dalloc(S, &S->ps[b], size_t(S->cap_s) * kSpringRec * n));
             HG_TRYC(dalloc(S, &S->pcnt[b], n));
         }
         std::vector<uint32_t> pj(size_t(S->cap_s) * n, kNoEntry), cnt(n);
@@ -298,7 +298,7 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
         }
         HG_HIPC(hipMemcpyAsync(S->pj[0], pj.data(), pj.size() * 4, hipMemcpyHostToDevice, S->stream));
         HG_HIPC(hipMemcpyAsync(S->pcnt[0], cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, S->stream));
-        HG_HIPC(launch_zero_f64(S->ps[0], size_t(S->cap_s) * 9 * n, S->stream));
+        HG_HIPC(launch_zero_f64(S->ps[0], size_t(S->cap_s) * kSpringRec * n, S->stream));
         HG_HIPC(hipStreamSynchronize(S->stream));
     }
     if (S->p.binary_model == HG_BINARY_SINTER && S->p.sinter_breaking) {
@@ -541,7 +541,7 @@ hg_status hg_get_springs(hg_system *S, int which, uint32_t *ij, double *springs,
     if (S->p.binary_model == HG_BINARY_NONE) return HG_OK;
     const size_t n = S->n, cs = S->cap_s;
     std::vector<uint32_t> pj(cs * n), cnt(n);
-    std::vector<double> ps(cs * 9 * n);
+    std::vector<double> ps(cs * kSpringRec * n);
     HG_HIP(hipMemcpyAsync(pj.data(), S->pj[S->cur], pj.size() * 4, hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipMemcpyAsync(ps.data(), S->ps[S->cur], ps.size() * 8, hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipMemcpyAsync(cnt.data(), S->pcnt[S->cur], cnt.size() * 4, hipMemcpyDeviceToHost, S->stream));
@@ -558,7 +558,7 @@ hg_status hg_get_springs(hg_system *S, int which, uint32_t *ij, double *springs,
                     ij[2 * tot + 1] = tag & kIdxMask;
                 }
                 if (springs)
-                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * 9 + c) * n + i];
+                    for (int c = 0; c < 9; c++) springs[9 * tot + c] = ps[(size_t(s) * n + i) * kSpringRec + c];
             }
             tot++;
         }
