@@ -58,7 +58,13 @@ static_assert(kSpringRec == 10, "ld_rec/st_rec move 10 doubles");
 // usual one to four entries needs no dependent load. The first kNewTagBuf
 // tags of the new row are held in registers and written by finish(), when
 // every lane writes its tag row s in the same instruction.
-struct walker {
+// EAGER loads the springs of old entry p together with its tag when the walk
+// reaches p (every old entry is either taken or copied, so that load is never
+// wasted) and keeps them in 18 registers until used, a whole pair evaluation
+// later; otherwise they are loaded when used. The mixed-precision kernels
+// walk eagerly; the FP64 ones have no registers to spare for it.
+template <bool EAGER>
+struct walker_t {
     const uint32_t *pj_old;
     const double *ps_old;
     uint32_t *pj_new;
@@ -70,6 +76,7 @@ struct walker {
     uint32_t i, cnt_old, p, q, cap, ej;
     uint32_t pre[kTagPrefetch];
     uint32_t nt[kNewTagBuf];
+    spring9 rec;  // springs of old entry p (EAGER)
 
     __device__ void init(const pair_table &T, uint32_t i_) {
         pj_old = T.pj_old;
@@ -87,7 +94,11 @@ struct walker {
         for (uint32_t s = 0; s < kTagPrefetch; s++) pre[s] = pj_old[size_t(s) * stride + i];
         p = 0;
         q = 0;
-        ej = cnt_old > 0 ? pre[0] : kNoEntry;
+        ej = kNoEntry;
+        if (cnt_old > 0) {
+            ej = pre[0];
+            if constexpr (EAGER) rec = ld_rec(ps_old + size_t(i) * kSpringRec);
+        }
 #pragma unroll
         for (uint32_t s = 0; s < kNewTagBuf; s++) nt[s] = kNoEntry;
     }
@@ -99,9 +110,16 @@ struct walker {
     }
     __device__ void next() {
         p++;
-        ej = p < cnt_old ? tag_at(p) : kNoEntry;
+        ej = kNoEntry;
+        if (p < cnt_old) {
+            ej = tag_at(p);
+            if constexpr (EAGER) rec = ld_rec(ps_old + (size_t(p) * stride + i) * kSpringRec);
+        }
     }
-    __device__ spring9 load() const { return ld_rec(ps_old + (size_t(p) * stride + i) * kSpringRec); }
+    __device__ spring9 load() const {
+        if constexpr (EAGER) return rec;
+        else return ld_rec(ps_old + (size_t(p) * stride + i) * kSpringRec);
+    }
     __device__ void store(uint32_t tag, const spring9 &s) {
         if (q < cap) {
             if (q >= kNewTagBuf) pj_new[size_t(q) * stride + i] = tag;
@@ -172,9 +190,9 @@ struct id_list {
 
 // One ordered pair through the binary container fold
 // (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool EAGER>
 __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
-                                          walker &W, acc6 &acc) {
+                                          walker_t<EAGER> &W, acc6 &acc) {
     acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
     if constexpr (BIN != HG_BINARY_NONE) {
         W.advance_to(j);
@@ -230,9 +248,9 @@ __device__ __forceinline__ acc6 facet_fold(const hg_params &P, const facet_dev *
 
 // Mixed-precision fold of one ordered pair (dem_mixed.h); the spring walk is
 // the FP64 one.
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool EAGER>
 __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj,
-                                             d3 wj, walker &W, accf &acc) {
+                                             d3 wj, walker_t<EAGER> &W, accf &acc) {
     double dist;
     d3 n64;
     f3 n;
@@ -274,8 +292,8 @@ __device__ __forceinline__ accf facet_fold_mx(const hg_params &P, const mixed_co
     return s;
 }
 
-template <int BIN>
-__device__ __forceinline__ void close_table(walker &W, uint32_t *pcnt_new, uint32_t i, int *flags) {
+template <int BIN, bool EAGER>
+__device__ __forceinline__ void close_table(walker_t<EAGER> &W, uint32_t *pcnt_new, uint32_t i, int *flags) {
     if constexpr (BIN != HG_BINARY_NONE) {
         W.finish();
         pcnt_new[i] = W.q < W.cap ? W.q : W.cap;

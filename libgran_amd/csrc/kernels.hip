@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     if (i >= S.n) return;
     const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     acc6 acc{zero3(), zero3()};
-    walker W;
+    walker_t<false> W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     if (i >= S.n) return;
     const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     accf acc{zerof(), zerof()};
-    walker W;
+    walker_t<true> W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
         wi = ld3(S.w, i);
     }
     acc6 acc{zero3(), zero3()};
-    walker W;
+    walker_t<false> W;
     if constexpr (BIN != HG_BINARY_NONE) {
         if (live) W.init(S.pt, i);
     }

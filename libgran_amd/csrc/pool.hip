@@ -144,7 +144,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
     for (uint32_t g = g_first; g < g_end; g += g_step) {
         d3 xi, vi, wi;
         load_state<LDS>(V, M.offset, g, xi, vi, wi);
-        walker W;
+        walker_t<PREC == HG_PRECISION_MIXED> W;
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
         id_list ids;
         if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
