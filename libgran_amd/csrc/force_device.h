@@ -188,6 +188,32 @@ struct id_list {
     }
 };
 
+// A particle's listed partners in list order: fold(j, xj, vj, wj) for each,
+// with load(j, x, v, w) fetching a partner's state. The id of partner k+2 and
+// the state of partner k+1 are loaded before pair k is evaluated, so the state
+// load waits on an id that arrived an iteration earlier and both loads overlap
+// the pair's arithmetic.
+template <class LOAD, class FOLD>
+__device__ __forceinline__ void nl_partners(const id_list &ids, LOAD &&load, FOLD &&fold) {
+    uint32_t jn = 0, jn2 = 0;
+    d3 xn = zero3(), vn = zero3(), wn = zero3();
+    if (ids.n > 0) {
+        jn = ids.at(0);
+        load(jn, xn, vn, wn);
+        if (ids.n > 1) jn2 = ids.at(1);
+    }
+    for (uint32_t k = 0; k < ids.n; k++) {
+        const uint32_t j = jn;
+        const d3 xj = xn, vj = vn, wj = wn;
+        if (k + 1 < ids.n) {
+            jn = jn2;
+            load(jn, xn, vn, wn);
+            if (k + 2 < ids.n) jn2 = ids.at(k + 2);
+        }
+        fold(j, xj, vj, wj);
+    }
+}
+
 // One ordered pair through the binary container fold
 // (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
 template <int BIN, int HAM, bool EAGER>

@@ -35,23 +35,9 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    // the next partner's id and state are loaded before this pair is
-    // evaluated, so their latency overlaps its arithmetic
-    uint32_t jn = 0;
-    d3 xn = zero3(), vn = zero3(), wn = zero3();
-    if (ids.n > 0) {
-        jn = ids.at(0);
-        xn = ld3(S.x, jn), vn = ld3(S.v, jn), wn = ld3(S.w, jn);
-    }
-    for (uint32_t k = 0; k < ids.n; k++) {
-        const uint32_t j = jn;
-        const d3 xj = xn, vj = vn, wj = wn;
-        if (k + 1 < ids.n) {
-            jn = ids.at(k + 1);
-            xn = ld3(S.x, jn), vn = ld3(S.v, jn), wn = ld3(S.w, jn);
-        }
-        pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
-    }
+    nl_partners(
+        ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { x = ld3(S.x, j), v = ld3(S.v, j), w = ld3(S.w, j); },
+        [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc); });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
@@ -69,21 +55,9 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    uint32_t jn = 0;
-    d3 xn = zero3(), vn = zero3(), wn = zero3();
-    if (ids.n > 0) {
-        jn = ids.at(0);
-        xn = ld3(S.x, jn), vn = ld3(S.v, jn), wn = ld3(S.w, jn);
-    }
-    for (uint32_t k = 0; k < ids.n; k++) {
-        const uint32_t j = jn;
-        const d3 xj = xn, vj = vn, wj = wn;
-        if (k + 1 < ids.n) {
-            jn = ids.at(k + 1);
-            xn = ld3(S.x, jn), vn = ld3(S.v, jn), wn = ld3(S.w, jn);
-        }
-        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
-    }
+    nl_partners(
+        ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { x = ld3(S.x, j), v = ld3(S.v, j), w = ld3(S.w, j); },
+        [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc); });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold_mx(P, MC, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, tod(acc.lin));

@@ -151,23 +151,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                // the next partner's id and state are loaded before this
-                // pair is evaluated, so their latency overlaps its arithmetic
-                uint32_t jn = 0;
-                d3 xn = zero3(), vn = zero3(), wn = zero3();
-                if (ids.n > 0) {
-                    jn = ids.at(0);
-                    load_state<LDS>(V, M.offset, jn, xn, vn, wn);
-                }
-                for (uint32_t k = 0; k < ids.n; k++) {
-                    const uint32_t j = jn;
-                    const d3 xj = xn, vj = vn, wj = wn;
-                    if (k + 1 < ids.n) {
-                        jn = ids.at(k + 1);
-                        load_state<LDS>(V, M.offset, jn, xn, vn, wn);
-                    }
-                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
-                }
+                nl_partners(
+                    ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); },
+                    [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
@@ -182,21 +168,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         } else {
             accf acc{zerof(), zerof()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                uint32_t jn = 0;
-                d3 xn = zero3(), vn = zero3(), wn = zero3();
-                if (ids.n > 0) {
-                    jn = ids.at(0);
-                    load_state<LDS>(V, M.offset, jn, xn, vn, wn);
-                }
-                for (uint32_t k = 0; k < ids.n; k++) {
-                    const uint32_t j = jn;
-                    const d3 xj = xn, vj = vn, wj = wn;
-                    if (k + 1 < ids.n) {
-                        jn = ids.at(k + 1);
-                        load_state<LDS>(V, M.offset, jn, xn, vn, wn);
-                    }
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
-                }
+                nl_partners(
+                    ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); },
+                    [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
