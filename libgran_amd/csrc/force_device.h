@@ -94,11 +94,11 @@ struct walker_t {
         for (uint32_t s = 0; s < kTagPrefetch; s++) pre[s] = pj_old[size_t(s) * stride + i];
         p = 0;
         q = 0;
-        ej = kNoEntry;
-        if (cnt_old > 0) {
-            ej = pre[0];
-            if constexpr (EAGER) rec = ld_rec(ps_old + size_t(i) * kSpringRec);
-        }
+        // ej and rec describe old entry p only while p < cnt_old; they are
+        // loaded into the same registers whatever the count (slot 0 always
+        // exists), so no register copy has to wait for the loads
+        ej = pre[0];
+        if constexpr (EAGER) rec = ld_rec(ps_old + size_t(i) * kSpringRec);
 #pragma unroll
         for (uint32_t s = 0; s < kNewTagBuf; s++) nt[s] = kNoEntry;
     }
@@ -108,13 +108,14 @@ struct walker_t {
         for (uint32_t k = 1; k < kTagPrefetch; k++) t = s == k ? pre[k] : t;
         return s < kTagPrefetch ? t : pj_old[size_t(s) * stride + i];
     }
+    // Past the last entry the registers are reloaded from the last slot again
+    // (just read, so cached): a conditional reload would leave two values for
+    // the compiler to merge, and the merge waits for the load at once.
     __device__ void next() {
         p++;
-        ej = kNoEntry;
-        if (p < cnt_old) {
-            ej = tag_at(p);
-            if constexpr (EAGER) rec = ld_rec(ps_old + (size_t(p) * stride + i) * kSpringRec);
-        }
+        const uint32_t pc = p < cnt_old ? p : p - 1;
+        ej = pj_old[size_t(pc) * stride + i];
+        if constexpr (EAGER) rec = ld_rec(ps_old + (size_t(pc) * stride + i) * kSpringRec);
     }
     __device__ spring9 load() const {
         if constexpr (EAGER) return rec;
@@ -129,14 +130,21 @@ struct walker_t {
         }
         q++;
     }
+    // Copies the entries below j (pairs no longer listed; rare). The loop is
+    // entered through a separate test and its path ends with a full wait for
+    // its loads, so that the common path past it keeps exact wait counts
+    // (the compiler's count after a loop that loads is otherwise zero).
     __device__ void advance_to(uint32_t j) {
-        while ((ej & kIdxMask) < j) {
-            store(ej, load());
-            next();
+        if (p < cnt_old && (ej & kIdxMask) < j) {
+            do {
+                store(ej, load());
+                next();
+            } while (p < cnt_old && (ej & kIdxMask) < j);
+            __builtin_amdgcn_s_waitcnt(0);
         }
     }
     __device__ bool take(uint32_t j, spring9 &s, bool &bonded) {
-        if ((ej & kIdxMask) == j) {
+        if (p < cnt_old && (ej & kIdxMask) == j) {
             s = load();
             bonded = (ej & kBondBit) != 0;
             next();
@@ -186,30 +194,30 @@ struct id_list {
         for (uint32_t s = 1; s < kIdPrefetch; s++) t = k == s ? pre[s] : t;
         return k < kIdPrefetch ? t : nb[size_t(k) * stride + i];
     }
+    // Slot k, or the last slot for k >= n (n >= 1), always from memory.
+    __device__ uint32_t at_clamped(uint32_t k) const { return nb[size_t(k < n ? k : n - 1) * stride + i]; }
 };
 
 // A particle's listed partners in list order: fold(j, xj, vj, wj) for each,
 // with load(j, x, v, w) fetching a partner's state. The id of partner k+2 and
 // the state of partner k+1 are loaded before pair k is evaluated, so the state
 // load waits on an id that arrived an iteration earlier and both loads overlap
-// the pair's arithmetic.
+// the pair's arithmetic. The loads are unconditional (past the end of the list
+// they fetch the last partner again, from cache): a conditional load leaves
+// two values for the compiler to merge, and the merge waits for the load and
+// every memory operation issued before it.
 template <class LOAD, class FOLD>
 __device__ __forceinline__ void nl_partners(const id_list &ids, LOAD &&load, FOLD &&fold) {
-    uint32_t jn = 0, jn2 = 0;
-    d3 xn = zero3(), vn = zero3(), wn = zero3();
-    if (ids.n > 0) {
-        jn = ids.at(0);
-        load(jn, xn, vn, wn);
-        if (ids.n > 1) jn2 = ids.at(1);
-    }
+    if (ids.n == 0) return;
+    uint32_t jn = ids.at(0), jn2 = ids.at_clamped(1);
+    d3 xn, vn, wn;
+    load(jn, xn, vn, wn);
     for (uint32_t k = 0; k < ids.n; k++) {
         const uint32_t j = jn;
         const d3 xj = xn, vj = vn, wj = wn;
-        if (k + 1 < ids.n) {
-            jn = jn2;
-            load(jn, xn, vn, wn);
-            if (k + 2 < ids.n) jn2 = ids.at(k + 2);
-        }
+        jn = jn2;
+        load(jn, xn, vn, wn);
+        jn2 = ids.at_clamped(k + 2);
         fold(j, xj, vj, wj);
     }
 }
