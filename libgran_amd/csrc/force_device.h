@@ -198,6 +198,18 @@ struct id_list {
     __device__ uint32_t at_clamped(uint32_t k) const { return nb[size_t(k < n ? k : n - 1) * stride + i]; }
 };
 
+// The first n listed partners of local particle t of a system, copied to LDS
+// as offsets from the system's first particle `base`: slot k at
+// ids[k*nsys + t] (consecutive lanes read consecutive halfwords). LDS loads
+// are counted apart from global loads and stores, so waiting for an id does
+// not wait for the pair-state stores issued after it.
+struct lds_id_list {
+    const uint16_t *ids;
+    uint32_t nsys, t, base, n;
+    __device__ uint32_t at(uint32_t k) const { return base + ids[k * nsys + t]; }
+    __device__ uint32_t at_clamped(uint32_t k) const { return at(k < n ? k : n - 1); }
+};
+
 // A particle's listed partners in list order: fold(j, xj, vj, wj) for each,
 // with load(j, x, v, w) fetching a partner's state. The id of partner k+2 and
 // the state of partner k+1 are loaded before pair k is evaluated, so the state
@@ -206,8 +218,9 @@ struct id_list {
 // they fetch the last partner again, from cache): a conditional load leaves
 // two values for the compiler to merge, and the merge waits for the load and
 // every memory operation issued before it.
-template <class LOAD, class FOLD>
-__device__ __forceinline__ void nl_partners(const id_list &ids, LOAD &&load, FOLD &&fold) {
+// IDS is id_list (global memory) or lds_id_list.
+template <class IDS, class LOAD, class FOLD>
+__device__ __forceinline__ void nl_partners(const IDS &ids, LOAD &&load, FOLD &&fold) {
     if (ids.n == 0) return;
     uint32_t jn = ids.at(0), jn2 = ids.at_clamped(1);
     d3 xn, vn, wn;

@@ -121,13 +121,41 @@ __device__ __forceinline__ void lds_fill(const pool_view &V, const pool_meta &M)
     }
 }
 
+// LDS copy of a neighbour-list system's partner ids for one rebuild interval:
+// kmax slots per particle (kmax = the system's longest list), slot k of local
+// particle t at lids[k*n + t] as a halfword offset from the system's first
+// particle, then the list counts at lids[kmax*n + t]. A system of at most
+// 65,535 particles fits the halfwords.
+__host__ __device__ inline size_t lds_ids_bytes(uint32_t n, uint32_t kmax) { return size_t(n) * 2 * (kmax + 1); }
+
+__device__ __forceinline__ void lds_ids_fill(const pool_view &V, const pool_meta &M, uint32_t kmax, uint16_t *lids) {
+    for (uint32_t t = threadIdx.x; t < M.n; t += blockDim.x) {
+        const uint32_t g = M.offset + t, c = V.ncnt[g];
+        lids[kmax * M.n + t] = uint16_t(c);
+        for (uint32_t k = 0; k < c; k++) lids[k * M.n + t] = uint16_t(V.nb[size_t(k) * V.stride + g] - M.offset);
+    }
+}
+
+// Largest value of v over the workgroup (s_red: one word per wavefront).
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *s_red) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, uint32_t(__shfl_xor(int(v), o)));
+    if ((threadIdx.x & 63u) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = 0;
+    for (uint32_t w = 0; w < (blockDim.x + 63u) / 64u; w++) r = max(r, s_red[w]);
+    __syncthreads();
+    return r;
+}
+
 // Acceleration pass for particles g = g_first, g_first + g_step, ... < g_end
 // of system M (a workgroup strides over the whole system; a wide launch gives
-// each thread one particle).
-template <int BIN, int HAM, int PAIRS, int PREC, int LDS>
+// each thread one particle). With LIDS the partner ids come from the system's
+// LDS copy `lids` (lds_ids_fill).
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                                  const pool_meta &M, uint32_t cur, const facet_dev *F,
-                                                 uint32_t g_first, uint32_t g_end, uint32_t g_step) {
+                                                 uint32_t g_first, uint32_t g_end, uint32_t g_step,
+                                                 const uint16_t *lids = nullptr, uint32_t kmax = 0) {
     pair_table T;
     T.pj_old = cur ? V.pj1 : V.pj0;
     T.ps_old = cur ? V.ps1 : V.ps0;
@@ -147,13 +175,23 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         walker_t<PREC == HG_PRECISION_MIXED> W;
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
         id_list ids;
-        if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
+        if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST && !LIDS) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
+        // listed partners of g through fold, in list order
+        auto partners = [&](auto &&fold) {
+            auto ld = [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); };
+            if constexpr (LIDS) {
+                const uint32_t t = g - M.offset;
+                nl_partners(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ld, fold);
+            } else {
+                nl_partners(ids, ld, fold);
+            }
+        };
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                nl_partners(
-                    ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); },
-                    [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc); });
+                partners([&](uint32_t j, d3 xj, d3 vj, d3 wj) {
+                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
+                });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
@@ -168,9 +206,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         } else {
             accf acc{zerof(), zerof()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                nl_partners(
-                    ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); },
-                    [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc); });
+                partners([&](uint32_t j, d3 xj, d3 vj, d3 wj) {
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
+                });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
@@ -244,27 +282,33 @@ __device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_m
     }
 }
 
-// All steps of one rebuild interval for one system, by one workgroup.
-template <int BIN, int HAM, int PAIRS, int PREC, int LDS>
+// All steps of one rebuild interval for one system, by one workgroup. With
+// LIDS the system's partner ids (kmax slots per particle) are copied to LDS
+// after its state records.
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS>
 __device__ __forceinline__ void run_system(const hg_params &P, const mixed_consts &MC, const pool_view &V,
-                                           const pool_meta &M, facet_dev *F, uint32_t nsteps, double dt) {
+                                           const pool_meta &M, facet_dev *F, uint32_t nsteps, double dt,
+                                           uint32_t kmax) {
     const double half = dt / 2.0;
     uint32_t cur = M.cur;
     const uint32_t g0 = M.offset + threadIdx.x, g_end = M.offset + M.n, gs = blockDim.x;
-    if constexpr (LDS != STATE_GLOBAL) {
-        lds_fill<LDS>(V, M);
-        __syncthreads();
+    if constexpr (LDS != STATE_GLOBAL) lds_fill<LDS>(V, M);
+    uint16_t *lids = nullptr;
+    if constexpr (LIDS) {
+        lids = reinterpret_cast<uint16_t *>(
+            pool_lds + (LDS == STATE_GLOBAL ? 0 : size_t(M.n) * (LDS == STATE_FULL ? kFullStride : kMirrorStride)));
+        lds_ids_fill(V, M, kmax, lids);
     }
-    if (!M.acc_valid) {
-        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F, g0, g_end, gs);
-        cur ^= 1u;
-        __syncthreads();
-    }
-    for (uint32_t s = 0; s < nsteps; s++) {
-        pool_kick_drift<LDS>(V, M, s > 0, half, dt, g0, g_end, gs);
-        if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
-        __syncthreads();
-        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS>(P, MC, V, M, cur, F, g0, g_end, gs);
+    if constexpr (LDS != STATE_GLOBAL || LIDS) __syncthreads();
+    // step -1 is the initial acceleration pass of a system without valid
+    // accelerations; one call site keeps a single copy of the pass in the code
+    for (int s = M.acc_valid ? 0 : -1; s < int(nsteps); s++) {
+        if (s >= 0) {
+            pool_kick_drift<LDS>(V, M, s > 0, half, dt, g0, g_end, gs);
+            if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
+            __syncthreads();
+        }
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS, LIDS>(P, MC, V, M, cur, F, g0, g_end, gs, lids, kmax);
         cur ^= 1u;
         __syncthreads();
     }
@@ -293,6 +337,7 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
                                                     uint32_t count, uint32_t *counter, uint32_t nsteps, double dt,
                                                     uint32_t lds_bytes) {
     __shared__ uint32_t s_item;
+    __shared__ uint32_t s_red[4];
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -303,12 +348,28 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
         const pool_meta M = V.meta[slot];
         facet_dev *F = V.facets + size_t(slot) * V.max_facets;
         const size_t need = size_t(M.n) * sizeof(double);
-        if (need * kFullStride <= lds_bytes)
-            run_system<BIN, HAM, PAIRS, PREC, STATE_FULL>(P, MC, V, M, F, nsteps, dt);
-        else if (need * kMirrorStride <= lds_bytes)
-            run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR>(P, MC, V, M, F, nsteps, dt);
-        else
-            run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL>(P, MC, V, M, F, nsteps, dt);
+        if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
+            // the partner ids go to LDS too wherever they fit beside the state
+            uint32_t kmax = 0;
+            for (uint32_t t = threadIdx.x; t < M.n; t += blockDim.x) kmax = max(kmax, V.ncnt[M.offset + t]);
+            kmax = block_max_u32(kmax, s_red);
+            const size_t ids = M.n <= 65535u ? lds_ids_bytes(M.n, kmax) : ~size_t(0) / 2;
+            if (need * kFullStride + ids <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true>(P, MC, V, M, F, nsteps, dt, kmax);
+            else if (need * kMirrorStride + ids <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true>(P, MC, V, M, F, nsteps, dt, kmax);
+            else if (ids <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true>(P, MC, V, M, F, nsteps, dt, kmax);
+            else
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false>(P, MC, V, M, F, nsteps, dt, 0);
+        } else {
+            if (need * kFullStride <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, false>(P, MC, V, M, F, nsteps, dt, 0);
+            else if (need * kMirrorStride <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, false>(P, MC, V, M, F, nsteps, dt, 0);
+            else
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false>(P, MC, V, M, F, nsteps, dt, 0);
+        }
     }
 }
 
