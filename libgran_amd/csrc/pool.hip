@@ -474,6 +474,7 @@ __global__ void __launch_bounds__(256) k_pool_cell_fill(uint32_t hw, const uint3
     cell_items[bucket_start[h] + atomicAdd(&bucket_fill[h], 1u)] = g;
 }
 
+constexpr uint32_t kListRegs = 16;
 __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uint32_t *owner, const uint32_t *cell_of,
                                                       uint32_t hw, double cs, uint32_t mask, double r_verlet,
                                                       const uint32_t *bucket_count, const uint32_t *bucket_start,
@@ -485,36 +486,71 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
     const d3 xi = ld3(x, g);
     long long cx, cy, cz;
     pool_cell(xi, cs, cx, cy, cz);
-    uint32_t seen[27];
-    int nseen = 0;
+    // squared-distance prefilter: a correctly rounded sqrt(d2) < r_verlet
+    // implies d2 < r_verlet^2 (1 + 2^-52), so this bound drops no pair the
+    // exact test keeps
+    const double d2_max = r_verlet * r_verlet * (1.0 + 1e-12);
+    // The list is kept sorted in registers (indices fixed at compile time,
+    // so no scratch memory) while it has at most kListRegs entries, and in
+    // nb itself by insertion once it grows past that.
+    uint32_t r[kListRegs];
+#pragma unroll
+    for (uint32_t s = 0; s < kListRegs; s++) r[s] = 0;
     uint32_t cnt = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const uint32_t h = pool_hash(slot, cx + dx, cy + dy, cz + dz, mask);
-                bool dup = false;
-                for (int s = 0; s < nseen; s++) dup |= seen[s] == h;
-                if (dup) continue;
-                seen[nseen++] = h;
-                const uint32_t b = bucket_start[h], e = b + bucket_count[h];
-                for (uint32_t q = b; q < e; q++) {
-                    const uint32_t j = cell_items[q];
-                    if (j == g || owner[j] != slot) continue;
-                    if (norm(xi - ld3(x, j)) < r_verlet) {
-                        if (cnt < cap) {
-                            uint32_t pos = cnt;
-                            while (pos > 0) {
-                                const uint32_t prev = nb[size_t(pos - 1) * stride + g];
-                                if (prev < j) break;
-                                nb[size_t(pos) * stride + g] = prev;
-                                pos--;
-                            }
-                            nb[size_t(pos) * stride + g] = j;
-                        }
-                        cnt++;
-                    }
+    bool spilled = false;
+    auto insert = [&](uint32_t j) {
+        if (!spilled && cnt == kListRegs) {
+#pragma unroll
+            for (uint32_t s = 0; s < kListRegs; s++)
+                if (s < cap) nb[size_t(s) * stride + g] = r[s];
+            spilled = true;
+        }
+        if (spilled) {
+            if (cnt < cap) {
+                uint32_t pos = cnt;
+                while (pos > 0) {
+                    const uint32_t prev = nb[size_t(pos - 1) * stride + g];
+                    if (prev < j) break;
+                    nb[size_t(pos) * stride + g] = prev;
+                    pos--;
                 }
+                nb[size_t(pos) * stride + g] = j;
             }
+        } else {
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t s = 0; s < kListRegs; s++) pos += (s < cnt && r[s] < j) ? 1u : 0u;
+#pragma unroll
+            for (uint32_t s = kListRegs - 1; s >= 1; s--) r[s] = s > pos ? r[s - 1] : (s == pos ? j : r[s]);
+            r[0] = pos == 0 ? j : r[0];
+        }
+        cnt++;
+    };
+    // the 27 cells unrolled, so the hashes already visited (a bucket shared
+    // by two cells is scanned once) sit in registers too
+    uint32_t seen[27];
+#pragma unroll
+    for (int c = 0; c < 27; c++) {
+        const uint32_t h = pool_hash(slot, cx + (c % 3 - 1), cy + (c / 3 % 3 - 1), cz + (c / 9 - 1), mask);
+        seen[c] = h;
+        bool dup = false;
+#pragma unroll
+        for (int s = 0; s < c; s++) dup |= seen[s] == h;
+        if (dup) continue;
+        const uint32_t b = bucket_start[h], e = b + bucket_count[h];
+        for (uint32_t q = b; q < e; q++) {
+            const uint32_t j = cell_items[q];
+            if (j == g || owner[j] != slot) continue;
+            const d3 d = xi - ld3(x, j);
+            if (!(dot(d, d) < d2_max)) continue;
+            if (norm(d) < r_verlet) insert(j);
+        }
+    }
+    if (!spilled) {
+#pragma unroll
+        for (uint32_t s = 0; s < kListRegs; s++)
+            if (s < cnt && s < cap) nb[size_t(s) * stride + g] = r[s];
+    }
     ncnt[g] = cnt < cap ? cnt : cap;
     if (cnt > cap) atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
     atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
