@@ -188,7 +188,9 @@ COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counte
 
 def measured_traffic(tuning, count):
     """HBM bytes per chunk launch from the committed FETCH_SIZE/WRITE_SIZE
-    profile (copy-calibrated), valid for the split it was measured at."""
+    profile (copy-calibrated). Valid when the run's wide threshold is the
+    profile's: the threshold decides which systems the chunk kernel steps, the
+    CU room left for the wide launches only its grid."""
     try:
         txt = open(COUNTERS).read()
     except OSError:
@@ -197,9 +199,11 @@ def measured_traffic(tuning, count):
     split = re.search(r"launch split fixed at\s*(\d+):(\d+)", txt)
     if not m or not split:
         return None, "counter profile unreadable"
-    if count != 4096 or (tuning["wide_min"], tuning["wide_cus"]) != (int(split.group(1)), int(split.group(2))):
-        return None, f"counter profile was measured at split {split.group(1)}:{split.group(2)}, this run differs"
-    return int(m.group(1).replace(",", "")), "profiles/pool_chunk_mixed_batch4096_hbm_counters.txt"
+    if count != 4096 or tuning["wide_min"] != int(split.group(1)):
+        return None, f"counter profile was measured at wide threshold {split.group(1)}, this run differs"
+    return (int(m.group(1).replace(",", "")),
+            f"profiles/pool_chunk_mixed_batch4096_hbm_counters.txt (measured at split {split.group(1)}:"
+            f"{split.group(2)})")
 
 
 def cpu_baseline(x, v0, steps):
