@@ -53,14 +53,17 @@ typedef struct hg_sinter_params {
 } hg_sinter_params;
 
 /* hamaker_functor (hamaker_force.h:20-26) / limited_range_hamaker_functor
- * (limited_range_hamaker_force.h:20-27) / surface_hamaker_functor
+ * (limited_range_hamaker_force.h:20-27) / simplified_hamaker_functor
+ * (simplified_hamaker_force.h:10-16) / surface_hamaker_functor
  * (surface_hamaker_force.h:12-18). r_cutoff is read only in LIMITED mode. */
 typedef struct hg_hamaker_params {
     double A, h0, r_cutoff, r_part, mass;
 } hg_hamaker_params;
 
 enum { HG_BINARY_NONE = 0, HG_BINARY_CONTACT = 1, HG_BINARY_SINTER = 2 };
-enum { HG_HAMAKER_NONE = 0, HG_HAMAKER_FULL = 1, HG_HAMAKER_LIMITED = 2 };
+/* HG_HAMAKER_SIMPLIFIED: simplified_hamaker_functor
+ * (simplified_hamaker_force.h:18-30), 2 A r / h^2 along x_j - x_i. */
+enum { HG_HAMAKER_NONE = 0, HG_HAMAKER_FULL = 1, HG_HAMAKER_LIMITED = 2, HG_HAMAKER_SIMPLIFIED = 3 };
 /* HG_PAIRS_ALL: every ordered pair j != i, as granular_system
  *   (granular_system.h:25-86).
  * HG_PAIRS_NEIGHBOR_LIST: listed pairs only, as granular_system_neighbor_list

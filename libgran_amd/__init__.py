@@ -14,12 +14,14 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_NONE,
+                   HAMAKER_SIMPLIFIED,
                    PAIRS_ALL, PAIRS_NEIGHBOR_LIST, PRECISION_FP64, PRECISION_MIXED, HipgranError, Params)
 
 _abi.lib()  # fail at import time when the HIP library is absent
 
 __all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
            "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
+           "HAMAKER_SIMPLIFIED",
            "PAIRS_ALL", "PAIRS_NEIGHBOR_LIST", "PRECISION_FP64", "PRECISION_MIXED"]
 
 

@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "hipgran.h")
 
 HG_OK, HG_ERR_INVALID, HG_ERR_OOM, HG_ERR_HIP, HG_ERR_NUMERIC, HG_ERR_CAPACITY, HG_ERR_NO_DEVICE = range(7)
 BINARY_NONE, BINARY_CONTACT, BINARY_SINTER = 0, 1, 2
-HAMAKER_NONE, HAMAKER_FULL, HAMAKER_LIMITED = 0, 1, 2
+HAMAKER_NONE, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_SIMPLIFIED = 0, 1, 2, 3
 PAIRS_ALL, PAIRS_NEIGHBOR_LIST = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 

@@ -172,6 +172,15 @@ HGD acc6 lr_hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
     return hamaker_eval(p, xi, xj);
 }
 
+// ---- simplified_hamaker_force.h:18-30: 2 A r / h^2 along x_j - x_i ----------
+HGD acc6 simplified_hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
+    d3 r = xj - xi;
+    double h = norm(r) - 2.0 * p.r_part;
+    if (h < p.h0) h = p.h0;
+    d3 f = 2.0 * p.A * p.r_part / sq(h) * normalized(r);
+    return {f / p.mass, zero3()};
+}
+
 // ---- triangular facet (triangular_facet.h:18-137) --------------------------
 struct facet_dev {
     d3 vert[3];

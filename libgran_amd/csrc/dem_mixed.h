@@ -58,7 +58,7 @@ struct mixed_consts {
     float c_k, c_gn, c_kt, c_gt, c_mus, c_mud, c_kr, c_gr, c_mur, c_mudr, c_ko, c_go, c_muo, c_mudo, c_r, c_im, c_ii;
     float s_k, s_gn, s_kt, s_gt, s_kr, s_gr, s_ko, s_go, s_r, s_im, s_ii, s_m;
     int s_breaking;
-    float h_a6, h_r, h_im;   // -A/6, r, 1/m
+    float h_a6, h_r, h_im, h_2ar;   // -A/6, r, 1/m, 2 A r
     double c_r2, s_r2, h_r2, h_h0, h_cut;
     double c_dt, s_dt;
     float sc_k, sc_gn, sc_kt, sc_gt, sc_mus, sc_mud, sc_kr, sc_gr, sc_mur, sc_mudr, sc_ko, sc_go, sc_muo, sc_mudo, sc_r,
@@ -85,6 +85,7 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
     m.s_m = float(s.mass); m.s_breaking = p.sinter_breaking;
     const hg_hamaker_params &h = p.hamaker;
     m.h_a6 = float(-h.A / 6.0); m.h_r = float(h.r_part); m.h_im = float(1.0 / h.mass);
+    m.h_2ar = float(2.0 * h.A * h.r_part);
     m.h_r2 = 2.0 * h.r_part; m.h_h0 = h.h0; m.h_cut = h.r_cutoff;
     const hg_contact_params &sc = p.surface_contact;
     m.sc_k = float(sc.k); m.sc_gn = float(sc.gamma_n); m.sc_kt = float(sc.k_t); m.sc_gt = float(sc.gamma_t);
@@ -209,7 +210,7 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
     return {(f_n * n + f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
 }
 
-// Hamaker (full or limited range) with an FP64 gap.
+// Hamaker (full, limited range or simplified) with an FP64 gap.
 template <int HAM>
 __device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 n) {
 #pragma clang fp contract(fast)
@@ -217,6 +218,11 @@ __device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 
     double hd = dist - m.h_r2;
     if (hd < m.h_h0) hd = m.h_h0;
     const float h = float(hd), rp = m.h_r;
+    if constexpr (HAM == HG_HAMAKER_SIMPLIFIED) {
+        // simplified_hamaker_force.h:26: 2 A r / h^2 along x_j - x_i = -n
+        const float ih = rcp32(h);
+        return (-(m.h_2ar * ih * ih) * m.h_im) * n;
+    }
     const float a = 4.0f * rp + h, b = 2.0f * rp + h;
     const float ia = rcp32(a), ib = rcp32(b), ih = rcp32(h);
     const float s = m.h_a6 * ((a + h) * ia * ih - 2.0f * ib - 4.0f * rp * rp * ib * ib * ib -

@@ -262,6 +262,7 @@ __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi,
     }
     if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
     if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
+    if constexpr (HAM == HG_HAMAKER_SIMPLIFIED) h = simplified_hamaker_eval(P.hamaker, xi, xj);
     acc6 e;
     if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) e = b + h;
     else if constexpr (BIN != HG_BINARY_NONE) e = b;

@@ -8,7 +8,8 @@
 //   neighbour list: transposed fixed-capacity slots, nb[k*n + i], ascending j
 //     per particle; consecutive lanes (particles) read consecutive words.
 //   pair-state table: per particle a j-sorted list of (tag, spring9) where
-//     tag = j | bonded-bit, transposed as pj[s*n + i] and ps[(s*9+c)*n + i].
+//     tag = j | bonded-bit; tags transposed as pj[s*n + i], springs in one
+//     contiguous 80-byte record per entry at ps[(s*n + i)*10].
 //     It holds every bonded pair and every pair whose contact spring the
 //     reference would hold non-zero. Each acceleration pass reads the old
 //     table and writes the new one while walking the partners in ascending j,
@@ -411,12 +412,15 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_SIMPLIFIED)
     HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
     HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_SIMPLIFIED)
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_SIMPLIFIED)
 #undef HG_CASE
     return hipErrorInvalidValue;
 }

@@ -601,12 +601,15 @@ hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_SIMPLIFIED)
     HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_PCASE(HG_BINARY_CONTACT, HG_HAMAKER_SIMPLIFIED)
     HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+    HG_PCASE(HG_BINARY_SINTER, HG_HAMAKER_SIMPLIFIED)
 #undef HG_PCASE
     return hipErrorInvalidValue;
 }
@@ -631,12 +634,15 @@ hipError_t launch_wide_force_p(const hg_params &p, const pool_view &V, const wid
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_SIMPLIFIED)
     HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
     HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
     HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_WCASE(HG_BINARY_CONTACT, HG_HAMAKER_SIMPLIFIED)
     HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
     HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
     HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+    HG_WCASE(HG_BINARY_SINTER, HG_HAMAKER_SIMPLIFIED)
 #undef HG_WCASE
     return hipErrorInvalidValue;
 }

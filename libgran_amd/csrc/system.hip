@@ -196,7 +196,7 @@ hg_status validate(const hg_params *p, size_t n) {
     if (n == 0 || n >= (size_t(1) << 31)) return fail(HG_ERR_INVALID, "n must be in [1, 2^31)");
     if (p->binary_model < HG_BINARY_NONE || p->binary_model > HG_BINARY_SINTER)
         return fail(HG_ERR_INVALID, "unknown binary_model");
-    if (p->hamaker_model < HG_HAMAKER_NONE || p->hamaker_model > HG_HAMAKER_LIMITED)
+    if (p->hamaker_model < HG_HAMAKER_NONE || p->hamaker_model > HG_HAMAKER_SIMPLIFIED)
         return fail(HG_ERR_INVALID, "unknown hamaker_model");
     if (p->pair_mode != HG_PAIRS_ALL && p->pair_mode != HG_PAIRS_NEIGHBOR_LIST)
         return fail(HG_ERR_INVALID, "unknown pair_mode");

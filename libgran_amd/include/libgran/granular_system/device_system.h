@@ -7,7 +7,8 @@
 // Supported functor sets (anything else is a compile-time error: there is no
 // host evaluation path): in the binary container at most one of
 // contact_force_functor / alt_sinter_functor and at most one of
-// hamaker_functor / limited_range_hamaker_functor; in the unary container
+// hamaker_functor / limited_range_hamaker_functor / simplified_hamaker_functor;
+// in the unary container
 // triangular facets carrying surface_contact_force_functor and/or
 // surface_hamaker_functor with the same parameters on every facet; the
 // default integrator and step handler.
@@ -44,12 +45,13 @@ class device_system {
     using UK = tuple_kinds<decltype(unary_container_t::unary_force_functors)>;
     static_assert(BK::size > 0, "at least one binary force functor must be provided as a template parameter");
     static_assert(BK::all_known(), "the device path implements contact_force_functor, alt_sinter_functor, "
-                                   "hamaker_functor and limited_range_hamaker_functor only");
+                                   "hamaker_functor, limited_range_hamaker_functor and simplified_hamaker_functor only");
     static_assert(BK::count(kind_contact) + BK::count(kind_sinter) <= 1,
                   "at most one contact-type functor (contact_force_functor or alt_sinter_functor)");
-    static_assert(BK::count(kind_hamaker) + BK::count(kind_limited_hamaker) <= 1, "at most one Hamaker functor");
+    static_assert(BK::count(kind_hamaker) + BK::count(kind_limited_hamaker) + BK::count(kind_simplified_hamaker) <= 1,
+                  "at most one Hamaker functor");
     static_assert(BK::count(kind_contact) + BK::count(kind_sinter) + BK::count(kind_hamaker) +
-                          BK::count(kind_limited_hamaker) == BK::size,
+                          BK::count(kind_limited_hamaker) + BK::count(kind_simplified_hamaker) == BK::size,
                   "unsupported binary functor");
     static_assert(UK::count(kind_facet) == UK::size, "the unary container may hold triangular_facet objects only");
 

@@ -27,7 +27,8 @@ enum functor_kind : int {
     kind_limited_hamaker = 4,
     kind_surface_contact = 5,
     kind_surface_hamaker = 6,
-    kind_facet = 7
+    kind_facet = 7,
+    kind_simplified_hamaker = 8
 };
 
 template <typename T, typename = void>

@@ -18,6 +18,7 @@
 //   sinter_topology          include/libgran/sinter_bridge/alt_sinter_bridge.h:19-90
 //   hamaker_eval             include/libgran/hamaker_force/hamaker_force.h:28-43
 //   lr_hamaker_eval          include/libgran/hamaker_force/limited_range_hamaker_force.h:29-47
+//   simplified_hamaker_eval  include/libgran/hamaker_force/simplified_hamaker_force.h:18-30
 //   pair fold                include/libgran/granular_system/binary_force_container.h:20-32
 //                            + include/libgran/utility/operator_overloads.h:10-20
 //   unary fold               include/libgran/granular_system/unary_force_container.h:20-32
@@ -101,7 +102,7 @@ struct hamaker_params {
 };
 
 enum { BIN_NONE = 0, BIN_CONTACT = 1, BIN_SINTER = 2 };
-enum { HAM_NONE = 0, HAM_FULL = 1, HAM_LIMITED = 2 };
+enum { HAM_NONE = 0, HAM_FULL = 1, HAM_LIMITED = 2, HAM_SIMPLIFIED = 3 };
 
 struct spring3 {
     v3 s[3];
@@ -271,6 +272,22 @@ inline acc2 hamaker_eval(const hamaker_params &p, v3 xi_, v3 xj) {
 inline acc2 lr_hamaker_eval(const hamaker_params &p, v3 xi_, v3 xj) {
     if (norm(xi_ - xj) > p.r_cutoff) return {vzero, vzero};
     return hamaker_eval(p, xi_, xj);
+}
+
+// simplified_hamaker_force.h:18-30: 2 A r / h^2 along x_j - x_i, h clamped at h0
+inline acc2 simplified_hamaker_eval(const hamaker_params &p, v3 xi_, v3 xj) {
+    v3 r = xj - xi_;
+    double h = norm(r) - 2.0 * p.r_part;
+    if (h < p.h0) h = p.h0;
+    v3 f = 2.0 * p.A * p.r_part / square(h) * normalized(r);
+    return {f / p.mass, vzero};
+}
+
+// the second functor of the binary container, by model
+inline acc2 binary_hamaker_eval(int model, const hamaker_params &p, v3 xi_, v3 xj) {
+    if (model == HAM_FULL) return hamaker_eval(p, xi_, xj);
+    if (model == HAM_SIMPLIFIED) return simplified_hamaker_eval(p, xi_, xj);
+    return lr_hamaker_eval(p, xi_, xj);
 }
 
 // ------------------------------------------------------------- facets -------
@@ -485,8 +502,7 @@ public:
             }
         }
         if (have_h) {
-            h = p.hamaker_model == HAM_FULL ? hamaker_eval(p.hamaker, x[i], x[j])
-                                            : lr_hamaker_eval(p.hamaker, x[i], x[j]);
+            h = binary_hamaker_eval(p.hamaker_model, p.hamaker, x[i], x[j]);
         }
         if (have_b && have_h) return b + h;
         return have_b ? b : h;
@@ -495,8 +511,18 @@ public:
     // The fold of one ordered pair on copies of its springs (no state change):
     // the denominator of per-particle force errors, sum_j |a_ij|.
     acc2 pair_peek(size_t i, uint32_t j) const {
+        acc2 b, h;
+        pair_peek_parts(i, j, b, h);
         bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
-        acc2 b{vzero, vzero}, h{vzero, vzero};
+        if (have_b && have_h) return b + h;
+        return have_b ? b : h;
+    }
+
+    // The two functors of one ordered pair separately (zero when absent).
+    void pair_peek_parts(size_t i, uint32_t j, acc2 &b, acc2 &h) const {
+        bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
+        b = acc2{vzero, vzero};
+        h = acc2{vzero, vzero};
         auto find = [&](const spring_rows &rows) {
             spring3 s{{vzero, vzero, vzero}};
             const auto &r = rows.rows[i];
@@ -516,9 +542,7 @@ public:
             }
         }
         if (have_h)
-            h = p.hamaker_model == HAM_FULL ? hamaker_eval(p.hamaker, x[i], x[j]) : lr_hamaker_eval(p.hamaker, x[i], x[j]);
-        if (have_b && have_h) return b + h;
-        return have_b ? b : h;
+            h = binary_hamaker_eval(p.hamaker_model, p.hamaker, x[i], x[j]);
     }
 
     // triangular_facet.h:32-51 then the unary container's right fold over facets.

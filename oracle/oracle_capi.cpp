@@ -139,14 +139,15 @@ void or_sys_set_neck_strengths(void *s, const double *strengths) {
     static_cast<dem_system *>(s)->set_neck_strengths(strengths);
 }
 
-// mode 1: hamaker_functor, mode 2: limited_range_hamaker_functor.
+// mode 1: hamaker_functor, 2: limited_range_hamaker_functor,
+// 3: simplified_hamaker_functor.
 void or_hamaker_seq(int mode, const double *ham5, size_t nsteps, const double *states, double *out) {
     hamaker_params hp = hp_from(ham5);
     for (size_t k = 0; k < nsteps; k++) {
         const double *s = states + 18 * k;
         v3 x0 = ld(s), x1 = ld(s + 3);
-        acc2 e01 = mode == 1 ? hamaker_eval(hp, x0, x1) : lr_hamaker_eval(hp, x0, x1);
-        acc2 e10 = mode == 1 ? hamaker_eval(hp, x1, x0) : lr_hamaker_eval(hp, x1, x0);
+        acc2 e01 = binary_hamaker_eval(mode, hp, x0, x1);
+        acc2 e10 = binary_hamaker_eval(mode, hp, x1, x0);
         st(out + 12 * k, e01.lin);
         st(out + 12 * k + 3, e01.ang);
         st(out + 12 * k + 6, e10.lin);
@@ -323,6 +324,34 @@ size_t or_sys_get_springs(void *s, int which, uint32_t *ij, double *springs, siz
             tot++;
         }
     return tot;
+}
+
+// Per particle: sum over evaluated partners and over the functors of each
+// pair of |functor acceleration| (linear, angular), without changing any
+// spring: the scale of the terms a rounded fold adds, including terms that
+// cancel inside one pair (a bond holding a clamped Hamaker pull). out = 2*n.
+void or_sys_functor_magnitudes(void *s, double *out) {
+    auto *S = static_cast<dem_system *>(s);
+    const long nn = long(S->n);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long ii = 0; ii < nn; ii++) {
+        size_t i = size_t(ii);
+        double sl = 0.0, sa = 0.0;
+        auto add = [&](uint32_t j) {
+            acc2 b, h;
+            S->pair_peek_parts(i, j, b, h);
+            sl += norm(b.lin) + norm(h.lin);
+            sa += norm(b.ang) + norm(h.ang);
+        };
+        if (S->p.neighbor_mode == 1) {
+            for (uint32_t j : S->nbr[i]) add(j);
+        } else {
+            for (uint32_t j = 0; j < uint32_t(S->n); j++)
+                if (j != i) add(j);
+        }
+        out[2 * i] = sl;
+        out[2 * i + 1] = sa;
+    }
 }
 
 // Per particle: sum over evaluated partners of |a_ij| (linear) and |alpha_ij|

@@ -86,7 +86,7 @@ def _fill(struct, arr):
 
 
 BIN_NONE, BIN_CONTACT, BIN_SINTER = 0, 1, 2
-HAM_NONE, HAM_FULL, HAM_LIMITED = 0, 1, 2
+HAM_NONE, HAM_FULL, HAM_LIMITED, HAM_SIMPLIFIED = 0, 1, 2, 3
 
 
 def system_params(binary_model, hamaker_model, neighbor_mode=0, r_verlet=0.0, contact=None, sinter=None,
@@ -134,6 +134,7 @@ def lib():
         L.or_sys_get_springs.argtypes = [C.c_void_p, C.c_int, _u32p, _dp, C.c_size_t]
         L.or_sys_diagnostics.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
         L.or_sys_pair_magnitudes.argtypes = [C.c_void_p, _dp]
+        L.or_sys_functor_magnitudes.argtypes = [C.c_void_p, _dp]
         L.or_contact_seq.argtypes = [_dp, C.c_size_t, _dp, _dp, _dp]
         L.or_sinter_seq.argtypes = [_dp, _dp, C.c_size_t, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.or_hamaker_seq.argtypes = [C.c_int, _dp, C.c_size_t, _dp, _dp]
@@ -158,6 +159,7 @@ def ref():
         R.ref_sinter_seq.argtypes = [_dp, _dp, C.c_size_t, _dp, _dp, _dp, C.POINTER(C.c_int)]
         R.ref_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
         R.ref_lr_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        R.ref_simplified_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
         R.ref_fold_seq.argtypes = [_dp, _dp, _dp, C.c_size_t, _dp, _dp]
         R.ref_facet_seq.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_size_t, _dp, _dp]
         R.ref_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
@@ -275,7 +277,8 @@ def ref_sinter_seq(sinter, contact, states):
 def ref_hamaker_seq(mode, ham, states):
     states = c64(states)
     out = np.zeros((states.shape[0], 12))
-    fn = ref().ref_hamaker_seq if mode == HAM_FULL else ref().ref_lr_hamaker_seq
+    fn = {HAM_FULL: ref().ref_hamaker_seq, HAM_LIMITED: ref().ref_lr_hamaker_seq,
+          HAM_SIMPLIFIED: ref().ref_simplified_hamaker_seq}[mode]
     fn(dptr(c64(ham)), states.shape[0], dptr(states), dptr(out))
     return out
 
@@ -378,6 +381,13 @@ class OracleSystem:
         """(n, 2): sum_j |a_ij| and sum_j |alpha_ij| at the current state."""
         out = np.zeros((self.n, 2))
         lib().or_sys_pair_magnitudes(self._h, dptr(out))
+        return out
+
+    def functor_magnitudes(self):
+        """(n, 2): sum over pairs and over each pair's functors of the functor's
+        |a| and |alpha| (no cancellation inside a pair) at the current state."""
+        out = np.zeros((self.n, 2))
+        lib().or_sys_functor_magnitudes(self._h, dptr(out))
         return out
 
     def diagnostics(self, mass=MASS, inertia=INERTIA):

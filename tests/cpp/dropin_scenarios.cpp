@@ -6,7 +6,7 @@
 // with the CPU oracle.
 //
 //   dropin_scenarios <scenario> <steps> <output-file>
-//   scenarios: contact hamaker hamaker_neighbor alt_sintered breaking surface plate
+//   scenarios: contact hamaker hamaker_neighbor simplified alt_sintered breaking surface plate
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +19,7 @@
 #include <libgran/granular_system/granular_system.h>
 #include <libgran/granular_system/granular_system_neighbor_list.h>
 #include <libgran/hamaker_force/hamaker_force.h>
+#include <libgran/hamaker_force/simplified_hamaker_force.h>
 #include <libgran/hamaker_force/surface_hamaker_force.h>
 #include <libgran/sinter_bridge/alt_breaking_sinter_bridge.h>
 #include <libgran/sinter_bridge/alt_sinter_bridge.h>
@@ -103,6 +104,29 @@ int run_contact(size_t steps, const char *out, bool with_hamaker, bool neighbor)
         if (n % 20 == 0) system.update_neighbor_list();
         system.do_step(dt);
     }
+    finish(system, out);
+    return 0;
+}
+
+// test/hamaker.cpp's set-up with simplified_hamaker_functor in place of
+// hamaker_functor (all pairs).
+int run_simplified(size_t steps, const char *out) {
+    VV x0, v0;
+    lattice_with_impactor(x0, v0);
+    VV th0(x0.size()), w0(x0.size());
+    const double dt = 1e-13;
+    rotational_step_handler<VV, V3> handler;
+    contact_force_functor<V3, double> contact(x0.size(), k, gamma_n, k, gamma_t, mu, phi, k, gamma_r, mu_o, phi, k,
+                                              gamma_o, mu_o, phi, r_part, mass, inertia, dt, V3(), 0.0);
+    simplified_hamaker_functor<V3, double> hamaker(1.0e-20, 1.0e-9, r_part, mass, V3(), 0.0);
+    unary_force_functor_container<V3, double> unary;
+    binary_force_functor_container<V3, double, contact_force_functor<V3, double>,
+                                   simplified_hamaker_functor<V3, double>>
+        binary{contact, hamaker};
+    granular_system<V3, double, rotational_velocity_verlet_half, rotational_step_handler, decltype(binary),
+                    decltype(unary)>
+        system(x0, v0, th0, w0, 0.0, V3(), 0.0, handler, binary, unary);
+    for (size_t n = 0; n < steps; n++) system.do_step(dt);
     finish(system, out);
     return 0;
 }
@@ -241,6 +265,7 @@ int main(int argc, char **argv) {
         if (s == "contact") return run_contact(steps, argv[3], false, false);
         if (s == "hamaker") return run_contact(steps, argv[3], true, false);
         if (s == "hamaker_neighbor") return run_contact(steps, argv[3], true, true);
+        if (s == "simplified") return run_simplified(steps, argv[3]);
         if (s == "alt_sintered") return run_alt_sintered(steps, argv[3]);
         if (s == "breaking") return run_breaking(steps, argv[3]);
         if (s == "surface") return run_surface(steps, argv[3], false);

@@ -82,6 +82,7 @@ def main():
         out[f"ham_{seed}_states"] = s
         out[f"ham_full_{seed}_out"] = O.ref_hamaker_seq(O.HAM_FULL, ham, s)
         out[f"ham_lr_{seed}_out"] = O.ref_hamaker_seq(O.HAM_LIMITED, ham, s)
+        out[f"ham_simplified_{seed}_out"] = O.ref_hamaker_seq(O.HAM_SIMPLIFIED, ham, s)
     for seed in range(2):
         s = _seq.bonded_sequence(400 + seed, amp=0.08)
         out[f"fold_{seed}_states"] = s

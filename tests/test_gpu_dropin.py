@@ -80,6 +80,17 @@ def test_dropin_hamaker_neighbor_list(tmp_path):
     compare(o, got, "hamaker_neighbor")
 
 
+def test_dropin_simplified_hamaker(tmp_path):
+    """test/hamaker.cpp set-up with simplified_hamaker_functor as the second
+    functor of the binary container."""
+    steps = 4000
+    got = run_driver(tmp_path, "simplified", steps)
+    x, v = S.lattice_with_impactor()
+    o = oracle(O.BIN_CONTACT, O.HAM_SIMPLIFIED, x, v, hamaker=O.hamaker_params(A=1e-20, h0=1e-9))
+    o.step(1e-13, steps)
+    compare(o, got, "simplified")
+
+
 def test_dropin_alt_sintered(tmp_path):
     """test/alt_sintered.cpp set-up: alt_sinter_functor built from x0 (the
     driver also checks the dense bonded_contacts view and that

@@ -105,6 +105,29 @@ def test_golden_hamaker_bit_exact():
         np.testing.assert_array_equal(lr[d < hp[2]], full[d < hp[2]])
 
 
+def test_golden_simplified_hamaker_bit_exact():
+    """simplified_hamaker_functor (simplified_hamaker_force.h:18-30): the
+    oracle equals the reference functor bit for bit; independently, the force
+    is 2 A r / h^2 along x_j - x_i with h = |x_j - x_i| - 2r clamped at h0,
+    no torque, and the two ordered pairs are equal and opposite."""
+    hp = GOLD["ham_params"]
+    A, h0, r, m = hp[0], hp[1], hp[3], hp[4]
+    clamped = 0
+    for seed in range(2):
+        s = GOLD[f"ham_{seed}_states"]
+        ref = GOLD[f"ham_simplified_{seed}_out"]
+        np.testing.assert_array_equal(O.oracle_hamaker_seq(O.HAM_SIMPLIFIED, hp, s), ref)
+        rv = s[:, 3:6] - s[:, 0:3]
+        d = np.linalg.norm(rv, axis=1)
+        h = np.maximum(d - 2.0 * r, h0)
+        clamped += int(np.sum(d - 2.0 * r < h0))
+        want = (2.0 * A * r / h**2 / m)[:, None] * rv / d[:, None]
+        np.testing.assert_allclose(ref[:, 0:3], want, rtol=1e-13, atol=0.0)
+        assert np.all(ref[:, 3:6] == 0.0) and np.all(ref[:, 9:12] == 0.0)
+        np.testing.assert_array_equal(ref[:, 6:9], -ref[:, 0:3])
+    assert clamped > 0  # the sequences reach the h0 clamp
+
+
 def test_golden_fold_bit_exact():
     for seed in range(2):
         s = GOLD[f"fold_{seed}_states"]
