@@ -14,7 +14,9 @@ tuned online during the warm-up steps.
 
 Beside it on the same line: the same batch in an FP64 pool, the plain FP64
 GPU path (one system at a time, per-step launches) on a stratified sample of
-the batch, the single N=20,000 aggregate (FP64 and mixed), the serving stream
+the batch, the single N=20,000 aggregate (FP64 and mixed), the energy ledger's
+drift on that aggregate with the conservative part of the model (FP64 and
+mixed), the serving stream
 (Poisson arrivals of plate compression/tension tests through the
 continuous-batching server, p50/p99 completion latency), the roofline of the
 dominant kernel from HIP events on its own stream, and the CPU oracle on this
@@ -142,6 +144,39 @@ def single_aggregate(L, precision, steps, warmup):
     el = time.perf_counter() - t0
     launches, ms = s.profile_read()
     return 20000 * steps / el, x, v0, launches, ms
+
+
+def energy_drift(L, precision, steps=20000, every=1000):
+    """Energy ledger drift on the N=20,000 aggregate with the conservative
+    part of the benchmark's model: bonds and contacts with their normal
+    springs only (no damping, zero tangential, rolling and torsion stiffness)
+    and limited-range Hamaker. The reference advances the shear springs after
+    taking their force (an explicit step whose energy error is first order in
+    dt and, undamped, grows), so they are left out; what remains changes the
+    ledger total only by integrator and precision error. Returns the largest
+    |E(t) - E(0)| over samples every `every` steps, relative to the ledger's
+    scale (the sum of the magnitudes of its components at t = 0)."""
+    k = 1.0e4
+    c = [k, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0, R_PART, MASS, INERTIA, DT]
+    s = [k, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, R_PART, MASS, INERTIA, DT, 1e-3 * R_PART]
+    h = aggregate_params()[2]
+    p = L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R_PART,
+                      contact=c, sinter=s, hamaker=h, precision=precision)
+    x = L.generate_aggregate(20000, R_PART, 1.8, 1.3, 1)
+    v0 = np.random.default_rng(1).normal(0.0, 1.0, x.shape)
+    sys_ = L.System(p, x, v0)
+    e0 = sys_.energy(MASS, INERTIA)
+    scale = sum(abs(e0[t]) for t in L.ENERGY_TERMS[:7])
+    drift = 0.0
+    for _ in range(steps // every):
+        sys_.step(DT, every, rebuild_every=REBUILD)
+        d = abs(sys_.energy(MASS, INERTIA)["total"] - e0["total"])
+        drift = max(drift, d) if math.isfinite(d) else math.inf
+    e1 = sys_.energy(MASS, INERTIA)
+    sys_.close()
+    return {"steps": steps, "max_rel_drift": float(f"{drift / scale:.3e}"), "scale_J": float(f"{scale:.6e}"),
+            "start_J": {t: float(f"{e0[t]:.6e}") for t in L.ENERGY_TERMS},
+            "end_J": {t: float(f"{e1[t]:.6e}") for t in L.ENERGY_TERMS}}
 
 
 def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0, seed=7):
@@ -316,6 +351,10 @@ def main():
             "single_n20000_fp64": round(single64, 1), "single_n20000_mixed": round(single_mx, 1),
             "single_n20000_force_kernel_us": round(float(ms[0]) * 1e3 / max(int(launches[0]), 1), 2),
         })
+        extra["energy_drift_undamped_n20000"] = {
+            "note": "the aggregate above with normal springs (bonds, contacts) and limited-range Hamaker only, no "
+                    "damping; largest ledger total drift over 20,000 steps relative to the ledger scale, dt 1e-13 s",
+            "fp64": energy_drift(L, L.PRECISION_FP64), "mixed": energy_drift(L, L.PRECISION_MIXED)}
         if args.serving_requests > 0:
             extra["serving"] = serving_leg(L, args.serving_requests)
         if not args.no_cpu_baseline:

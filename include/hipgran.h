@@ -185,6 +185,19 @@ hg_status hg_get_springs(hg_system *sys, int which, uint32_t *ij, double *spring
  * |L|, Rg, centre of mass (3). */
 hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *out);
 
+/* Energy ledger in FP64 (the reference defines no potential energy; it
+ * reports kinetic energy only, test/compute_energy.cpp:7-58). out[8] =
+ * KE_trans, KE_rot, U_normal, U_tangential, U_rolling, U_torsion, U_hamaker,
+ * E_total. Pairs are the evaluated ones (the neighbour list in that mode) and
+ * each counts once: U_normal = k/2 overlap^2 for overlapping contacts and for
+ * every bond (either sign), with the contact or sinter stiffness; the springs
+ * k/2 |xi|^2 with the pair's two ordered copies averaged; U_hamaker the
+ * README.md:154-157 potential (or -2Ar/h for HG_HAMAKER_SIMPLIFIED),
+ * continued linearly below h0 and, for HG_HAMAKER_LIMITED, shifted to zero at
+ * r_cutoff, so that its derivative is the force the functors apply. Facet
+ * terms are not included. Per-particle terms are summed in particle order. */
+hg_status hg_energy(hg_system *sys, double mass, double inertia, double *out);
+
 /* The alt_sinter bond search on its own (host only, no device needed): the
  * same bonds and skip count hg_create derives for HG_BINARY_SINTER.
  * Replaces: the topology part of the alt_sinter_functor ctor
@@ -244,6 +257,8 @@ hg_status hg_pool_step(hg_pool *pool, double dt, uint64_t nsteps, uint32_t rebui
 hg_status hg_pool_set_neck_strengths(hg_pool *pool, int32_t slot, const double *strengths);
 /* Per-system copies, as the hg_system getters. */
 hg_status hg_pool_get_state(hg_pool *pool, int32_t slot, double *x, double *v, double *theta, double *omega);
+/* hg_energy for one system of the pool. */
+hg_status hg_pool_energy(hg_pool *pool, int32_t slot, double mass, double inertia, double *out);
 hg_status hg_pool_get_accelerations(hg_pool *pool, int32_t slot, double *a, double *alpha);
 hg_status hg_pool_get_neighbors(hg_pool *pool, int32_t slot, uint64_t *offsets, uint32_t *idx, size_t cap,
                                 size_t *total);

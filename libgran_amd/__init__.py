@@ -19,6 +19,9 @@ from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, HAMAKER_FULL, HAM
 
 _abi.lib()  # fail at import time when the HIP library is absent
 
+# hg_energy's terms, in its output order
+ENERGY_TERMS = ("ke_trans", "ke_rot", "u_normal", "u_tangential", "u_rolling", "u_torsion", "u_hamaker", "total")
+
 __all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
            "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
            "HAMAKER_SIMPLIFIED",
@@ -176,6 +179,13 @@ class System:
         _abi.check(_abi.lib().hg_diagnostics(self._h, mass, inertia, _abi.dptr(out)))
         return dict(ke=out[0], ke_trans=out[1], ke_rot=out[2], P=out[3], L=out[4], rg=out[5], cm=out[6:9].copy())
 
+    def energy(self, mass, inertia):
+        """FP64 energy ledger (hg_energy): kinetic, pair elastic, spring and
+        Hamaker terms and their total, in joules."""
+        out = np.zeros(8)
+        _abi.check(_abi.lib().hg_energy(self._h, mass, inertia, _abi.dptr(out)))
+        return dict(zip(ENERGY_TERMS, out))
+
     def profile(self, enable=True):
         _abi.check(_abi.lib().hg_profile_enable(self._h, 1 if enable else 0))
 
@@ -241,6 +251,12 @@ class Pool:
         n, t, k = C.c_size_t(0), C.c_double(0), C.c_uint64(0)
         _abi.check(_abi.lib().hg_pool_info(self._h, slot, C.byref(n), C.byref(t), C.byref(k)))
         return n.value, t.value, k.value
+
+    def energy(self, slot, mass, inertia):
+        """hg_energy for one system of the pool."""
+        out = np.zeros(8)
+        _abi.check(_abi.lib().hg_pool_energy(self._h, slot, mass, inertia, _abi.dptr(out)))
+        return dict(zip(ENERGY_TERMS, out))
 
     def state(self, slot):
         n = self.info(slot)[0]

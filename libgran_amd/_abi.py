@@ -95,6 +95,7 @@ SIGNATURES = {
     "hg_get_bonds": (C.c_int, [_sysp, _u32p, C.c_size_t, _szp, _szp]),
     "hg_get_springs": (C.c_int, [_sysp, C.c_int, _u32p, _dp, C.c_size_t, _szp]),
     "hg_diagnostics": (C.c_int, [_sysp, C.c_double, C.c_double, _dp]),
+    "hg_energy": (C.c_int, [_sysp, C.c_double, C.c_double, _dp]),
     "hg_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp, _szp]),
     "hg_set_neck_strengths": (C.c_int, [_sysp, _dp]),
     "hg_breaking_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp]),
@@ -110,6 +111,7 @@ SIGNATURES = {
     "hg_pool_set_facets": (C.c_int, [_sysp, C.c_int32, C.c_size_t, _dp, _dp]),
     "hg_pool_step": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
     "hg_pool_get_state": (C.c_int, [_sysp, C.c_int32, _dp, _dp, _dp, _dp]),
+    "hg_pool_energy": (C.c_int, [_sysp, C.c_int32, C.c_double, C.c_double, _dp]),
     "hg_pool_get_accelerations": (C.c_int, [_sysp, C.c_int32, _dp, _dp]),
     "hg_pool_get_neighbors": (C.c_int, [_sysp, C.c_int32, _u64p, _u32p, C.c_size_t, _szp]),
     "hg_pool_get_springs": (C.c_int, [_sysp, C.c_int32, C.c_int, _u32p, _dp, C.c_size_t, _szp]),

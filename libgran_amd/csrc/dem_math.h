@@ -172,6 +172,34 @@ HGD acc6 lr_hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
     return hamaker_eval(p, xi, xj);
 }
 
+// ---- energy ledger: Hamaker potentials (the reference defines no potential
+// energy; README.md:154-157 gives the one below, whose derivative dU/dd is
+// hamaker_scalar). Below the saturation gap h0 the functors apply a constant
+// force, so U continues linearly there, and the limited-range functor's
+// potential is shifted to zero at its cutoff: dU/dd is the applied force.
+HGD double hamaker_potential_raw(double A, double r, double d) {
+    return (-A) / 6.0 *
+           (2.0 * sq(r) / ((4.0 * r + d) * d) + 2.0 * sq(r) / sq(2.0 * r + d) + log((4.0 * r + d) * d / sq(2.0 * r + d)));
+}
+HGD double hamaker_u(int model, const hg_hamaker_params &p, double d) {
+    return model == HG_HAMAKER_SIMPLIFIED ? -2.0 * p.A * p.r_part / d : hamaker_potential_raw(p.A, p.r_part, d);
+}
+HGD double hamaker_dudd(int model, const hg_hamaker_params &p, double d) {
+    return model == HG_HAMAKER_SIMPLIFIED ? 2.0 * p.A * p.r_part / sq(d) : hamaker_scalar(p.A, p.r_part, d);
+}
+HGD double hamaker_u_sat(int model, const hg_hamaker_params &p, double d) {
+    return d < p.h0 ? hamaker_u(model, p, p.h0) + hamaker_dudd(model, p, p.h0) * (d - p.h0) : hamaker_u(model, p, d);
+}
+// Potential of one pair at centre distance `dist`.
+HGD double hamaker_potential(int model, const hg_hamaker_params &p, double dist) {
+    const double d = dist - 2.0 * p.r_part;
+    if (model == HG_HAMAKER_LIMITED) {
+        if (dist > p.r_cutoff) return 0.0;
+        return hamaker_u_sat(model, p, d) - hamaker_u_sat(model, p, p.r_cutoff - 2.0 * p.r_part);
+    }
+    return hamaker_u_sat(model, p, d);
+}
+
 // ---- simplified_hamaker_force.h:18-30: 2 A r / h^2 along x_j - x_i ----------
 HGD acc6 simplified_hamaker_eval(const hg_hamaker_params &p, d3 xi, d3 xj) {
     d3 r = xj - xi;

@@ -116,6 +116,7 @@ struct hg_system_impl {
     int *flags_host = nullptr;  // pinned
     // reductions
     double *red = nullptr;
+    double *energy = nullptr;  // per-particle ledger terms, 8 per particle (allocated on first use)
     // host copies of the topology
     std::vector<std::pair<uint32_t, uint32_t>> bonds;
     size_t bonds_skipped = 0;
@@ -161,6 +162,33 @@ hipError_t launch_zero_u32(uint32_t *p, size_t count, hipStream_t s);
 hipError_t launch_zero_f64(double *p, size_t count, hipStream_t s);
 hipError_t launch_diagnostics(const double *x, const double *v, const double *w, uint32_t n, double mass,
                               double inertia, double *out, hipStream_t s);
+
+// Inputs of the energy ledger for particles [offset, offset + n) of an arena
+// whose per-particle arrays have row stride `stride` (a single system:
+// offset 0, stride n). nb/ncnt are null in all-pairs mode; pj/ps/pcnt are the
+// current pair-state table (null without a binary functor).
+struct energy_view {
+    const double *x, *v, *w;
+    uint32_t offset, n;
+    size_t stride;
+    const uint32_t *nb, *ncnt;
+    const uint32_t *pj, *pcnt;
+    const double *ps;
+};
+// Eight terms per particle into out[8*k + c] (k = local index): kinetic
+// translational and rotational; the particle's half of its pairs' normal
+// elastic energy; its half of the tangential, rolling and torsion spring
+// energy; its half of the pairs' Hamaker potential; 0.
+hipError_t launch_energy(const hg_params &p, const energy_view &E, double mass, double inertia, double *out,
+                         hipStream_t s);
+// Sums the per-particle ledger terms in particle order: out[0..6] the seven
+// components, out[7] their total.
+inline void sum_energy(const double *terms, size_t n, double *out) {
+    for (int c = 0; c < 8; c++) out[c] = 0.0;
+    for (size_t i = 0; i < n; i++)
+        for (int c = 0; c < 7; c++) out[c] += terms[8 * i + c];
+    for (int c = 0; c < 7; c++) out[7] += out[c];
+}
 
 // host-side set-up (topology.cpp)
 // prevent_cycles: alt_sinter's spanning forest; false: alt_breaking_sinter's

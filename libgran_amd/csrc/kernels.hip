@@ -388,6 +388,55 @@ __global__ void __launch_bounds__(1024) k_diag(const double *x, const double *v,
     }
 }
 
+// Energy ledger terms of one particle (see launch_energy). Pair terms are
+// split evenly between the two particles of a pair and each ordered pair's
+// springs carry half of that pair's spring energy, so the sums over all
+// particles count every pair once.
+__global__ void __launch_bounds__(256) k_energy(hg_params P, energy_view E, double mass, double inertia,
+                                                double *out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E.n) return;
+    const uint32_t g = E.offset + k;
+    const d3 xi = ld3(E.x, g), vi = ld3(E.v, g), wi = ld3(E.w, g);
+    double e[8] = {0.5 * mass * dot(vi, vi), 0.5 * inertia * dot(wi, wi), 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint32_t cnt = E.pcnt ? E.pcnt[g] : 0u;
+    for (uint32_t s = 0; s < cnt; s++) {
+        const bool bonded = (E.pj[size_t(s) * E.stride + g] & kBondBit) != 0;
+        const spring9 sp = ld_rec(E.ps + (size_t(s) * E.stride + g) * kSpringRec);
+        const double kt = bonded ? P.sinter.k_t : P.contact.k_t, kr = bonded ? P.sinter.k_r : P.contact.k_r,
+                     ko = bonded ? P.sinter.k_o : P.contact.k_o;
+        e[3] += 0.25 * kt * dot(sp.s[0], sp.s[0]);
+        e[4] += 0.25 * kr * dot(sp.s[1], sp.s[1]);
+        e[5] += 0.25 * ko * dot(sp.s[2], sp.s[2]);
+    }
+    auto pair = [&](uint32_t j) {
+        const double dist = norm(xi - ld3(E.x, j));
+        if (P.binary_model != HG_BINARY_NONE) {
+            bool bonded = false;
+            for (uint32_t s = 0; s < cnt; s++) {
+                const uint32_t tag = E.pj[size_t(s) * E.stride + g];
+                if ((tag & kIdxMask) == j) bonded = (tag & kBondBit) != 0;
+            }
+            if (bonded) {
+                const double ov = 2.0 * P.sinter.r_part - dist;
+                e[2] += 0.25 * P.sinter.k * ov * ov;
+            } else {
+                const double ov = 2.0 * P.contact.r_part - dist;
+                if (ov > 0.0) e[2] += 0.25 * P.contact.k * ov * ov;
+            }
+        }
+        if (P.hamaker_model != HG_HAMAKER_NONE) e[6] += 0.5 * hamaker_potential(P.hamaker_model, P.hamaker, dist);
+    };
+    if (E.nb) {
+        const uint32_t m = E.ncnt[g];
+        for (uint32_t q = 0; q < m; q++) pair(E.nb[size_t(q) * E.stride + g]);
+    } else {
+        for (uint32_t j = E.offset; j < E.offset + E.n; j++)
+            if (j != g) pair(j);
+    }
+    for (int c = 0; c < 8; c++) out[8 * size_t(k) + c] = e[c];
+}
+
 inline unsigned blocks_for(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
 template <int BIN, int HAM>
@@ -492,6 +541,13 @@ hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s) {
 hipError_t launch_diagnostics(const double *x, const double *v, const double *w, uint32_t n, double mass,
                               double inertia, double *out, hipStream_t s) {
     k_diag<<<1, 1024, 0, s>>>(x, v, w, n, mass, inertia, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const hg_params &p, const energy_view &E, double mass, double inertia, double *out,
+                         hipStream_t s) {
+    if (E.n == 0) return hipSuccess;
+    k_energy<<<blocks_for(E.n, 256), 256, 0, s>>>(p, E, mass, inertia, out);
     return hipGetLastError();
 }
 

@@ -1434,6 +1434,38 @@ hg_status hg_pool_get_state(hg_pool *Q, int32_t slot, double *x, double *v, doub
     return HG_OK;
 }
 
+hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, double *out) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!out) return fail(HG_ERR_INVALID, "hg_pool_energy: out is required");
+    const pool_meta &M = Q->meta[slot];
+    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, tab = Q->cap_s != 0;
+    if (nl && M.rebuild)
+        return fail(HG_ERR_INVALID, "hg_pool_energy: the system is listed by its first step; step it first");
+    double *terms = nullptr;
+    HG_HIP(hipMalloc(reinterpret_cast<void **>(&terms), size_t(8) * M.n * sizeof(double)));
+    energy_view E{Q->x,
+                  Q->v,
+                  Q->w,
+                  M.offset,
+                  M.n,
+                  Q->P,
+                  nl ? Q->nb : nullptr,
+                  nl ? Q->ncnt : nullptr,
+                  tab ? Q->pj[M.cur] : nullptr,
+                  tab ? Q->pcnt[M.cur] : nullptr,
+                  tab ? Q->ps[M.cur] : nullptr};
+    std::vector<double> h(size_t(8) * M.n);
+    hipError_t e = launch_energy(Q->p, E, mass, inertia, terms, Q->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), terms, h.size() * sizeof(double), hipMemcpyDeviceToHost, Q->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(Q->stream);
+    const hipError_t ef = hipFree(terms);
+    HG_HIP(e);
+    HG_HIP(ef);
+    sum_energy(h.data(), M.n, out);
+    return HG_OK;
+}
+
 hg_status hg_pool_get_accelerations(hg_pool *Q, int32_t slot, double *a, double *alpha) {
     PG_TRY(valid_slot(Q, slot));
     const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;

@@ -649,4 +649,27 @@ hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out)
     return HG_OK;
 }
 
+hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
+    if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    if (!S->energy) HG_TRY(dalloc(S, &S->energy, size_t(8) * S->n));
+    const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, bin = S->p.binary_model != HG_BINARY_NONE;
+    energy_view E{S->x,
+                  S->v,
+                  S->w,
+                  0u,
+                  uint32_t(S->n),
+                  S->n,
+                  nl ? S->nb : nullptr,
+                  nl ? S->ncnt : nullptr,
+                  bin ? S->pj[S->cur] : nullptr,
+                  bin ? S->pcnt[S->cur] : nullptr,
+                  bin ? S->ps[S->cur] : nullptr};
+    HG_HIP(launch_energy(S->p, E, mass, inertia, S->energy, S->stream));
+    std::vector<double> h(size_t(8) * S->n);
+    HG_HIP(hipMemcpyAsync(h.data(), S->energy, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    sum_energy(h.data(), S->n, out);
+    return HG_OK;
+}
+
 }  // extern "C"

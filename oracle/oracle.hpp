@@ -283,6 +283,34 @@ inline acc2 simplified_hamaker_eval(const hamaker_params &p, v3 xi_, v3 xj) {
     return {f / p.mass, vzero};
 }
 
+// Energy ledger, Hamaker part (the reference defines no potential energy).
+// README.md:154-157 gives U(d) for two spheres at surface gap d; its
+// derivative dU/dd is hamaker_scalar. Below h0 the functors apply the force at
+// h0, so U continues linearly; the limited-range potential is shifted to zero
+// at r_cutoff. The simplified functor's potential is -2 A r / d.
+inline double hamaker_u(int model, const hamaker_params &p, double d) {
+    if (model == HAM_SIMPLIFIED) return -2.0 * p.A * p.r_part / d;
+    const double r = p.r_part;
+    return (-p.A) / 6.0 *
+           (2.0 * square(r) / ((4.0 * r + d) * d) + 2.0 * square(r) / square(2.0 * r + d) +
+            std::log((4.0 * r + d) * d / square(2.0 * r + d)));
+}
+inline double hamaker_dudd(int model, const hamaker_params &p, double d) {
+    if (model == HAM_SIMPLIFIED) return 2.0 * p.A * p.r_part / square(d);
+    return hamaker_scalar(p.A, p.r_part, d);
+}
+inline double hamaker_u_sat(int model, const hamaker_params &p, double d) {
+    return d < p.h0 ? hamaker_u(model, p, p.h0) + hamaker_dudd(model, p, p.h0) * (d - p.h0) : hamaker_u(model, p, d);
+}
+inline double hamaker_potential(int model, const hamaker_params &p, double dist) {
+    const double d = dist - 2.0 * p.r_part;
+    if (model == HAM_LIMITED) {
+        if (dist > p.r_cutoff) return 0.0;
+        return hamaker_u_sat(model, p, d) - hamaker_u_sat(model, p, p.r_cutoff - 2.0 * p.r_part);
+    }
+    return hamaker_u_sat(model, p, d);
+}
+
 // the second functor of the binary container, by model
 inline acc2 binary_hamaker_eval(int model, const hamaker_params &p, v3 xi_, v3 xj) {
     if (model == HAM_FULL) return hamaker_eval(p, xi_, xj);
@@ -506,6 +534,50 @@ public:
         }
         if (have_b && have_h) return b + h;
         return have_b ? b : h;
+    }
+
+    // Energy ledger terms of every particle, out[8*i + c]: kinetic translational
+    // and rotational; the particle's half of its evaluated pairs' normal
+    // elastic energy (k/2 overlap^2 for overlapping contacts and for bonds of
+    // either sign); its half of its pairs' tangential, rolling and torsion
+    // spring energy (each ordered pair's springs k/2 |xi|^2, halved); its half
+    // of the pairs' Hamaker potential; 0. The builder's own ledger: the
+    // reference reports kinetic energy only (test/compute_energy.cpp:7-58).
+    void energy_terms(double mass, double inertia, double *out) const {
+        for (size_t i = 0; i < n; i++) {
+            double e[8] = {0.5 * mass * dot(v[i], v[i]), 0.5 * inertia * dot(w[i], w[i]), 0, 0, 0, 0, 0, 0};
+            auto springs = [&](const spring_rows &rows, double kt, double kr, double ko) {
+                for (const auto &en : rows.rows[i]) {
+                    e[3] += 0.25 * kt * dot(en.second.s[0], en.second.s[0]);
+                    e[4] += 0.25 * kr * dot(en.second.s[1], en.second.s[1]);
+                    e[5] += 0.25 * ko * dot(en.second.s[2], en.second.s[2]);
+                }
+            };
+            if (p.binary_model != BIN_NONE) {
+                springs(contact_springs, p.contact.k_t, p.contact.k_r, p.contact.k_o);
+                springs(sinter_springs, p.sinter.k_t, p.sinter.k_r, p.sinter.k_o);
+            }
+            auto pair = [&](uint32_t j) {
+                const double dist = norm(x[i] - x[j]);
+                if (p.binary_model != BIN_NONE) {
+                    if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
+                        const double ov = 2.0 * p.sinter.r_part - dist;
+                        e[2] += 0.25 * p.sinter.k * ov * ov;
+                    } else {
+                        const double ov = 2.0 * p.contact.r_part - dist;
+                        if (ov > 0.0) e[2] += 0.25 * p.contact.k * ov * ov;
+                    }
+                }
+                if (p.hamaker_model != HAM_NONE) e[6] += 0.5 * hamaker_potential(p.hamaker_model, p.hamaker, dist);
+            };
+            if (p.neighbor_mode == 1) {
+                for (uint32_t j : nbr[i]) pair(j);
+            } else {
+                for (uint32_t j = 0; j < uint32_t(n); j++)
+                    if (j != i) pair(j);
+            }
+            for (int c = 0; c < 8; c++) out[8 * i + c] = e[c];
+        }
     }
 
     // The fold of one ordered pair on copies of its springs (no state change):

@@ -326,6 +326,11 @@ size_t or_sys_get_springs(void *s, int which, uint32_t *ij, double *springs, siz
     return tot;
 }
 
+// Energy ledger terms, 8 per particle (dem_system::energy_terms).
+void or_sys_energy_terms(void *s, double mass, double inertia, double *out) {
+    static_cast<dem_system *>(s)->energy_terms(mass, inertia, out);
+}
+
 // Per particle: sum over evaluated partners and over the functors of each
 // pair of |functor acceleration| (linear, angular), without changing any
 // spring: the scale of the terms a rounded fold adds, including terms that

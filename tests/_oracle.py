@@ -135,6 +135,7 @@ def lib():
         L.or_sys_diagnostics.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
         L.or_sys_pair_magnitudes.argtypes = [C.c_void_p, _dp]
         L.or_sys_functor_magnitudes.argtypes = [C.c_void_p, _dp]
+        L.or_sys_energy_terms.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp]
         L.or_contact_seq.argtypes = [_dp, C.c_size_t, _dp, _dp, _dp]
         L.or_sinter_seq.argtypes = [_dp, _dp, C.c_size_t, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.or_hamaker_seq.argtypes = [C.c_int, _dp, C.c_size_t, _dp, _dp]
@@ -388,6 +389,21 @@ class OracleSystem:
         |a| and |alpha| (no cancellation inside a pair) at the current state."""
         out = np.zeros((self.n, 2))
         lib().or_sys_functor_magnitudes(self._h, dptr(out))
+        return out
+
+    def energy_terms(self, mass=MASS, inertia=INERTIA):
+        """(n, 8) per-particle energy ledger terms (dem_system::energy_terms)."""
+        out = np.zeros((self.n, 8))
+        lib().or_sys_energy_terms(self._h, mass, inertia, dptr(out))
+        return out
+
+    def energy(self, mass=MASS, inertia=INERTIA):
+        """The ledger's seven components summed in particle order, and their total."""
+        t = self.energy_terms(mass, inertia)
+        out = np.zeros(8)
+        for i in range(self.n):
+            out[:7] += t[i, :7]
+        out[7] = sum(out[:7])
         return out
 
     def diagnostics(self, mass=MASS, inertia=INERTIA):
