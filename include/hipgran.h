@@ -34,7 +34,8 @@ typedef enum hg_status {
     HG_ERR_HIP = 3,         /* a HIP runtime call failed */
     HG_ERR_NUMERIC = 4,     /* NaN/Inf reached the particle state */
     HG_ERR_CAPACITY = 5,    /* a neighbour or pair-state table overflowed */
-    HG_ERR_NO_DEVICE = 6    /* no gfx950 device visible */
+    HG_ERR_NO_DEVICE = 6,   /* no gfx950 device visible */
+    HG_ERR_IO = 7           /* a file could not be created or written */
 } hg_status;
 
 /* contact_force_functor ctor arguments, same order and meaning
@@ -379,6 +380,23 @@ hg_status hg_generate_aggregate(size_t n, double r_part, double df, double kf, u
  * hg_generate_aggregate with the same arguments. */
 hg_status hg_generate_aggregates(size_t count, const uint32_t *n, const double *df, double kf, const uint64_t *seeds,
                                  double r_part, double *x_out, int threads);
+
+/* Legacy-VTK ASCII POLYDATA of n particles at path: positions divided by
+ * r_part, and fields thetax, thetay, thetaz holding each particle's body x,
+ * y and z axes for the orientation theta (rotations about the fixed x, y, z
+ * axes, composed left to right). Numbers as an std::ostream prints doubles
+ * (printf "%g"), the reference's byte layout. Replaces write_particles
+ * (writer.cpp:51-118), which names the file itself (dir/particles_<k>.vtk);
+ * here the caller passes the whole path. HG_ERR_IO if it cannot be written. */
+hg_status hg_write_vtk_particles(const char *path, size_t n, const double *x, const double *theta, double r_part);
+/* Legacy-VTK ASCII POLYDATA of sinter necks at path: for each of the nbonds
+ * pairs (i, j) = (pairs[2b], pairs[2b+1]), in the given order, the midpoint
+ * of x_i and x_j divided by r_part and the field `normals` holding
+ * (x_i - x_j) / |x_i - x_j|. Replaces write_necks (writer.cpp:120-172), which
+ * takes the n*n enabled_contacts matrix and lists its pairs i < j row by
+ * row; hg_get_bonds returns the alt_sinter bonds in that order. */
+hg_status hg_write_vtk_necks(const char *path, size_t n, const double *x, size_t nbonds, const uint32_t *pairs,
+                             double r_part);
 
 #ifdef __cplusplus
 }

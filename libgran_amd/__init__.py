@@ -9,6 +9,7 @@ thin hosts over that ABI. Importing this module loads the library and fails
 loudly when it is missing.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -24,7 +25,7 @@ ENERGY_TERMS = ("ke_trans", "ke_rot", "u_normal", "u_tangential", "u_rolling", "
 
 __all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
            "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
-           "HAMAKER_SIMPLIFIED",
+           "HAMAKER_SIMPLIFIED", "write_vtk_particles", "write_vtk_necks",
            "PAIRS_ALL", "PAIRS_NEIGHBOR_LIST", "PRECISION_FP64", "PRECISION_MIXED"]
 
 
@@ -64,6 +65,25 @@ def generate_aggregates(sizes, dfs, kf, seeds, r_part, threads=0):
     _abi.check(_abi.lib().hg_generate_aggregates(len(sizes), _abi.u32ptr(sizes), _abi.dptr(dfs), kf,
                                                  seeds.ctypes.data_as(_abi._u64p), r_part, _abi.dptr(x), threads))
     return np.split(x, np.cumsum(sizes)[:-1])
+
+
+def write_vtk_particles(path, x, theta, r_part):
+    """Legacy-VTK particle file (hg_write_vtk_particles): positions / r_part
+    and the body axes of orientation theta, as writer.cpp:51-118 writes it."""
+    x, theta = _abi.c64(x), _abi.c64(theta)
+    if x.shape != theta.shape:
+        raise ValueError("x and theta must have the same shape")
+    _abi.check(_abi.lib().hg_write_vtk_particles(os.fsencode(path), x.shape[0], _abi.dptr(x), _abi.dptr(theta),
+                                                 r_part))
+
+
+def write_vtk_necks(path, x, pairs, r_part):
+    """Legacy-VTK neck file (hg_write_vtk_necks): midpoints / r_part and
+    normals of the bonded pairs, in the given order (writer.cpp:120-172)."""
+    x = _abi.c64(x)
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    _abi.check(_abi.lib().hg_write_vtk_necks(os.fsencode(path), x.shape[0], _abi.dptr(x), pairs.shape[0],
+                                             _abi.u32ptr(pairs), r_part))
 
 
 def sinter_topology(x0, r_part, d_crit):

@@ -25,6 +25,7 @@
 #include <libgran/sinter_bridge/alt_sinter_bridge.h>
 #include <libgran/surface_force/triangular_facet.h>
 #include <libgran/vec3.h>
+#include <writer.h>
 
 using V3 = libgran_amd::vec3;
 using VV = std::vector<V3>;
@@ -174,6 +175,14 @@ int run_alt_sintered(size_t steps, const char *out) {
         std::fprintf(stderr, "bonded springs read back as zero\n");
         return 5;
     }
+    // the final state and the necks as VTK, as the reference drivers write
+    // them (main.cpp:150), beside the output file
+    const std::string path(out);
+    const std::string dir = path.rfind('/') == std::string::npos ? "." : path.substr(0, path.rfind('/'));
+    write_particles(dir, system.get_x(), system.get_theta(), r_part);
+    std::vector<bool> enabled(x0.size() * x0.size());
+    for (size_t q = 0; q < enabled.size(); q++) enabled[q] = !sinter.bonded_contacts[q];
+    write_necks(dir, system.get_x(), r_part, enabled);
     finish(system, out);
     return 0;
 }

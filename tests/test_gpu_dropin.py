@@ -37,6 +37,14 @@ def run_driver(tmp_path, scenario, steps):
     return dict(zip(("x", "v", "theta", "omega"), vals))
 
 
+def vtk_oracle():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("vtk_oracle", os.path.join(ROOT, "oracle", "vtk_writer.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def compare(osys, got, what):
     ref = osys.state()
     for k in ("x", "v", "theta", "omega"):
@@ -102,6 +110,14 @@ def test_dropin_alt_sintered(tmp_path):
                sinter=O.sinter_params(k_bond=1000.0, d_crit=1.0e-9, dt=5e-14))
     o.step(5e-14, steps)
     compare(o, got, "alt_sintered")
+    # the driver wrote the final state and the bonds' necks through the
+    # drop-in writer.h (writer.cpp:51-172): the oracle's text for that state
+    V = vtk_oracle()
+    xs, ths = got["x"].tolist(), got["theta"].tolist()
+    assert (tmp_path / "particles_1.vtk").read_text() == V.particles_text(xs, ths, R)
+    bonds = [tuple(map(int, p)) for p in o.bonds()[0]]
+    assert len(bonds) == 26
+    assert (tmp_path / "necks_1.vtk").read_text() == V.necks_text(xs, bonds, R)
 
 
 def test_dropin_alt_breaking_sintered(tmp_path):
