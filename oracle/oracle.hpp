@@ -16,6 +16,9 @@
 //   sinter_bonded_eval       include/libgran/sinter_bridge/alt_sinter_bridge.h:92-129
 //   shear_sinter             include/libgran/sinter_bridge/alt_sinter_bridge.h:137-156
 //   sinter_topology          include/libgran/sinter_bridge/alt_sinter_bridge.h:19-90
+//   legacy_sinter_frames     include/libgran/sinter_bridge/sinter_bridge.h:138-211
+//   legacy_bonded_eval       include/libgran/sinter_bridge/sinter_bridge.h:235-259
+//   legacy connector moves   include/libgran/sinter_bridge/sinter_bridge.h:24-75
 //   hamaker_eval             include/libgran/hamaker_force/hamaker_force.h:28-43
 //   lr_hamaker_eval          include/libgran/hamaker_force/limited_range_hamaker_force.h:29-47
 //   simplified_hamaker_eval  include/libgran/hamaker_force/simplified_hamaker_force.h:18-30
@@ -42,6 +45,10 @@
 //     a(t0) is computed lazily by the first step, with whatever neighbour list
 //     exists at that moment.
 //   * neighbour list: j != i with norm(x_i - x_j) < r_verlet, ascending j.
+//   * sinter_step_handler: within one particle's drift, increment_x (the
+//     connectors follow dx) comes before increment_theta (they turn about the
+//     new centre); the two orders are the same rotation and differ only in
+//     rounding.
 //   * the alt_sinter bond predicate uses fabs (the author's intent); the
 //     reference's unqualified abs() can bind to abs(int) on some toolchains.
 //   * spring tables are sparse rows keyed by j; an absent entry is the zero
@@ -101,7 +108,7 @@ struct hamaker_params {
     double A, h0, r_cutoff, r_part, mass;
 };
 
-enum { BIN_NONE = 0, BIN_CONTACT = 1, BIN_SINTER = 2 };
+enum { BIN_NONE = 0, BIN_CONTACT = 1, BIN_SINTER = 2, BIN_SINTER_LEGACY = 3 };
 enum { HAM_NONE = 0, HAM_FULL = 1, HAM_LIMITED = 2, HAM_SIMPLIFIED = 3 };
 
 struct spring3 {
@@ -252,6 +259,91 @@ inline void sinter_topology(size_t n, const std::vector<v3> &x0, double r_part, 
 }
 
 // --------------------------------------------------------------- hamaker ----
+// ------------------------------------------------- legacy sinter_functor --
+// Spring connectors of one bond: the five points attached to each of its two
+// particles, moved with that particle by the step handler.
+struct legacy_bond {
+    uint32_t i, j;
+    v3 ci[5], cj[5];
+};
+
+// The constructor's bond search and connector frames
+// (sinter_bridge.h:146-211): pairs i < j in row order whose gap
+// |x_j - x_i| - 2r has magnitude at most critical_separation (fabs, as for
+// alt_sinter), each given a frame (ds, db, n) from the caller's random unit
+// vectors, drawn until one is not collinear with the normal.
+template <class NextVec>
+inline void legacy_sinter_frames(size_t n, const std::vector<v3> &x0, double r_part, double d_crit, NextVec next,
+                                 std::vector<legacy_bond> &out) {
+    out.clear();
+    for (size_t i = 0; i + 1 < n; i++) {
+        for (size_t j = i + 1; j < n; j++) {
+            const v3 pi = x0[i], pj = x0[j];
+            double separation = norm(pj - pi) - 2.0 * r_part;
+            if (std::fabs(separation) > d_crit) continue;
+            v3 n_i = normalized(pj - pi);
+            v3 n_j = -n_i;
+            v3 v, db;
+            do {
+                v = next();
+                db = cross(v, n_j);
+            } while (norm(db) == 0.0);
+            v3 ds = v - dot(v, n_i) * n_i;
+            db = normalized(db);
+            ds = normalized(ds);
+            legacy_bond b;
+            b.i = uint32_t(i);
+            b.j = uint32_t(j);
+            b.ci[0] = pi + ds * r_part;
+            b.ci[1] = pi - ds * r_part;
+            b.ci[2] = pi + db * r_part;
+            b.ci[3] = pi - db * r_part;
+            b.ci[4] = pi + n_i * r_part;
+            b.cj[0] = pj + ds * r_part;
+            b.cj[1] = pj - ds * r_part;
+            b.cj[2] = pj + db * r_part;
+            b.cj[3] = pj - db * r_part;
+            b.cj[4] = pj + n_j * r_part;
+            out.push_back(b);
+        }
+    }
+}
+
+// Bonded pair (i, j) of sinter_functor (sinter_bridge.h:235-259): five
+// springs between matching connectors, rest length 2r for the four side
+// springs and 0 for the one at the contact point; ci are i's connectors of
+// the bond, cj j's.
+inline acc2 legacy_bonded_eval(double k_d, double gamma_d, double r_part, double mass, double inertia,
+                               const v3 *ci, const v3 *cj, v3 xi_, v3 xj, v3 vi, v3 vj, v3 wi, v3 wj) {
+    v3 force = vzero, torque = vzero;
+    for (int q = 0; q < 5; q++) {
+        v3 spring = cj[q] - ci[q];
+        double spring_length = norm(spring);
+        double equilibrium_length = q < 4 ? 2.0 * r_part : 0.0;
+        v3 conn_i_v = vi + cross(wi, ci[q] - xi_);
+        v3 conn_j_v = vj + cross(wj, cj[q] - xj);
+        v3 rel_v = conn_j_v - conn_i_v;
+        v3 dforce = -(equilibrium_length - spring_length) * k_d * normalized(spring) + gamma_d * rel_v;
+        force = force + dforce;
+        v3 arm = ci[q] - xi_;
+        torque = torque + cross(arm, dforce);
+    }
+    return {force / mass, torque / inertia};
+}
+
+// sinter_step_handler (sinter_bridge.h:24-42, 55-75): a particle's connectors
+// follow its position increment dx, then turn with its orientation increment
+// dtheta about its (new) centre.
+inline void legacy_move_x(v3 *c, v3 dx) {
+    for (int q = 0; q < 5; q++) c[q] = c[q] + dx;
+}
+inline void legacy_move_theta(v3 *c, v3 x, v3 dtheta) {
+    for (int q = 0; q < 5; q++) {
+        v3 spring_vector = c[q] - x;
+        c[q] = c[q] + cross(dtheta, spring_vector);
+    }
+}
+
 inline double hamaker_scalar(double A, double rp, double h) {
     return (-A) / 6.0 *
            ((((4.0 * rp + 2.0 * h) / (4.0 * rp + h) / h - 2.0 / (2.0 * rp + h)) -
@@ -469,6 +561,31 @@ public:
         if (p.neighbor_mode == 1) rebuild_neighbors(true);
     }
 
+    // sinter_functor (BIN_SINTER_LEGACY): bonds with their connectors, and per
+    // particle its (partner, bond index) pairs sorted by partner. Set by
+    // set_legacy before the first step.
+    std::vector<legacy_bond> legacy;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> legacy_of;
+    void set_legacy(const std::vector<legacy_bond> &b) {
+        legacy = b;
+        legacy_of.assign(n, {});
+        bonds.clear();
+        for (size_t k = 0; k < b.size(); k++) {
+            legacy_of[b[k].i].push_back({b[k].j, uint32_t(k)});
+            legacy_of[b[k].j].push_back({b[k].i, uint32_t(k)});
+            bonds.push_back({b[k].i, b[k].j});
+        }
+        for (auto &r : legacy_of) std::sort(r.begin(), r.end());
+    }
+    // i's connectors of bond k
+    v3 *legacy_side(size_t i, uint32_t k) { return legacy[k].i == i ? legacy[k].ci : legacy[k].cj; }
+    int64_t legacy_bond_of(size_t i, uint32_t j) const {
+        if (legacy_of.empty()) return -1;
+        const auto &r = legacy_of[i];
+        auto it = std::lower_bound(r.begin(), r.end(), std::make_pair(j, uint32_t(0)));
+        return (it != r.end() && it->first == j) ? int64_t(it->second) : -1;
+    }
+
     bool is_bonded(size_t i, uint32_t j) const {
         const auto &r = bonded_to[i];
         return std::binary_search(r.begin(), r.end(), j);
@@ -499,7 +616,13 @@ public:
     acc2 pair(size_t i, uint32_t j) {
         bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
         acc2 b{vzero, vzero}, h{vzero, vzero};
-        if (have_b) {
+        int64_t lb = p.binary_model == BIN_SINTER_LEGACY ? legacy_bond_of(i, j) : -1;
+        if (have_b && lb >= 0) {
+            // sinter_bridge.h:222-224
+            b = legacy_bonded_eval(p.sinter.k, p.sinter.gamma_n, p.contact.r_part, p.contact.mass, p.contact.inertia,
+                                   legacy_side(i, uint32_t(lb)), legacy_side(j, uint32_t(lb)), x[i], x[j], v[i], v[j],
+                                   w[i], w[j]);
+        } else if (have_b) {
             if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
                 v3 ftot;
                 b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], sinter_springs.at(i, j), &ftot);
@@ -730,8 +853,13 @@ public:
             w[i] = w[i] + al[i] * half;
         }
         for (size_t i = 0; i < n; i++) {
-            x[i] = x[i] + v[i] * dt;
-            th[i] = th[i] + w[i] * dt;
+            const v3 dx = v[i] * dt, dth = w[i] * dt;
+            x[i] = x[i] + dx;
+            if (!legacy_of.empty())
+                for (auto &e : legacy_of[i]) legacy_move_x(legacy_side(i, e.second), dx);
+            th[i] = th[i] + dth;
+            if (!legacy_of.empty())
+                for (auto &e : legacy_of[i]) legacy_move_theta(legacy_side(i, e.second), x[i], dth);
         }
         t = t + dt;
         compute_accelerations();

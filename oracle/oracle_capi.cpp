@@ -326,6 +326,137 @@ size_t or_sys_get_springs(void *s, int which, uint32_t *ij, double *springs, siz
     return tot;
 }
 
+// sinter_functor's constructor (sinter_bridge.h:146-211) on positions x0:
+// the random unit vectors come from `vecs` (nvecs draws of 3 doubles) in
+// order. Returns the bond count; pairs (i < j) and connectors (30 doubles per
+// bond: i's five points, then j's) are written when cap allows. *used gets
+// the number of draws taken, or nvecs + 1 when they ran out (the frames are
+// then not valid).
+size_t or_legacy_frames(size_t n, const double *x0, double r_part, double d_crit, const double *vecs, size_t nvecs,
+                        uint32_t *pairs, double *conn, size_t cap, size_t *used) {
+    std::vector<v3> x(n);
+    for (size_t i = 0; i < n; i++) x[i] = ld(x0 + 3 * i);
+    size_t k = 0;
+    bool out_of_draws = false;
+    auto next = [&]() -> v3 {
+        if (k >= nvecs) {
+            out_of_draws = true;
+            return v3{1.0, 0.0, 0.0};
+        }
+        v3 r = ld(vecs + 3 * k);
+        k++;
+        return r;
+    };
+    std::vector<legacy_bond> b;
+    legacy_sinter_frames(n, x, r_part, d_crit, next, b);
+    *used = out_of_draws ? nvecs + 1 : k;
+    for (size_t q = 0; q < b.size() && q < cap; q++) {
+        pairs[2 * q] = b[q].i;
+        pairs[2 * q + 1] = b[q].j;
+        for (int c = 0; c < 5; c++) {
+            st(conn + 30 * q + 3 * c, b[q].ci[c]);
+            st(conn + 30 * q + 15 + 3 * c, b[q].cj[c]);
+        }
+    }
+    return b.size();
+}
+
+// Gives a BIN_SINTER_LEGACY system its bonds and connectors (the layout of
+// or_legacy_frames).
+void or_sys_legacy_set(void *s, size_t nb, const uint32_t *pairs, const double *conn) {
+    auto *S = static_cast<dem_system *>(s);
+    std::vector<legacy_bond> b(nb);
+    for (size_t q = 0; q < nb; q++) {
+        b[q].i = pairs[2 * q];
+        b[q].j = pairs[2 * q + 1];
+        for (int c = 0; c < 5; c++) {
+            b[q].ci[c] = ld(conn + 30 * q + 3 * c);
+            b[q].cj[c] = ld(conn + 30 * q + 15 + 3 * c);
+        }
+    }
+    S->set_legacy(b);
+}
+
+// The current connectors of every bond, in the order they were set.
+void or_sys_legacy_get(void *s, double *conn) {
+    auto *S = static_cast<dem_system *>(s);
+    for (size_t q = 0; q < S->legacy.size(); q++)
+        for (int c = 0; c < 5; c++) {
+            st(conn + 30 * q + 3 * c, S->legacy[q].ci[c]);
+            st(conn + 30 * q + 15 + 3 * c, S->legacy[q].cj[c]);
+        }
+}
+
+// sinter_functor at functor level, as oracle/ref_harness/ref_legacy.cpp
+// drives the reference: frames from x0 and `vecs`; then per step, for each
+// particle p in order, its increments (12 doubles per particle per step: dx,
+// dtheta, and the v, omega the pairs see) through the step handler
+// (increment_x, then increment_theta), and every ordered pair (i, j), i != j,
+// evaluated: out[((k*n + i)*n + j)*6] = linear, angular acceleration (zero
+// for i == j). *nbonds and *used as or_legacy_frames.
+void or_legacy_seq(size_t n, const double *x0, const double *th0, const double *s13, const double *c18,
+                   const double *vecs, size_t nvecs, size_t nsteps, const double *incs, double *out, size_t *nbonds,
+                   size_t *used) {
+    sinter_params sp = sp_from(s13);
+    contact_params cp = cp_from(c18);
+    std::vector<v3> x(n), th(n), v(n), w(n);
+    for (size_t i = 0; i < n; i++) {
+        x[i] = ld(x0 + 3 * i);
+        th[i] = ld(th0 + 3 * i);
+    }
+    size_t k = 0;
+    bool out_of_draws = false;
+    auto next = [&]() -> v3 {
+        if (k >= nvecs) {
+            out_of_draws = true;
+            return v3{1.0, 0.0, 0.0};
+        }
+        return ld(vecs + 3 * (k++));
+    };
+    std::vector<legacy_bond> b;
+    legacy_sinter_frames(n, x, cp.r_part, sp.d_crit, next, b);
+    *nbonds = b.size();
+    *used = out_of_draws ? nvecs + 1 : k;
+    std::vector<int64_t> bond_of(n * n, -1);
+    for (size_t q = 0; q < b.size(); q++) {
+        bond_of[b[q].i * n + b[q].j] = int64_t(q);
+        bond_of[b[q].j * n + b[q].i] = int64_t(q);
+    }
+    auto side = [&](size_t i, size_t q) { return b[q].i == i ? b[q].ci : b[q].cj; };
+    std::vector<spring3> springs(n * n, spring3{{vzero, vzero, vzero}});
+    for (size_t step = 0; step < nsteps; step++) {
+        for (size_t p = 0; p < n; p++) {
+            const double *in = incs + 12 * (step * n + p);
+            v3 dx = ld(in), dth = ld(in + 3);
+            x[p] = x[p] + dx;
+            for (size_t q = 0; q < b.size(); q++)
+                if (b[q].i == p || b[q].j == p) legacy_move_x(side(p, q), dx);
+            th[p] = th[p] + dth;
+            for (size_t q = 0; q < b.size(); q++)
+                if (b[q].i == p || b[q].j == p) legacy_move_theta(side(p, q), x[p], dth);
+            v[p] = ld(in + 6);
+            w[p] = ld(in + 9);
+        }
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = 0; j < n; j++) {
+                double *o = out + ((step * n + i) * n + j) * 6;
+                acc2 e{vzero, vzero};
+                if (i != j) {
+                    const int64_t q = bond_of[i * n + j];
+                    if (q >= 0) {
+                        e = legacy_bonded_eval(sp.k, sp.gamma_n, cp.r_part, cp.mass, cp.inertia, side(i, size_t(q)),
+                                               side(j, size_t(q)), x[i], x[j], v[i], v[j], w[i], w[j]);
+                    } else {
+                        bool r;
+                        e = contact_eval(cp, x[i], x[j], v[i], v[j], w[i], w[j], springs[i * n + j], r);
+                    }
+                }
+                st(o, e.lin);
+                st(o + 3, e.ang);
+            }
+    }
+}
+
 // Energy ledger terms, 8 per particle (dem_system::energy_terms).
 void or_sys_energy_terms(void *s, double mass, double inertia, double *out) {
     static_cast<dem_system *>(s)->energy_terms(mass, inertia, out);

@@ -85,7 +85,7 @@ def _fill(struct, arr):
     return struct
 
 
-BIN_NONE, BIN_CONTACT, BIN_SINTER = 0, 1, 2
+BIN_NONE, BIN_CONTACT, BIN_SINTER, BIN_SINTER_LEGACY = 0, 1, 2, 3
 HAM_NONE, HAM_FULL, HAM_LIMITED, HAM_SIMPLIFIED = 0, 1, 2, 3
 
 
@@ -144,6 +144,14 @@ def lib():
         L.or_closest_point.argtypes = [_dp, C.c_size_t, _dp, _dp]
         L.or_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
         L.or_sys_set_neck_strengths.argtypes = [C.c_void_p, _dp]
+        _szp = C.POINTER(C.c_size_t)
+        L.or_legacy_frames.restype = C.c_size_t
+        L.or_legacy_frames.argtypes = [C.c_size_t, _dp, C.c_double, C.c_double, _dp, C.c_size_t, _u32p, _dp,
+                                       C.c_size_t, _szp]
+        L.or_sys_legacy_set.argtypes = [C.c_void_p, C.c_size_t, _u32p, _dp]
+        L.or_sys_legacy_get.argtypes = [C.c_void_p, _dp]
+        L.or_legacy_seq.argtypes = [C.c_size_t, _dp, _dp, _dp, _dp, _dp, C.c_size_t, C.c_size_t, _dp, _dp, _szp,
+                                    _szp]
         L.or_sinter_topology.restype = C.c_size_t
         L.or_sinter_topology.argtypes = [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t,
                                          C.POINTER(C.c_size_t)]
@@ -164,6 +172,8 @@ def ref():
         R.ref_fold_seq.argtypes = [_dp, _dp, _dp, C.c_size_t, _dp, _dp]
         R.ref_facet_seq.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_size_t, _dp, _dp]
         R.ref_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
+        R.ref_legacy_seq.argtypes = [C.c_size_t, _dp, _dp, _dp, _dp, _dp, C.c_size_t, C.c_size_t, _dp, _dp,
+                                     C.POINTER(C.c_size_t)]
         R.ref_sinter_topology.restype = C.c_size_t
         R.ref_sinter_topology.argtypes = [C.c_size_t, _dp, _dp, _dp, _u32p, C.c_size_t]
         _ref = R
@@ -310,6 +320,47 @@ def ref_topology(x0, sinter, contact):
 
 
 # -------------------------------------------------------- system driver -----
+def legacy_frames(x0, r_part, d_crit, vecs):
+    """sinter_functor's constructor (or_legacy_frames): bonds (i < j, row
+    order), connectors (nb, 2, 5, 3) = i's five points then j's, and the
+    number of unit vectors drawn from `vecs` (ValueError when they ran out)."""
+    x0, vecs = c64(x0), c64(vecs).reshape(-1, 3)
+    used = C.c_size_t(0)
+    nb = lib().or_legacy_frames(len(x0), dptr(x0), r_part, d_crit, dptr(vecs), len(vecs), None, None, 0,
+                                C.byref(used))
+    pairs = np.zeros((max(nb, 1), 2), dtype=np.uint32)
+    conn = np.zeros((max(nb, 1), 2, 5, 3))
+    lib().or_legacy_frames(len(x0), dptr(x0), r_part, d_crit, dptr(vecs), len(vecs), u32ptr(pairs), dptr(conn), nb,
+                           C.byref(used))
+    if used.value > len(vecs):
+        raise ValueError("legacy_frames: the unit vectors ran out")
+    return pairs[:nb], conn[:nb], used.value
+
+
+def _legacy_seq(fn, n, x0, th0, sinter13, contact18, vecs, incs, with_bonds):
+    x0, th0, vecs, incs = c64(x0), c64(th0), c64(vecs).reshape(-1, 3), c64(incs)
+    nsteps = incs.shape[0]
+    out = np.zeros((nsteps, n, n, 6))
+    used = C.c_size_t(0)
+    args = [n, dptr(x0), dptr(th0), dptr(c64(sinter13)), dptr(c64(contact18)), dptr(vecs), len(vecs), nsteps,
+            dptr(incs), dptr(out)]
+    if with_bonds:
+        nb = C.c_size_t(0)
+        fn(*args, C.byref(nb), C.byref(used))
+    else:
+        fn(*args, C.byref(used))
+    return out, used.value
+
+
+def oracle_legacy_seq(n, x0, th0, sinter13, contact18, vecs, incs):
+    """or_legacy_seq: incs is (nsteps, n, 12) = dx, dtheta, v, omega."""
+    return _legacy_seq(lib().or_legacy_seq, n, x0, th0, sinter13, contact18, vecs, incs, True)
+
+
+def ref_legacy_seq(n, x0, th0, sinter13, contact18, vecs, incs):
+    return _legacy_seq(ref().ref_legacy_seq, n, x0, th0, sinter13, contact18, vecs, incs, False)
+
+
 class OracleSystem:
     """The restated libgran system (functor fold + libtimestep contract)."""
 
@@ -331,6 +382,17 @@ class OracleSystem:
     def set_neck_strengths(self, strengths):
         """alt_breaking_sinter neck strengths, one per bond in bonds() order."""
         lib().or_sys_set_neck_strengths(self._h, dptr(c64(strengths)))
+
+    def set_legacy(self, pairs, conn):
+        """sinter_functor bonds and connectors (the layout of legacy_frames)."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32)
+        lib().or_sys_legacy_set(self._h, len(pairs), u32ptr(pairs), dptr(c64(conn)))
+        self._nlegacy = len(pairs)
+
+    def legacy_connectors(self):
+        out = np.zeros((max(self._nlegacy, 1), 2, 5, 3))
+        lib().or_sys_legacy_get(self._h, dptr(out))
+        return out[:self._nlegacy]
 
     def set_facets(self, verts, vels):
         verts, vels = c64(verts), c64(vels)

@@ -2,7 +2,8 @@
 headers (compiled from the reference tree into oracle/_ref/libref.so by
 `make -C oracle ref`), for the pair/facet sequences of tests/_seq.py, plus the
 bond topology of the reference's shipped aggregate (resources/aggregate.h) and
-of the 3x3x3 lattice of test/alt_sintered.cpp:69-90.
+of the 3x3x3 lattice of test/alt_sintered.cpp:69-90, and sequences of the
+legacy sinter_functor with its step handler (sinter_bridge.h).
 
 Run from the repo root:  python tests/golden/make_golden.py
 """
@@ -37,6 +38,14 @@ def lattice_333(r):
             xx = -1.0 * r + j * 2.0 * r
             x += [[xx, y, 0.0], [xx, y, -2.0 * r], [xx, y, -4.0 * r]]
     return np.array(x)
+
+
+def legacy_four(r):
+    """Four spheres: 0-1 touching along x, 1-2 along y with a 0.5 nm gap
+    (within the 1 nm critical separation), 3 overlapping 0 by 3 nm (a contact,
+    not a bond)."""
+    return np.array([[0.0, 0.0, 0.0], [2.0 * r, 0.0, 0.0], [2.0 * r, 2.0 * r + 5e-10, 0.0],
+                     [-(2.0 * r - 3e-9) * 0.6, (2.0 * r - 3e-9) * 0.8, 0.0]])
 
 
 def main():
@@ -102,6 +111,24 @@ def main():
     lat = lattice_333(O.R_PART)
     out["lattice_x"] = lat
     out["lattice_bonds"] = O.ref_topology(lat, O.sinter_params(d_crit=1.0e-9), contact_sets["std"])
+    # sinter_functor + sinter_step_handler (sinter_bridge.h): frames from
+    # the given unit vectors, per-step increments through the step handler,
+    # every ordered pair evaluated
+    for name, x0, steps in (("legacy4", legacy_four(O.R_PART), 40), ("legacy_lattice", lat, 3)):
+        n = len(x0)
+        rng = np.random.default_rng(77 + n)
+        vecs = rng.normal(size=(400, 3))
+        vecs /= np.linalg.norm(vecs, axis=1, keepdims=True)
+        vecs[0] = [1.0, 0.0, 0.0]  # collinear with the first bond's normal: drawn again
+        incs = np.concatenate([rng.normal(0.0, 2e-11, (steps, n, 3)), rng.normal(0.0, 2e-3, (steps, n, 3)),
+                               rng.normal(0.0, 1.0, (steps, n, 3)), rng.normal(0.0, 1e7, (steps, n, 3))], axis=2)
+        th0 = rng.normal(0.0, 0.5, (n, 3))
+        s13 = O.sinter_params(k_bond=1000.0, d_crit=1.0e-9)
+        s13[1] = 3e-9  # bond damping gamma_d
+        ref, used = O.ref_legacy_seq(n, x0, th0, s13, contact_sets["std"], vecs, incs)
+        out[f"{name}_x0"], out[f"{name}_th0"], out[f"{name}_vecs"] = x0, th0, vecs
+        out[f"{name}_incs"], out[f"{name}_sinter"], out[f"{name}_out"] = incs, s13, ref
+        out[f"{name}_used"] = np.array([used])
     np.savez_compressed(os.path.join(HERE, "functors.npz"), **out)
     print("wrote", os.path.join(HERE, "functors.npz"), len(out), "arrays")
 

@@ -166,3 +166,85 @@ def test_live_reference_fresh_sequences():
         o2, s2, _ = O.ref_sinter_seq(O.sinter_params(), cp, b)
         np.testing.assert_array_equal(o1, o2)
         np.testing.assert_array_equal(s1, s2)
+
+
+def test_golden_legacy_sinter_bit_exact():
+    """sinter_functor with its sinter_step_handler (sinter_bridge.h:13-272):
+    the constructor's frames from the same unit vectors (the first one
+    collinear with a bond normal, so it is drawn again), per-step connector
+    increments, and every ordered pair (bonded: five connector springs;
+    otherwise the embedded contact functor) equal the reference bit for bit,
+    on four particles over 40 steps and on the 27-particle lattice of
+    test/sintered.cpp."""
+    for name in ("legacy4", "legacy_lattice"):
+        x0, th0 = GOLD[f"{name}_x0"], GOLD[f"{name}_th0"]
+        out, used = O.oracle_legacy_seq(len(x0), x0, th0, GOLD[f"{name}_sinter"], GOLD["contact_std_params"],
+                                        GOLD[f"{name}_vecs"], GOLD[f"{name}_incs"])
+        assert used == int(GOLD[f"{name}_used"][0])
+        np.testing.assert_array_equal(out, GOLD[f"{name}_out"])
+    assert int(GOLD["legacy4_used"][0]) == 3  # two bonds, one draw rejected
+
+
+def test_legacy_frames_geometry():
+    """The connector frame of each bond (sinter_bridge.h:163-206): ds and db
+    unit, normal to the bond and to each other; the side connectors at
+    x +- r ds and x +- r db on both particles, so each side spring is the
+    centre-to-centre vector; the fifth at the contact point of each sphere."""
+    r = O.R_PART
+    x0 = np.array([[0.0, 0.0, 0.0], [2.0 * r, 0.0, 0.0], [2.0 * r, 2.0 * r + 5e-10, 0.0],
+                   [-(2.0 * r - 3e-9) * 0.6, (2.0 * r - 3e-9) * 0.8, 0.0]])
+    vecs = np.array([[1.0, 0.0, 0.0], [0.0, 0.6, 0.8], [0.6, 0.0, -0.8], [0.0, 0.0, 1.0]])
+    pairs, conn, used = O.legacy_frames(x0, r, 1.0e-9, vecs)
+    assert pairs.tolist() == [[0, 1], [1, 2]] and used == 3
+    for (i, j), c in zip(pairs, conn):
+        d = x0[j] - x0[i]
+        n = d / np.linalg.norm(d)
+        ds, db = (c[0, 0] - x0[i]) / r, (c[0, 2] - x0[i]) / r
+        for u in (ds, db):
+            assert abs(np.linalg.norm(u) - 1.0) < 1e-9 and abs(u @ n) < 1e-9
+        assert abs(ds @ db) < 1e-9
+        np.testing.assert_allclose(c[0, 1] - x0[i], -ds * r, atol=1e-22)
+        np.testing.assert_allclose(c[0, 3] - x0[i], -db * r, atol=1e-22)
+        for q in range(4):
+            np.testing.assert_allclose(c[1, q] - c[0, q], d, atol=1e-22)
+        np.testing.assert_allclose(c[0, 4], x0[i] + r * n, atol=1e-22)
+        np.testing.assert_allclose(c[1, 4], x0[j] - r * n, atol=1e-22)
+    with pytest.raises(ValueError):
+        O.legacy_frames(x0, r, 1.0e-9, vecs[:2])
+
+
+def test_legacy_rigid_motion_leaves_bonds_unloaded():
+    """Two touching bonded spheres turned rigidly about their centre, with
+    consistent increments (dx = Omega x (x - c) dt, dtheta = Omega dt) and
+    velocities, no bond damping (the functor damps the connectors' raw
+    relative velocity, Omega x d in a rigid turn): a rigid motion stretches
+    no spring, so the bond force stays at rounding level against its scale
+    k_d r. Connectors not turned with the particle (the step handler's
+    rotation left out, a planted mistake) load the bond at once."""
+    r = O.R_PART
+    x0 = np.array([[-r, 0.0, 0.0], [r, 0.0, 0.0]])
+    s13 = O.sinter_params(k_bond=1000.0, d_crit=1.0e-9)
+    s13[1] = 0.0
+    omega = np.array([0.0, 0.0, 2e8])
+    dt, steps = 1e-13, 200
+    incs = np.zeros((steps, 2, 12))
+    x = x0.copy()
+    for k in range(steps):
+        xn = x.copy()
+        for p in range(2):
+            ang = np.linalg.norm(omega) * dt
+            c, s = np.cos(ang), np.sin(ang)
+            xn[p] = [c * x[p, 0] - s * x[p, 1], s * x[p, 0] + c * x[p, 1], 0.0]
+            incs[k, p, 0:3] = xn[p] - x[p]
+            incs[k, p, 3:6] = omega * dt
+            incs[k, p, 6:9] = np.cross(omega, xn[p])
+            incs[k, p, 9:12] = omega
+        x = xn
+    vecs = np.array([[0.0, 1.0, 0.0]])
+    out, used = O.oracle_legacy_seq(2, x0, np.zeros((2, 3)), s13, O.contact_params(), vecs, incs)
+    scale = 1000.0 * r / O.MASS
+    assert np.abs(out[:, 0, 1, 0:3]).max() < 1e-6 * scale
+    wrong = incs.copy()
+    wrong[:, :, 3:6] = 0.0  # dtheta withheld from the step handler: connectors only translate
+    bad, _ = O.oracle_legacy_seq(2, x0, np.zeros((2, 3)), s13, O.contact_params(), vecs, wrong)
+    assert np.abs(bad[-1, 0, 1, 0:3]).max() > 1e-3 * scale  # 4 mrad of lag: 0.8 % of the scale
