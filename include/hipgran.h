@@ -61,7 +61,16 @@ typedef struct hg_hamaker_params {
     double A, h0, r_cutoff, r_part, mass;
 } hg_hamaker_params;
 
-enum { HG_BINARY_NONE = 0, HG_BINARY_CONTACT = 1, HG_BINARY_SINTER = 2 };
+/* HG_BINARY_SINTER_LEGACY: the legacy sinter_functor with its
+ * sinter_step_handler (sinter_bridge.h:13-272), FP64 single systems only.
+ * Every pair i < j with |(|x_j - x_i| - 2r)| <= critical_separation is
+ * bonded (no cycle check) and held by five connector springs whose ends move
+ * with the particles (hg_legacy_sinter_frames, hg_set_legacy_connectors);
+ * other pairs take the embedded contact functor. Parameters: sinter.k = k_d,
+ * sinter.gamma_n = gamma_d, sinter.d_crit = critical_separation; the contact
+ * functor, r_part, mass and inertia from `contact`; other sinter fields are
+ * not read. */
+enum { HG_BINARY_NONE = 0, HG_BINARY_CONTACT = 1, HG_BINARY_SINTER = 2, HG_BINARY_SINTER_LEGACY = 3 };
 /* HG_HAMAKER_SIMPLIFIED: simplified_hamaker_functor
  * (simplified_hamaker_force.h:18-30), 2 A r / h^2 along x_j - x_i. */
 enum { HG_HAMAKER_NONE = 0, HG_HAMAKER_FULL = 1, HG_HAMAKER_LIMITED = 2, HG_HAMAKER_SIMPLIFIED = 3 };
@@ -380,6 +389,32 @@ hg_status hg_generate_aggregate(size_t n, double r_part, double df, double kf, u
  * hg_generate_aggregate with the same arguments. */
 hg_status hg_generate_aggregates(size_t count, const uint32_t *n, const double *df, double kf, const uint64_t *seeds,
                                  double r_part, double *x_out, int threads);
+
+/* A caller's source of random unit vectors, called as the reference's
+ * std::function<field_value_t()> random_unit_vector_generator
+ * (sinter_bridge.h:125): writes one vector to out[0..2]. */
+typedef void (*hg_unit_vector_fn)(void *user, double out[3]);
+/* The legacy sinter_functor constructor's set-up (sinter_bridge.h:138-211),
+ * on the host: bonds are the pairs i < j, in row order, with
+ * |(|x_j - x_i| - 2 r_part)| <= critical_separation; for each, unit vectors
+ * are drawn from `next` until one is not collinear with the bond normal, and
+ * the bond's frame gives ten connector points: connectors[30 b + 3 q + c] is
+ * point q (0..4) attached to particle pairs[2b], connectors[30 b + 15 + 3 q
+ * + c] to particle pairs[2b + 1]. *nbonds gets the bond count; pairs and
+ * connectors are written, and `next` called, only when both are given and
+ * cap >= *nbonds. The draws happen in the reference's order, so a generator
+ * that the reference's constructor would consume gives its connectors. */
+hg_status hg_legacy_sinter_frames(size_t n, const double *x0, double r_part, double critical_separation,
+                                  hg_unit_vector_fn next, void *user, uint32_t *pairs, double *connectors, size_t cap,
+                                  size_t *nbonds);
+/* Gives an HG_BINARY_SINTER_LEGACY system its connectors, in the layout of
+ * hg_legacy_sinter_frames; pairs must be the system's bonds (hg_get_bonds)
+ * in order. Required before the first acceleration pass; later calls replace
+ * the connectors. Replaces the connector state sinter_functor keeps for its
+ * step handler (sinter_bridge.h:261-264). */
+hg_status hg_set_legacy_connectors(hg_system *sys, size_t nbonds, const uint32_t *pairs, const double *connectors);
+/* The connectors as they are now (moved by every step), in the same layout. */
+hg_status hg_get_legacy_connectors(hg_system *sys, double *connectors, size_t cap, size_t *nbonds);
 
 /* Legacy-VTK ASCII POLYDATA of n particles at path: positions divided by
  * r_part, and fields thetax, thetay, thetaz holding each particle's body x,

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_NONE,
+from ._abi import (BINARY_CONTACT, BINARY_NONE, BINARY_SINTER, BINARY_SINTER_LEGACY, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_NONE,
                    HAMAKER_SIMPLIFIED,
                    PAIRS_ALL, PAIRS_NEIGHBOR_LIST, PRECISION_FP64, PRECISION_MIXED, HipgranError, Params)
 
@@ -24,7 +24,8 @@ _abi.lib()  # fail at import time when the HIP library is absent
 ENERGY_TERMS = ("ke_trans", "ke_rot", "u_normal", "u_tangential", "u_rolling", "u_torsion", "u_hamaker", "total")
 
 __all__ = ["System", "Pool", "Params", "make_params", "generate_aggregate", "sinter_topology", "HipgranError",
-           "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
+           "BINARY_NONE", "BINARY_CONTACT", "BINARY_SINTER", "BINARY_SINTER_LEGACY", "legacy_sinter_frames",
+           "HAMAKER_NONE", "HAMAKER_FULL", "HAMAKER_LIMITED",
            "HAMAKER_SIMPLIFIED", "write_vtk_particles", "write_vtk_necks",
            "PAIRS_ALL", "PAIRS_NEIGHBOR_LIST", "PRECISION_FP64", "PRECISION_MIXED"]
 
@@ -65,6 +66,40 @@ def generate_aggregates(sizes, dfs, kf, seeds, r_part, threads=0):
     _abi.check(_abi.lib().hg_generate_aggregates(len(sizes), _abi.u32ptr(sizes), _abi.dptr(dfs), kf,
                                                  seeds.ctypes.data_as(_abi._u64p), r_part, _abi.dptr(x), threads))
     return np.split(x, np.cumsum(sizes)[:-1])
+
+
+_UNIT_VECTOR_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double))
+
+
+def legacy_sinter_frames(x0, r_part, critical_separation, next_unit_vector):
+    """The legacy sinter_functor's bonds and connectors (hg_legacy_sinter_frames):
+    next_unit_vector() returns the reference constructor's random unit
+    vectors, drawn in its order. Returns (pairs (nb, 2), connectors
+    (nb, 2, 5, 3)): per bond i's five points, then j's."""
+    x0 = _abi.c64(x0)
+    nb = C.c_size_t(0)
+    _abi.check(_abi.lib().hg_legacy_sinter_frames(x0.shape[0], _abi.dptr(x0), r_part, critical_separation, None, None,
+                                                  None, None, 0, C.byref(nb)))
+    pairs = np.zeros((max(nb.value, 1), 2), dtype=np.uint32)
+    conn = np.zeros((max(nb.value, 1), 2, 5, 3))
+    errors = []
+
+    def trampoline(_user, out):
+        try:
+            v = next_unit_vector()
+            out[0], out[1], out[2] = float(v[0]), float(v[1]), float(v[2])
+        except Exception as e:  # noqa: BLE001  (reported after the call)
+            errors.append(e)
+            out[0] = out[1] = out[2] = 0.0
+
+    cb = _UNIT_VECTOR_FN(trampoline)
+    st = _abi.lib().hg_legacy_sinter_frames(x0.shape[0], _abi.dptr(x0), r_part, critical_separation,
+                                            C.cast(cb, C.c_void_p), None, _abi.u32ptr(pairs), _abi.dptr(conn),
+                                            nb.value, C.byref(nb))
+    if errors:
+        raise errors[0]
+    _abi.check(st)
+    return pairs[:nb.value], conn[:nb.value]
 
 
 def write_vtk_particles(path, x, theta, r_part):
@@ -193,6 +228,21 @@ class System:
         s = np.zeros((max(tot.value, 1), 9))
         _abi.check(_abi.lib().hg_get_springs(self._h, which, _abi.u32ptr(ij), _abi.dptr(s), tot.value, C.byref(tot)))
         return ij[:tot.value], s[:tot.value]
+
+    def set_legacy_connectors(self, pairs, connectors):
+        """Connectors of an HG_BINARY_SINTER_LEGACY system (hg_set_legacy_connectors),
+        in the layout legacy_sinter_frames returns."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        _abi.check(_abi.lib().hg_set_legacy_connectors(self._h, pairs.shape[0], _abi.u32ptr(pairs),
+                                                       _abi.dptr(_abi.c64(connectors))))
+
+    def legacy_connectors(self):
+        """The connectors now, (nb, 2, 5, 3) in bonds() order."""
+        nb = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_get_legacy_connectors(self._h, None, 0, C.byref(nb)))
+        out = np.zeros((max(nb.value, 1), 2, 5, 3))
+        _abi.check(_abi.lib().hg_get_legacy_connectors(self._h, _abi.dptr(out), nb.value, C.byref(nb)))
+        return out[:nb.value]
 
     def diagnostics(self, mass, inertia):
         out = np.zeros(9)

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libhipgran.so")
 HEADER = os.path.join(ROOT, "include", "hipgran.h")
 
 HG_OK, HG_ERR_INVALID, HG_ERR_OOM, HG_ERR_HIP, HG_ERR_NUMERIC, HG_ERR_CAPACITY, HG_ERR_NO_DEVICE, HG_ERR_IO = range(8)
-BINARY_NONE, BINARY_CONTACT, BINARY_SINTER = 0, 1, 2
+BINARY_NONE, BINARY_CONTACT, BINARY_SINTER, BINARY_SINTER_LEGACY = 0, 1, 2, 3
 HAMAKER_NONE, HAMAKER_FULL, HAMAKER_LIMITED, HAMAKER_SIMPLIFIED = 0, 1, 2, 3
 PAIRS_ALL, PAIRS_NEIGHBOR_LIST = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -104,6 +104,10 @@ SIGNATURES = {
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
     "hg_generate_aggregates": (C.c_int, [C.c_size_t, _u32p, _dp, C.c_double, _u64p, C.c_double, _dp, C.c_int]),
+    "hg_legacy_sinter_frames": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, C.c_void_p, C.c_void_p, _u32p,
+                                          _dp, C.c_size_t, _szp]),
+    "hg_set_legacy_connectors": (C.c_int, [_sysp, C.c_size_t, _u32p, _dp]),
+    "hg_get_legacy_connectors": (C.c_int, [_sysp, _dp, C.c_size_t, _szp]),
     "hg_write_vtk_particles": (C.c_int, [C.c_char_p, C.c_size_t, _dp, _dp, C.c_double]),
     "hg_write_vtk_necks": (C.c_int, [C.c_char_p, C.c_size_t, _dp, C.c_size_t, _u32p, C.c_double]),
     "hg_pool_create":(C.c_int, [C.POINTER(Params), C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(_sysp)]),

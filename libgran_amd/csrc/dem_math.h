@@ -150,6 +150,29 @@ HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 w
     return {ftot / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
 }
 
+// ---- sinter_bridge.h:235-259 (legacy sinter_functor, bonded pair) --------
+// Five springs between matching connectors, ci attached to i and cj to j:
+// rest length 2r for the four side springs, 0 for the one at the contact
+// point; force -(L0 - |s|) k_d s/|s| + gamma_d (relative connector velocity),
+// torque about x_i.
+HGD acc6 legacy_bonded_eval(double k_d, double gamma_d, double r_part, double mass, double inertia, const d3 *ci,
+                            const d3 *cj, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj) {
+    d3 force = zero3(), torque = zero3();
+    for (int q = 0; q < 5; q++) {
+        const d3 spring = cj[q] - ci[q];
+        const double spring_length = norm(spring);
+        const double equilibrium_length = q < 4 ? 2.0 * r_part : 0.0;
+        const d3 conn_i_v = vi + cross(wi, ci[q] - xi);
+        const d3 conn_j_v = vj + cross(wj, cj[q] - xj);
+        const d3 rel_v = conn_j_v - conn_i_v;
+        const d3 dforce = -(equilibrium_length - spring_length) * k_d * normalized(spring) + gamma_d * rel_v;
+        force = force + dforce;
+        const d3 arm = ci[q] - xi;
+        torque = torque + cross(arm, dforce);
+    }
+    return {force / mass, torque / inertia};
+}
+
 // ---- hamaker_force.h:36-39: the scalar factor of -dU/dh --------------------
 HGD double hamaker_scalar(double A, double rp, double h) {
     return (-A) / 6.0 *

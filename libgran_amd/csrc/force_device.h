@@ -77,6 +77,11 @@ struct walker_t {
     uint32_t pre[kTagPrefetch];
     uint32_t nt[kNewTagBuf];
     spring9 rec;  // springs of old entry p (EAGER)
+    // legacy sinter_functor bonds: cursor lb over i's lbc bonded partners
+    // (ascending), lbn the partner at the cursor (kNoEntry past the end)
+    const double *lconn;
+    const uint32_t *lslot;
+    uint32_t lb, lbc, lbn;
 
     __device__ void init(const pair_table &T, uint32_t i_) {
         pj_old = T.pj_old;
@@ -101,6 +106,22 @@ struct walker_t {
         if constexpr (EAGER) rec = ld_rec(ps_old + size_t(i) * kSpringRec);
 #pragma unroll
         for (uint32_t s = 0; s < kNewTagBuf; s++) nt[s] = kNoEntry;
+    }
+    __device__ void init_legacy(const pair_table &T) {
+        lconn = T.lconn;
+        lslot = T.lslot;
+        lb = 0;
+        lbc = bcnt[i];
+        lbn = lbc > 0 ? bj[i] : kNoEntry;
+    }
+    // Partners arrive in ascending order, so the cursor only moves forward.
+    __device__ bool legacy_bonded(uint32_t j, uint32_t &slot) {
+        while (lbn < j) {
+            lb++;
+            lbn = lb < lbc ? bj[size_t(lb) * stride + i] : kNoEntry;
+        }
+        slot = lb;
+        return lbn == j;
     }
     __device__ uint32_t tag_at(uint32_t s) const {
         uint32_t t = pre[0];
@@ -241,7 +262,21 @@ template <int BIN, int HAM, bool EAGER>
 __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
                                           walker_t<EAGER> &W, acc6 &acc) {
     acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
-    if constexpr (BIN != HG_BINARY_NONE) {
+    uint32_t lslot_ij = 0;
+    if (BIN == HG_BINARY_SINTER_LEGACY && W.legacy_bonded(j, lslot_ij)) {
+        // sinter_bridge.h:222-223: the bond's connectors, i's at its slot for
+        // j, j's at its slot for i
+        const double *ri = W.lconn + (size_t(lslot_ij) * W.stride + W.i) * kConnRec;
+        const double *rj = W.lconn + (size_t(W.lslot[size_t(lslot_ij) * W.stride + W.i]) * W.stride + j) * kConnRec;
+        d3 ci[5], cj[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            ci[q] = d3{ri[3 * q], ri[3 * q + 1], ri[3 * q + 2]};
+            cj[q] = d3{rj[3 * q], rj[3 * q + 1], rj[3 * q + 2]};
+        }
+        b = legacy_bonded_eval(P.sinter.k, P.sinter.gamma_n, P.contact.r_part, P.contact.mass, P.contact.inertia, ci,
+                               cj, xi, xj, vi, vj, wi, wj);
+    } else if constexpr (BIN != HG_BINARY_NONE) {
         W.advance_to(j);
         spring9 sp = zero_spring();
         bool bonded = false;

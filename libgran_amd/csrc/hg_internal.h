@@ -35,6 +35,8 @@ constexpr uint32_t kIdPrefetch = 1;
 // Doubles per pair-state record: the 9 spring components and one pad word,
 // so that records stay 16-byte aligned.
 constexpr uint32_t kSpringRec = 10;
+// Doubles per legacy connector record: five points and one pad word.
+constexpr uint32_t kConnRec = 16;
 // New-table tags a particle keeps in registers while it walks its partners
 // and writes at the end of the walk, all lanes together.
 constexpr uint32_t kNewTagBuf = 2;
@@ -60,6 +62,12 @@ struct pair_table {
     const uint32_t *bj;
     const double *bstr;
     const uint32_t *bcnt;
+    // legacy sinter_functor (HG_BINARY_SINTER_LEGACY): bonded partners at
+    // bj/bcnt as above; the connectors particle i holds for its slot-s bond
+    // at lconn[(s*stride + i)*kConnRec + 3q + c] (five points q), and the
+    // slot of i in the partner's list at lslot[s*stride + i] (null otherwise)
+    const double *lconn;
+    const uint32_t *lslot;
 };
 
 // Everything a force/integrate kernel needs, passed by value.
@@ -99,6 +107,10 @@ struct hg_system_impl {
     uint32_t bcap = 0;
     uint32_t *bj = nullptr, *bcnt = nullptr;
     double *bstr = nullptr;
+    // legacy sinter_functor connectors (pair_table::lconn / lslot)
+    double *lconn = nullptr;
+    uint32_t *lslot = nullptr;
+    bool legacy_ready = false;
     // neighbour list
     uint32_t cap_n = 0;
     uint32_t *nb = nullptr, *ncnt = nullptr;
@@ -155,6 +167,11 @@ hipError_t launch_final_kick(double *v, double *w, const double *a, const double
                              facet_dev *facets, int nf, double facet_dt, hipStream_t s);
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
 hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
+// sinter_step_handler's connector moves after a drift (x and v already
+// advanced): each of particle i's connectors follows dx = v_i dt, then turns
+// by dtheta = w_i dt about the new x_i.
+hipError_t launch_legacy_move(const double *x, const double *v, const double *w, uint32_t n, double dt,
+                              double *lconn, const uint32_t *bcnt, hipStream_t s);
 // Exclusive scan of count words; tmp holds scan_tmp_words(count) words.
 hipError_t launch_exclusive_scan(const uint32_t *in, uint32_t *out, uint32_t count, uint32_t *tmp, hipStream_t s);
 uint32_t scan_tmp_words(uint32_t count);
@@ -193,9 +210,11 @@ inline void sum_energy(const double *terms, size_t n, double *out) {
 // host-side set-up (topology.cpp)
 // prevent_cycles: alt_sinter's spanning forest; false: alt_breaking_sinter's
 // every-touching-pair rule.
+// inclusive: gaps of magnitude up to and including d_crit bond (the legacy
+// sinter_functor's rule, sinter_bridge.h:155-157) instead of strictly below.
 void sinter_topology_host(size_t n, const double *x0, double r_part, double d_crit,
                           std::vector<std::pair<uint32_t, uint32_t>> &bonds, size_t &skipped,
-                          bool prevent_cycles = true);
+                          bool prevent_cycles = true, bool inclusive = false);
 
 }  // namespace hg
 

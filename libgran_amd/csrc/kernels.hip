@@ -34,6 +34,7 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     acc6 acc{zero3(), zero3()};
     walker_t<false> W;
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
+    if constexpr (BIN == HG_BINARY_SINTER_LEGACY) W.init_legacy(S.pt);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
     nl_partners(
@@ -85,6 +86,9 @@ __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
     if constexpr (BIN != HG_BINARY_NONE) {
         if (live) W.init(S.pt, i);
     }
+    if constexpr (BIN == HG_BINARY_SINTER_LEGACY) {
+        if (live) W.init_legacy(S.pt);
+    }
     for (uint32_t base = 0; base < S.n; base += kTile) {
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < kTile && base + t < S.n; t += blockDim.x) {
@@ -134,6 +138,29 @@ __global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double
     st3(th, i, ld3(th, i) + wi * dt);
     st3(v, i, vi);
     st3(w, i, wi);
+}
+
+// sinter_step_handler (sinter_bridge.h:24-42, 55-75) after a drift: each of
+// particle i's connectors moves by dx = v_i dt (increment_x), then turns by
+// dtheta = w_i dt about the new centre (increment_theta). v, w and x are the
+// values the drift just used and wrote, so dx and dtheta are its increments.
+__global__ void __launch_bounds__(256) k_legacy_move(const double *x, const double *v, const double *w, uint32_t n,
+                                                     double dt, double *lconn, const uint32_t *bcnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const d3 dx = ld3(v, i) * dt, dth = ld3(w, i) * dt, xi = ld3(x, i);
+    const uint32_t c = bcnt[i];
+    for (uint32_t s = 0; s < c; s++) {
+        double *r = lconn + (size_t(s) * n + i) * kConnRec;
+        for (int q = 0; q < 5; q++) {
+            d3 p = d3{r[3 * q], r[3 * q + 1], r[3 * q + 2]} + dx;
+            const d3 spring_vector = p - xi;
+            p = p + cross(dth, spring_vector);
+            r[3 * q] = p.x;
+            r[3 * q + 1] = p.y;
+            r[3 * q + 2] = p.z;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const double *a, const double *al,
@@ -443,7 +470,8 @@ template <int BIN, int HAM>
 hipError_t launch_force_t(const hg_params &p, const sys_view &sv, hipStream_t s) {
     if (sv.n == 0) return hipSuccess;
     if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST && p.precision == HG_PRECISION_MIXED) {
-        k_force_nl_mx<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, make_mixed_consts(p), sv);
+        if constexpr (BIN == HG_BINARY_SINTER_LEGACY) return hipErrorInvalidValue;  // FP64 only
+        else k_force_nl_mx<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, make_mixed_consts(p), sv);
     } else if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
         // 64-lane blocks spread a modest particle count over more CUs.
         k_force_nl<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
@@ -470,6 +498,10 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
     HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_SIMPLIFIED)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_SIMPLIFIED)
 #undef HG_CASE
     return hipErrorInvalidValue;
 }
@@ -480,6 +512,13 @@ hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const 
     unsigned nb = blocks_for(n > 0 ? n : 1, 256);
     k_kick_drift<<<nb, 256, 0, s>>>(x, v, th, w, a, al, n, pending ? 1 : 0, half_prev, half, dt, facets, nf,
                                     facet_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_legacy_move(const double *x, const double *v, const double *w, uint32_t n, double dt,
+                              double *lconn, const uint32_t *bcnt, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_legacy_move<<<blocks_for(n, 256), 256, 0, s>>>(x, v, w, n, dt, lconn, bcnt);
     return hipGetLastError();
 }
 

@@ -858,6 +858,11 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     if (max_facets > size_t(kMaxFacets)) return fail(HG_ERR_INVALID, "hg_pool_create: at most 16 facets per system");
     if (params->precision != HG_PRECISION_FP64 && params->precision != HG_PRECISION_MIXED)
         return fail(HG_ERR_INVALID, "hg_pool_create: unknown precision");
+    if (params->binary_model < HG_BINARY_NONE || params->binary_model > HG_BINARY_SINTER)
+        return fail(HG_ERR_INVALID, "hg_pool_create: pools run no binary functor, contact_force, alt_sinter or "
+                                    "alt_breaking_sinter; the legacy sinter_functor runs on single systems");
+    if (params->hamaker_model < HG_HAMAKER_NONE || params->hamaker_model > HG_HAMAKER_SIMPLIFIED)
+        return fail(HG_ERR_INVALID, "hg_pool_create: unknown hamaker_model");
     if (params->sinter_breaking != 0 && (params->sinter_breaking != 1 || params->binary_model != HG_BINARY_SINTER))
         return fail(HG_ERR_INVALID, "hg_pool_create: sinter_breaking is 0, or 1 with HG_BINARY_SINTER");
     if (params->pair_mode == HG_PAIRS_NEIGHBOR_LIST && !(params->r_verlet > 0.0))

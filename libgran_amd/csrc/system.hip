@@ -176,6 +176,8 @@ sys_view make_view(hg_system *S) {
     v.pt.bj = S->bj;
     v.pt.bstr = S->bstr;
     v.pt.bcnt = S->bcnt;
+    v.pt.lconn = S->lconn;
+    v.pt.lslot = S->lslot;
     v.facets = S->facets;
     v.ss = S->ss;
     v.nf = S->nf;
@@ -185,6 +187,8 @@ sys_view make_view(hg_system *S) {
 
 // One acceleration pass (queued; no synchronisation).
 hg_status force_pass(hg_system *S) {
+    if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
+        return fail(HG_ERR_INVALID, "the legacy sinter_functor's connectors are not set (hg_set_legacy_connectors)");
     sys_view v = make_view(S);
     HG_HIP(timed(S, PHASE_FORCE, [&] { return launch_force(S->p, v, S->stream); }));
     if (S->p.binary_model != HG_BINARY_NONE) S->cur ^= 1;
@@ -194,8 +198,10 @@ hg_status force_pass(hg_system *S) {
 hg_status validate(const hg_params *p, size_t n) {
     if (!p) return fail(HG_ERR_INVALID, "params is null");
     if (n == 0 || n >= (size_t(1) << 31)) return fail(HG_ERR_INVALID, "n must be in [1, 2^31)");
-    if (p->binary_model < HG_BINARY_NONE || p->binary_model > HG_BINARY_SINTER)
+    if (p->binary_model < HG_BINARY_NONE || p->binary_model > HG_BINARY_SINTER_LEGACY)
         return fail(HG_ERR_INVALID, "unknown binary_model");
+    if (p->binary_model == HG_BINARY_SINTER_LEGACY && p->precision != HG_PRECISION_FP64)
+        return fail(HG_ERR_INVALID, "the legacy sinter_functor runs in HG_PRECISION_FP64 only");
     if (p->hamaker_model < HG_HAMAKER_NONE || p->hamaker_model > HG_HAMAKER_SIMPLIFIED)
         return fail(HG_ERR_INVALID, "unknown hamaker_model");
     if (p->pair_mode != HG_PAIRS_ALL && p->pair_mode != HG_PAIRS_NEIGHBOR_LIST)
@@ -300,6 +306,46 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
         HG_HIPC(hipMemcpyAsync(S->pcnt[0], cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, S->stream));
         HG_HIPC(launch_zero_f64(S->ps[0], size_t(S->cap_s) * kSpringRec * n, S->stream));
         HG_HIPC(hipStreamSynchronize(S->stream));
+    }
+    if (S->p.binary_model == HG_BINARY_SINTER_LEGACY) {
+        // sinter_functor ctor bond search (sinter_bridge.h:146-161): every
+        // pair with |gap| <= critical_separation, no cycle check; the
+        // connectors come with hg_set_legacy_connectors
+        sinter_topology_host(n, x0, S->p.contact.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped, false,
+                             true);
+        std::vector<std::vector<uint32_t>> part(n);
+        for (auto &b : S->bonds) {
+            part[b.first].push_back(b.second);
+            part[b.second].push_back(b.first);
+        }
+        size_t mb = 0;
+        for (auto &r : part) mb = std::max(mb, r.size());
+        if (mb > kMaxBondsPerParticle)
+            return cleanup_on(fail(HG_ERR_CAPACITY, "a particle has " + std::to_string(mb) +
+                                                        " sinter bonds; at most " +
+                                                        std::to_string(kMaxBondsPerParticle)));
+        S->bcap = kMaxBondsPerParticle;
+        std::vector<uint32_t> bj(size_t(S->bcap) * n, kNoEntry), cnt(n, 0), slot(size_t(S->bcap) * n, 0);
+        for (size_t i = 0; i < n; i++) {  // rows come out ascending: bonds are in row order
+            cnt[i] = uint32_t(part[i].size());
+            for (size_t k = 0; k < part[i].size(); k++) bj[k * n + i] = part[i][k];
+        }
+        for (size_t i = 0; i < n; i++)
+            for (size_t k = 0; k < part[i].size(); k++) {
+                const uint32_t j = part[i][k];
+                const auto it = std::lower_bound(part[j].begin(), part[j].end(), uint32_t(i));
+                slot[k * n + i] = uint32_t(it - part[j].begin());
+            }
+        HG_TRYC(dalloc(S, &S->bj, size_t(S->bcap) * n));
+        HG_TRYC(dalloc(S, &S->bcnt, n));
+        HG_TRYC(dalloc(S, &S->lslot, size_t(S->bcap) * n));
+        HG_TRYC(dalloc(S, &S->lconn, size_t(S->bcap) * kConnRec * n));
+        HG_HIPC(hipMemcpyAsync(S->bj, bj.data(), bj.size() * 4, hipMemcpyHostToDevice, S->stream));
+        HG_HIPC(hipMemcpyAsync(S->bcnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, S->stream));
+        HG_HIPC(hipMemcpyAsync(S->lslot, slot.data(), slot.size() * 4, hipMemcpyHostToDevice, S->stream));
+        HG_HIPC(launch_zero_f64(S->lconn, size_t(S->bcap) * kConnRec * n, S->stream));
+        HG_HIPC(hipStreamSynchronize(S->stream));
+        S->legacy_ready = S->bonds.empty();
     }
     if (S->p.binary_model == HG_BINARY_SINTER && S->p.sinter_breaking) {
         if (max_bonds > kMaxBondsPerParticle)
@@ -448,8 +494,10 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
         if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
             HG_HIP(timed(S, PHASE_REBUILD, [&] { return launch_neighbor_build(*S, S->stream); }));
         HG_HIP(timed(S, PHASE_KICK, [&] {
-            return launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt, S->facets,
-                                     nf_advance, dt, S->stream);
+            hipError_t e = launch_kick_drift(S->x, S->v, S->th, S->w, S->a, S->al, n, s > 0, half, half, dt,
+                                             S->facets, nf_advance, dt, S->stream);
+            if (e == hipSuccess && S->lconn) e = launch_legacy_move(S->x, S->v, S->w, n, dt, S->lconn, S->bcnt, S->stream);
+            return e;
         }));
         HG_TRY(force_pass(S));
         S->t = S->t + dt;
@@ -532,6 +580,74 @@ hg_status hg_get_bonds(hg_system *S, uint32_t *pairs, size_t cap, size_t *nbonds
     }
     if (nbonds) *nbonds = S->bonds.size();
     if (nskipped) *nskipped = S->bonds_skipped;
+    return HG_OK;
+}
+
+hg_status hg_set_legacy_connectors(hg_system *S, size_t nbonds, const uint32_t *pairs, const double *connectors) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (S->p.binary_model != HG_BINARY_SINTER_LEGACY)
+        return fail(HG_ERR_INVALID, "hg_set_legacy_connectors: the system is not HG_BINARY_SINTER_LEGACY");
+    if (nbonds != S->bonds.size())
+        return fail(HG_ERR_INVALID, "hg_set_legacy_connectors: the system has " + std::to_string(S->bonds.size()) +
+                                        " bonds, " + std::to_string(nbonds) + " given");
+    if (nbonds && (!pairs || !connectors)) return fail(HG_ERR_INVALID, "hg_set_legacy_connectors: pairs and connectors are required");
+    for (size_t b = 0; b < nbonds; b++)
+        if (pairs[2 * b] != S->bonds[b].first || pairs[2 * b + 1] != S->bonds[b].second)
+            return fail(HG_ERR_INVALID, "hg_set_legacy_connectors: pair " + std::to_string(b) +
+                                            " is not the system's bond (hg_get_bonds order)");
+    const size_t n = S->n;
+    std::vector<uint32_t> bj(size_t(S->bcap) * n), cnt(n);
+    HG_HIP(hipMemcpyAsync(bj.data(), S->bj, bj.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(cnt.data(), S->bcnt, n * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    auto slot_of = [&](uint32_t i, uint32_t j) {
+        uint32_t k = 0;
+        while (k < cnt[i] && bj[size_t(k) * n + i] != j) k++;
+        return k;
+    };
+    std::vector<double> rec(size_t(S->bcap) * kConnRec * n, 0.0);
+    for (size_t b = 0; b < nbonds; b++) {
+        const uint32_t i = pairs[2 * b], j = pairs[2 * b + 1];
+        double *ri = rec.data() + (size_t(slot_of(i, j)) * n + i) * kConnRec;
+        double *rj = rec.data() + (size_t(slot_of(j, i)) * n + j) * kConnRec;
+        for (int c = 0; c < 15; c++) {
+            ri[c] = connectors[30 * b + c];
+            rj[c] = connectors[30 * b + 15 + c];
+        }
+    }
+    HG_HIP(hipMemcpyAsync(S->lconn, rec.data(), rec.size() * 8, hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    S->legacy_ready = true;
+    return HG_OK;
+}
+
+hg_status hg_get_legacy_connectors(hg_system *S, double *connectors, size_t cap, size_t *nbonds) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (S->p.binary_model != HG_BINARY_SINTER_LEGACY)
+        return fail(HG_ERR_INVALID, "hg_get_legacy_connectors: the system is not HG_BINARY_SINTER_LEGACY");
+    if (nbonds) *nbonds = S->bonds.size();
+    if (!connectors || cap < S->bonds.size()) return HG_OK;
+    const size_t n = S->n;
+    std::vector<uint32_t> bj(size_t(S->bcap) * n), cnt(n);
+    std::vector<double> rec(size_t(S->bcap) * kConnRec * n);
+    HG_HIP(hipMemcpyAsync(bj.data(), S->bj, bj.size() * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(cnt.data(), S->bcnt, n * 4, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipMemcpyAsync(rec.data(), S->lconn, rec.size() * 8, hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    auto slot_of = [&](uint32_t i, uint32_t j) {
+        uint32_t k = 0;
+        while (k < cnt[i] && bj[size_t(k) * n + i] != j) k++;
+        return k;
+    };
+    for (size_t b = 0; b < S->bonds.size(); b++) {
+        const uint32_t i = S->bonds[b].first, j = S->bonds[b].second;
+        const double *ri = rec.data() + (size_t(slot_of(i, j)) * n + i) * kConnRec;
+        const double *rj = rec.data() + (size_t(slot_of(j, i)) * n + j) * kConnRec;
+        for (int c = 0; c < 15; c++) {
+            connectors[30 * b + c] = ri[c];
+            connectors[30 * b + 15 + c] = rj[c];
+        }
+    }
     return HG_OK;
 }
 
@@ -651,6 +767,8 @@ hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out)
 
 hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
     if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    if (S->p.binary_model == HG_BINARY_SINTER_LEGACY)
+        return fail(HG_ERR_INVALID, "hg_energy: the ledger does not cover the legacy sinter_functor's connector springs");
     if (!S->energy) HG_TRY(dalloc(S, &S->energy, size_t(8) * S->n));
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, bin = S->p.binary_model != HG_BINARY_NONE;
     energy_view E{S->x,
