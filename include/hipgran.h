@@ -120,7 +120,8 @@ const char *hg_last_error(void);
  * Replaces: the granular_system / granular_system_neighbor_list ctors
  * (granular_system.h:53-63, granular_system_neighbor_list.h:54-64) together
  * with the alt_sinter_functor ctor's bond search (alt_sinter_bridge.h:19-90),
- * which runs here on x0 when binary_model == HG_BINARY_SINTER. In
+ * which runs here on x0 when binary_model == HG_BINARY_SINTER (and the legacy
+ * sinter_functor's, sinter_bridge.h:146-161, for HG_BINARY_SINTER_LEGACY). In
  * neighbour-list mode the list is built once here. theta0/omega0/v0 may be
  * NULL (zeros). */
 hg_status hg_create(const hg_params *params, size_t n, const double *x0, const double *v0,

@@ -6,12 +6,13 @@
 //
 // Supported functor sets (anything else is a compile-time error: there is no
 // host evaluation path): in the binary container at most one of
-// contact_force_functor / alt_sinter_functor and at most one of
+// contact_force_functor / alt_sinter_functor / sinter_functor and at most one of
 // hamaker_functor / limited_range_hamaker_functor / simplified_hamaker_functor;
 // in the unary container
 // triangular facets carrying surface_contact_force_functor and/or
 // surface_hamaker_functor with the same parameters on every facet; the
-// default integrator and step handler.
+// default integrator, and the default step handler (sinter_functor's own
+// step handler when the binary container holds a sinter_functor).
 #ifndef LIBGRAN_DEVICE_SYSTEM_H
 #define LIBGRAN_DEVICE_SYSTEM_H
 
@@ -45,18 +46,22 @@ class device_system {
     using UK = tuple_kinds<decltype(unary_container_t::unary_force_functors)>;
     static_assert(BK::size > 0, "at least one binary force functor must be provided as a template parameter");
     static_assert(BK::all_known(), "the device path implements contact_force_functor, alt_sinter_functor, "
-                                   "hamaker_functor, limited_range_hamaker_functor and simplified_hamaker_functor only");
-    static_assert(BK::count(kind_contact) + BK::count(kind_sinter) <= 1,
-                  "at most one contact-type functor (contact_force_functor or alt_sinter_functor)");
+                                   "sinter_functor, hamaker_functor, limited_range_hamaker_functor and "
+                                   "simplified_hamaker_functor only");
+    static_assert(BK::count(kind_contact) + BK::count(kind_sinter) + BK::count(kind_legacy_sinter) <= 1,
+                  "at most one contact-type functor (contact_force_functor, alt_sinter_functor or sinter_functor)");
     static_assert(BK::count(kind_hamaker) + BK::count(kind_limited_hamaker) + BK::count(kind_simplified_hamaker) <= 1,
                   "at most one Hamaker functor");
-    static_assert(BK::count(kind_contact) + BK::count(kind_sinter) + BK::count(kind_hamaker) +
-                          BK::count(kind_limited_hamaker) + BK::count(kind_simplified_hamaker) == BK::size,
+    static_assert(BK::count(kind_contact) + BK::count(kind_sinter) + BK::count(kind_legacy_sinter) +
+                          BK::count(kind_hamaker) + BK::count(kind_limited_hamaker) +
+                          BK::count(kind_simplified_hamaker) == BK::size,
                   "unsupported binary functor");
     static_assert(UK::count(kind_facet) == UK::size, "the unary container may hold triangular_facet objects only");
 
 public:
     typedef std::vector<field_value_t> field_container_t;
+    // the binary container holds the legacy sinter_functor
+    static constexpr bool uses_legacy_sinter = BK::count(kind_legacy_sinter) == 1;
 
     device_system(int pair_mode, double r_verlet, const field_container_t &x0, const field_container_t &v0,
                   const field_container_t &theta0, const field_container_t &omega0, real_t t0,
@@ -157,7 +162,7 @@ protected:
 private:
     template <typename F>
     void attach_functor(F &f) {
-        if constexpr (kind_of<F>::value == kind_sinter) f.attach(sys);
+        if constexpr (kind_of<F>::value == kind_sinter || kind_of<F>::value == kind_legacy_sinter) f.attach(sys);
     }
 
     struct cache_t {
@@ -192,6 +197,25 @@ template <template <typename, typename> typename H>
 struct is_default_step_handler : std::false_type {};
 template <>
 struct is_default_step_handler<rotational_step_handler> : std::true_type {};
+
+// sinter_functor's step handler (sinter_bridge.h), recognised on the
+// instantiated type: drivers name it through an alias template that fixes
+// real_t (test/sintered.cpp:46-47).
+template <typename T, typename = void>
+struct is_legacy_sinter_handler : std::false_type {};
+template <typename T>
+struct is_legacy_sinter_handler<T, std::void_t<decltype(T::hg_legacy_sinter_handler)>> : std::true_type {};
+
+// The step handler a granular_system accepts: the default one, or with a
+// sinter_functor exactly that functor's handler (the device moves its
+// connectors; without them the bonds would not follow the particles).
+template <template <typename, typename> typename H, typename field_value_t, bool legacy>
+constexpr bool step_handler_ok() {
+    if constexpr (legacy)
+        return is_legacy_sinter_handler<H<std::vector<field_value_t>, field_value_t>>::value;
+    else
+        return is_default_step_handler<H>::value;
+}
 
 }  // namespace libgran_amd
 

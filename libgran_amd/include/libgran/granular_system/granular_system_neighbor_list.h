@@ -24,8 +24,9 @@ class granular_system_neighbor_list
                                             unary_force_functor_container_t>;
     static_assert(libgran_amd::is_default_integrator<integrator_t>::value,
                   "the device path implements rotational_velocity_verlet_half");
-    static_assert(libgran_amd::is_default_step_handler<step_handler_t>::value,
-                  "the device path implements the default rotational_step_handler");
+    static_assert(libgran_amd::step_handler_ok<step_handler_t, field_value_t, base::uses_legacy_sinter>(),
+                  "the device path implements the default rotational_step_handler, and sinter_functor's own "
+                  "step handler together with sinter_functor");
 
 public:
     typedef std::vector<field_value_t> field_container_t;

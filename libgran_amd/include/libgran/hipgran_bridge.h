@@ -28,7 +28,8 @@ enum functor_kind : int {
     kind_surface_contact = 5,
     kind_surface_hamaker = 6,
     kind_facet = 7,
-    kind_simplified_hamaker = 8
+    kind_simplified_hamaker = 8,
+    kind_legacy_sinter = 9
 };
 
 template <typename T, typename = void>

@@ -24,6 +24,15 @@ struct vec3 {
     }
     double dot(const vec3 &o) const { return (c[0] * o.c[0] + c[1] * o.c[1]) + c[2] * o.c[2]; }
     double norm() const { return std::sqrt(dot(*this)); }
+    double squaredNorm() const { return dot(*this); }
+    // Eigen's semantics: a zero vector is returned unchanged
+    vec3 normalized() const {
+        const double z = dot(*this);
+        return z > 0.0 ? *this / std::sqrt(z) : *this;
+    }
+    vec3 cross(const vec3 &o) const {
+        return {c[1] * o.c[2] - c[2] * o.c[1], c[2] * o.c[0] - c[0] * o.c[2], c[0] * o.c[1] - c[1] * o.c[0]};
+    }
 };
 inline vec3 operator*(double s, const vec3 &a) { return a * s; }
 }  // namespace libgran_amd

@@ -6,11 +6,12 @@
 // with the CPU oracle.
 //
 //   dropin_scenarios <scenario> <steps> <output-file>
-//   scenarios: contact hamaker hamaker_neighbor simplified alt_sintered breaking surface plate
+//   scenarios: contact hamaker hamaker_neighbor simplified alt_sintered sintered breaking surface plate
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,7 @@
 #include <libgran/hamaker_force/surface_hamaker_force.h>
 #include <libgran/sinter_bridge/alt_breaking_sinter_bridge.h>
 #include <libgran/sinter_bridge/alt_sinter_bridge.h>
+#include <libgran/sinter_bridge/sinter_bridge.h>
 #include <libgran/surface_force/triangular_facet.h>
 #include <libgran/vec3.h>
 #include <writer.h>
@@ -187,6 +189,59 @@ int run_alt_sintered(size_t steps, const char *out) {
     return 0;
 }
 
+// test/sintered.cpp: the legacy sinter_functor with its step handler on the
+// 3x3x3 lattice, unit vectors from the reference's own generator
+// (sintered.cpp:30-39). The vectors handed out are written to <out>.vecs
+// (count as uint64, then 3 doubles each) for the oracle to draw the same.
+template <typename field_container_t, typename field_value_t>
+using sinter_step_handler_double = sinter_step_handler<field_container_t, field_value_t, double>;
+
+int run_sintered(size_t steps, const char *out) {
+    VV x0, v0;
+    for (size_t i = 0; i < 3; i++) {
+        auto y = -1.0 * r_part + double(i) * 2.0 * r_part;
+        for (size_t j = 0; j < 3; j++) {
+            auto x = -1.0 * r_part + double(j) * 2.0 * r_part;
+            x0.emplace_back(x, y, 0.0);
+            x0.emplace_back(x, y, -2.0 * r_part);
+            x0.emplace_back(x, y, -4.0 * r_part);
+            V3 vel = j == 0 ? V3(0.0, 0.0, 5.0) : (j == 2 ? V3(0.0, 5.0, 0.0) : V3());
+            for (size_t q = 0; q < 3; q++) v0.push_back(vel);
+        }
+    }
+    VV th0(x0.size()), w0(x0.size());
+    const double dt = 1e-13, gamma_d = 0.0;
+    std::mt19937_64 mt(0);
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+    VV drawn;
+    auto generate_random_unit_vector = [&]() {
+        V3 vec;
+        do {
+            vec = V3{dist(mt), dist(mt), dist(mt)};
+        } while (vec.norm() == 0);
+        drawn.push_back(vec.normalized());
+        return drawn.back();
+    };
+    sinter_functor<V3, double> sinter_model(x0.size(), k, gamma_d, k, gamma_n, k, gamma_t, mu, phi, k, gamma_r, mu_o,
+                                            phi, k, gamma_o, mu_o, phi, r_part, mass, inertia, dt, V3(), 0.0,
+                                            x0.cbegin(), generate_random_unit_vector, 1.0e-9);
+    binary_force_functor_container<V3, double, sinter_functor<V3, double>> binary{sinter_model};
+    unary_force_functor_container<V3, double> unary;
+    auto step_handler_instance = sinter_model.get_step_handler<VV>();
+    granular_system<V3, double, rotational_velocity_verlet_half, sinter_step_handler_double, decltype(binary),
+                    decltype(unary)>
+        system(x0, v0, th0, w0, 0.0, V3(), 0.0, step_handler_instance, binary, unary);
+    for (size_t n = 0; n < steps; n++) system.do_step(dt);
+    FILE *f = std::fopen((std::string(out) + ".vecs").c_str(), "wb");
+    if (!f) return 3;
+    const uint64_t nv = drawn.size();
+    std::fwrite(&nv, sizeof(nv), 1, f);
+    for (const V3 &q : drawn) std::fwrite(q.c, sizeof(double), 3, f);
+    std::fclose(f);
+    finish(system, out);
+    return 0;
+}
+
 // Neck strengths 1e-8 N * (1 + b % 7) for the b-th draw.
 struct cycling_strength {
     size_t b = 0;
@@ -276,6 +331,7 @@ int main(int argc, char **argv) {
         if (s == "hamaker_neighbor") return run_contact(steps, argv[3], true, true);
         if (s == "simplified") return run_simplified(steps, argv[3]);
         if (s == "alt_sintered") return run_alt_sintered(steps, argv[3]);
+        if (s == "sintered") return run_sintered(steps, argv[3]);
         if (s == "breaking") return run_breaking(steps, argv[3]);
         if (s == "surface") return run_surface(steps, argv[3], false);
         if (s == "plate") return run_surface(steps, argv[3], true);

@@ -72,3 +72,45 @@ def test_generator_fractal_scaling():
         rg = np.sqrt(((x - x.mean(axis=0)) ** 2).sum(axis=1).mean())
         n_pred = 1.3 * (rg / O.R_PART) ** df
         assert 0.5 < n_pred / n < 2.0, (df, n_pred)
+
+
+LEGACY_DRIVER = r"""
+#include <functional>
+#include <libgran/granular_system/granular_system.h>
+#include <libgran/sinter_bridge/sinter_bridge.h>
+#include <libgran/vec3.h>
+using V = libgran_amd::vec3;
+template <typename C, typename F> using handler_double = sinter_step_handler<C, F, double>;
+int main() {
+    std::vector<V> x0{V(0, 0, 0), V(2.8e-8, 0, 0)}, z(2);
+    std::function<V()> gen = [] { return V(0, 1, 0); };
+    sinter_functor<V, double> s(2, 1e4, 0.0, 1e4, 5e-9, 1e4, 1e-9, 1.0, 1.0, 1e4, 2.5e-10, 0.1, 1.0, 1e4, 2.5e-10,
+                                0.1, 1.0, 1.4e-8, 1e-20, 1e-36, 1e-13, V(), 0.0, x0.cbegin(), gen, 1e-9);
+    binary_force_functor_container<V, double, sinter_functor<V, double>> b{s};
+    unary_force_functor_container<V, double> u;
+    auto h = s.get_step_handler<std::vector<V>>();
+    granular_system<V, double, rotational_velocity_verlet_half, HANDLER, decltype(b), decltype(u)>
+        sys(x0, z, z, z, 0.0, V(), 0.0, h, b, u);
+    return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("handler,ok", [("handler_double", True), ("rotational_step_handler", False)])
+def test_dropin_legacy_sinter_needs_its_step_handler(tmp_path, handler, ok):
+    """sinter_functor compiles with the step handler it hands out
+    (sinter_bridge.h:228-232, as test/sintered.cpp:117-127 uses it) and is a
+    compile-time error with the default handler: the device moves the
+    connectors, which the default handler would leave behind."""
+    import subprocess
+    src = tmp_path / "d.cpp"
+    src.write_text(LEGACY_DRIVER.replace("HANDLER", handler))
+    if handler == "rotational_step_handler":  # the default handler would also not bind to h's type
+        src.write_text(src.read_text().replace("auto h = s.get_step_handler<std::vector<V>>();",
+                                               "rotational_step_handler<std::vector<V>, V> h;"))
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libgran_amd", "include")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(src)], capture_output=True, text=True)
+    if ok:
+        assert r.returncode == 0, r.stderr
+    else:
+        assert r.returncode != 0 and "sinter_functor's own step handler" in r.stderr

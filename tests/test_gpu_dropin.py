@@ -120,6 +120,29 @@ def test_dropin_alt_sintered(tmp_path):
     assert (tmp_path / "necks_1.vtk").read_text() == V.necks_text(xs, bonds, R)
 
 
+def test_dropin_sintered(tmp_path):
+    """test/sintered.cpp set-up through the legacy sinter_functor and the step
+    handler it hands out, unit vectors from the reference's own
+    std::mt19937_64(0) generator: the oracle, given the vectors the driver
+    drew, reproduces the run bit for bit."""
+    steps = 5000
+    got = run_driver(tmp_path, "sintered", steps)
+    raw = (tmp_path / "sintered.bin.vecs").read_bytes()
+    nv = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
+    vecs = np.frombuffer(raw[8:], dtype=np.float64).reshape(nv, 3)
+    assert nv == 54  # one draw per bond of the lattice
+    x, v = S.sinter_lattice()
+    c = O.contact_params(dt=1e-13)
+    s13 = O.sinter_params(k_bond=1.0e4, d_crit=1.0e-9, dt=1e-13)
+    s13[1] = 0.0
+    pairs, conn, used = O.legacy_frames(x, R, 1.0e-9, vecs)
+    assert used == nv
+    o = oracle(O.BIN_SINTER_LEGACY, O.HAM_NONE, x, v, contact=c, sinter=s13)
+    o.set_legacy(pairs, conn)
+    o.step(1e-13, steps)
+    compare(o, got, "sintered")
+
+
 def test_dropin_alt_breaking_sintered(tmp_path):
     """The same lattice through alt_breaking_sinter_functor (every touching
     pair bonded, neck strengths 1e-8 N * (1 + b % 7) in draw order); the driver
