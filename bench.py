@@ -179,6 +179,63 @@ def energy_drift(L, precision, steps=20000, every=1000):
             "end_J": {t: float(f"{e1[t]:.6e}") for t in L.ENERGY_TERMS}}
 
 
+def legacy_sinter_leg(L, steps=2000):
+    """The legacy sinter_functor (five connector springs per bond, moved by
+    the step handler) with limited-range Hamaker on the N=20,000 aggregate,
+    FP64 neighbour list 3.2r rebuilt every 50 steps: particle-steps/s."""
+    import torch
+    c, s, h = aggregate_params()
+    s = list(s)
+    s[12] = 1e-3 * R_PART
+    x = L.generate_aggregate(20000, R_PART, 1.8, 1.3, 1)
+    v0 = np.random.default_rng(1).normal(0.0, 1.0, x.shape)
+    vecs = np.random.default_rng(2).uniform(-1.0, 1.0, (200000, 3))
+    it = iter(vecs / np.linalg.norm(vecs, axis=1, keepdims=True))
+    pairs, conn = L.legacy_sinter_frames(x, R_PART, s[12], lambda: next(it))
+    p = L.make_params(L.BINARY_SINTER_LEGACY, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST,
+                      r_verlet=3.2 * R_PART, contact=c, sinter=s, hamaker=h)
+    sys_ = L.System(p, x, v0)
+    sys_.set_legacy_connectors(pairs, conn)
+    sys_.step(DT, 200, rebuild_every=REBUILD)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sys_.step(DT, steps, rebuild_every=REBUILD)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sys_.close()
+    return {"value": round(20000 * steps / el, 1), "unit": "particle-steps/s", "bonds": int(len(pairs)),
+            "steps": steps}
+
+
+def ledger_and_writer_leg(L, calls=20):
+    """Cost of one energy ledger (hg_energy: kernel, copy back, host sum) on
+    the N=20,000 aggregate, and of writing its state as a VTK particle file
+    (hg_write_vtk_particles, 20,000 particles)."""
+    import tempfile
+    x = L.generate_aggregate(20000, R_PART, 1.8, 1.3, 1)
+    v0 = np.random.default_rng(1).normal(0.0, 1.0, x.shape)
+    sys_ = L.System(params(L, L.PRECISION_FP64), x, v0)
+    sys_.step(DT, 100, rebuild_every=REBUILD)
+    sys_.energy(MASS, INERTIA)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        sys_.energy(MASS, INERTIA)
+    ledger_ms = (time.perf_counter() - t0) * 1e3 / calls
+    st = sys_.state()
+    sys_.close()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "particles.vtk")
+        L.write_vtk_particles(path, st["x"], st["theta"], R_PART)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            L.write_vtk_particles(path, st["x"], st["theta"], R_PART)
+        vtk_s = (time.perf_counter() - t0) / 5
+        size = os.path.getsize(path)
+    return {"energy_ledger_ms_per_call": round(ledger_ms, 3),
+            "vtk_particles_per_s": round(20000 / vtk_s, 1), "vtk_mb_per_s": round(size / vtk_s / 1e6, 1),
+            "vtk_threads": min(16, os.cpu_count() or 1)}
+
+
 def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0, seed=7):
     """The serving stream: `count` plate tests (N log-uniform 100..20000, Df
     U(1.6, 2.2), plates at U(1, 10) m/s, compression or tension) arriving as
@@ -355,6 +412,8 @@ def main():
             "note": "the aggregate above with normal springs (bonds, contacts) and limited-range Hamaker only, no "
                     "damping; largest ledger total drift over 20,000 steps relative to the ledger scale, dt 1e-13 s",
             "fp64": energy_drift(L, L.PRECISION_FP64), "mixed": energy_drift(L, L.PRECISION_MIXED)}
+        extra["legacy_sinter_n20000_fp64"] = legacy_sinter_leg(L)
+        extra["ledger_and_vtk_n20000"] = ledger_and_writer_leg(L)
         if args.serving_requests > 0:
             extra["serving"] = serving_leg(L, args.serving_requests)
         if not args.no_cpu_baseline:
