@@ -276,6 +276,7 @@ def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0,
 
 
 COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counters.txt")
+VALU_COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_valu_counters.txt")
 
 
 def measured_traffic(tuning, count):
@@ -296,6 +297,26 @@ def measured_traffic(tuning, count):
     return (int(m.group(1).replace(",", "")),
             f"profiles/pool_chunk_mixed_batch4096_hbm_counters.txt (measured at split {split.group(1)}:"
             f"{split.group(2)})")
+
+
+def measured_valu(tuning, count, launch_us):
+    """FP32 and FP64 VALU throughput of the chunk kernel: FLOPs per launch from
+    the committed SQ_INSTS_VALU_* profile (valid at its wide threshold) over
+    this run's average launch duration, against MI355X's vector peaks."""
+    try:
+        txt = open(VALU_COUNTERS).read()
+    except OSError:
+        return None
+    split = re.search(r"launch split fixed at\s*(\d+):(\d+)", txt)
+    f32 = re.search(r"FP32\s+([\d,]+) flop per launch", txt)
+    f64 = re.search(r"FP64\s+([\d,]+) flop per launch", txt)
+    if not (split and f32 and f64) or count != 4096 or tuning["wide_min"] != int(split.group(1)):
+        return None
+    out = {"source": "profiles/pool_chunk_mixed_batch4096_valu_counters.txt"}
+    for name, m, peak in (("fp32", f32, 157.3), ("fp64", f64, 78.6)):
+        tf = int(m.group(1).replace(",", "")) / (launch_us * 1e-6) / 1e12
+        out[name] = {"achieved": round(tf, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4)}
+    return out
 
 
 def cpu_baseline(x, v0, steps):
@@ -380,6 +401,7 @@ def main():
     bytes_launch = bpps * narrow * REBUILD
     achieved = bytes_launch / (chunk_us * 1e-6) / 1e9
     traffic, traffic_note = measured_traffic(tuning, args.count)
+    valu = measured_valu(tuning, args.count, chunk_us)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                 "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate below the "
@@ -389,7 +411,8 @@ def main():
                 "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
                 "kbar": round(kbar, 3), "cbar": round(cbar, 3),
                 "rebuild_us_per_interval": round(prof_mx["rebuild_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1),
-                "wide_us_per_interval": round(prof_mx["wide_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1)}
+                "wide_us_per_interval": round(prof_mx["wide_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1),
+                "valu": valu}
     extra = {"aggregates": len(xs), "particles": total, "generation_s": round(t_gen, 1),
              "launch_split": {"wide_min": tuning["wide_min"], "wide_cus": tuning["wide_cus"],
                               "chunk_particles": narrow, "wide_particles": total - narrow}}
