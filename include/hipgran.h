@@ -205,7 +205,10 @@ hg_status hg_diagnostics(hg_system *sys, double mass, double inertia, double *ou
  * k/2 |xi|^2 with the pair's two ordered copies averaged; U_hamaker the
  * README.md:154-157 potential (or -2Ar/h for HG_HAMAKER_SIMPLIFIED),
  * continued linearly below h0 and, for HG_HAMAKER_LIMITED, shifted to zero at
- * r_cutoff, so that its derivative is the force the functors apply. Facet
+ * r_cutoff, so that its derivative is the force the functors apply. With
+ * HG_BINARY_SINTER_LEGACY a bond's U_normal is its five connector springs,
+ * k_d/2 (|s| - L0)^2 each (the potential of sinter_bridge.h:251), and the
+ * spring terms hold the contact springs of unbonded pairs. Facet
  * terms are not included. Per-particle terms are summed in particle order. */
 hg_status hg_energy(hg_system *sys, double mass, double inertia, double *out);
 

@@ -191,6 +191,10 @@ struct energy_view {
     const uint32_t *nb, *ncnt;
     const uint32_t *pj, *pcnt;
     const double *ps;
+    // legacy sinter_functor: bonded partners, reverse slots and connectors
+    // (pair_table::bj/bcnt/lslot/lconn layout; null otherwise)
+    const uint32_t *bj, *bcnt, *lslot;
+    const double *lconn;
 };
 // Eight terms per particle into out[8*k + c] (k = local index): kinetic
 // translational and rotational; the particle's half of its pairs' normal

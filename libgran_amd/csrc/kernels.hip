@@ -438,7 +438,21 @@ __global__ void __launch_bounds__(256) k_energy(hg_params P, energy_view E, doub
     }
     auto pair = [&](uint32_t j) {
         const double dist = norm(xi - ld3(E.x, j));
-        if (P.binary_model != HG_BINARY_NONE) {
+        uint32_t ls = kNoEntry;
+        if (E.lconn)
+            for (uint32_t s = 0; s < E.bcnt[g]; s++)
+                if (E.bj[size_t(s) * E.stride + g] == j) ls = s;
+        if (ls != kNoEntry) {
+            // the bond's five connector springs, k_d/2 (|s| - L0)^2 each, half
+            // to each ordered pair
+            const double *ri = E.lconn + (size_t(ls) * E.stride + g) * kConnRec;
+            const double *rj = E.lconn + (size_t(E.lslot[size_t(ls) * E.stride + g]) * E.stride + j) * kConnRec;
+            for (int q = 0; q < 5; q++) {
+                const d3 sq = d3{rj[3 * q], rj[3 * q + 1], rj[3 * q + 2]} - d3{ri[3 * q], ri[3 * q + 1], ri[3 * q + 2]};
+                const double d = (q < 4 ? 2.0 * P.contact.r_part : 0.0) - norm(sq);
+                e[2] += 0.25 * P.sinter.k * d * d;
+            }
+        } else if (P.binary_model != HG_BINARY_NONE) {
             bool bonded = false;
             for (uint32_t s = 0; s < cnt; s++) {
                 const uint32_t tag = E.pj[size_t(s) * E.stride + g];

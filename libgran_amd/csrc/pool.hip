@@ -1458,7 +1458,11 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
                   nl ? Q->ncnt : nullptr,
                   tab ? Q->pj[M.cur] : nullptr,
                   tab ? Q->pcnt[M.cur] : nullptr,
-                  tab ? Q->ps[M.cur] : nullptr};
+                  tab ? Q->ps[M.cur] : nullptr,
+                  nullptr,
+                  nullptr,
+                  nullptr,
+                  nullptr};
     std::vector<double> h(size_t(8) * M.n);
     hipError_t e = launch_energy(Q->p, E, mass, inertia, terms, Q->stream);
     if (e == hipSuccess)

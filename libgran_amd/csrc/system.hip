@@ -767,8 +767,6 @@ hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out)
 
 hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
     if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
-    if (S->p.binary_model == HG_BINARY_SINTER_LEGACY)
-        return fail(HG_ERR_INVALID, "hg_energy: the ledger does not cover the legacy sinter_functor's connector springs");
     if (!S->energy) HG_TRY(dalloc(S, &S->energy, size_t(8) * S->n));
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, bin = S->p.binary_model != HG_BINARY_NONE;
     energy_view E{S->x,
@@ -781,7 +779,13 @@ hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
                   nl ? S->ncnt : nullptr,
                   bin ? S->pj[S->cur] : nullptr,
                   bin ? S->pcnt[S->cur] : nullptr,
-                  bin ? S->ps[S->cur] : nullptr};
+                  bin ? S->ps[S->cur] : nullptr,
+                  S->lconn ? S->bj : nullptr,
+                  S->lconn ? S->bcnt : nullptr,
+                  S->lslot,
+                  S->lconn};
+    if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
+        return fail(HG_ERR_INVALID, "hg_energy: the legacy sinter_functor's connectors are not set");
     HG_HIP(launch_energy(S->p, E, mass, inertia, S->energy, S->stream));
     std::vector<double> h(size_t(8) * S->n);
     HG_HIP(hipMemcpyAsync(h.data(), S->energy, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));

@@ -579,6 +579,7 @@ public:
     }
     // i's connectors of bond k
     v3 *legacy_side(size_t i, uint32_t k) { return legacy[k].i == i ? legacy[k].ci : legacy[k].cj; }
+    const v3 *legacy_side(size_t i, uint32_t k) const { return legacy[k].i == i ? legacy[k].ci : legacy[k].cj; }
     int64_t legacy_bond_of(size_t i, uint32_t j) const {
         if (legacy_of.empty()) return -1;
         const auto &r = legacy_of[i];
@@ -682,7 +683,16 @@ public:
             }
             auto pair = [&](uint32_t j) {
                 const double dist = norm(x[i] - x[j]);
-                if (p.binary_model != BIN_NONE) {
+                const int64_t lb = p.binary_model == BIN_SINTER_LEGACY ? legacy_bond_of(i, j) : -1;
+                if (lb >= 0) {
+                    // the bond's five connector springs, k_d/2 (|s| - L0)^2
+                    // each, half to each ordered pair
+                    const v3 *ci = legacy_side(i, uint32_t(lb)), *cj = legacy_side(j, uint32_t(lb));
+                    for (int q = 0; q < 5; q++) {
+                        const double d = (q < 4 ? 2.0 * p.contact.r_part : 0.0) - norm(cj[q] - ci[q]);
+                        e[2] += 0.25 * p.sinter.k * d * d;
+                    }
+                } else if (p.binary_model != BIN_NONE) {
                     if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
                         const double ov = 2.0 * p.sinter.r_part - dist;
                         e[2] += 0.25 * p.sinter.k * ov * ov;

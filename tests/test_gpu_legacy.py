@@ -61,6 +61,14 @@ def same(g, o, what):
     for k in ("x", "v", "theta", "omega", "a", "alpha"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=f"{what}: {k}")
     np.testing.assert_array_equal(g.legacy_connectors(), o.legacy_connectors(), err_msg=f"{what}: connectors")
+    # the energy ledger (connector springs in the normal term): the oracle's
+    # to 1e-12 of its scale, as in test_gpu_energy.py
+    L = _L()
+    dev = np.array([g.energy(O.MASS, O.INERTIA)[t] for t in L.ENERGY_TERMS])
+    orc = o.energy()
+    err = np.abs(dev - orc).max() / np.abs(orc[:7]).sum()
+    print(f"[gpu-legacy] {what}: ledger max |dev - oracle| / scale {err:.2e}", flush=True)
+    assert err < 1e-12 and dev[2] > 0.0
 
 
 def test_legacy_lattice_all_pairs():
@@ -97,8 +105,8 @@ def test_legacy_aggregate_neighbor_list(ham_model):
 
 
 def test_legacy_errors():
-    """Stepping before the connectors are set, mixed precision, a pool, the
-    wrong bond list and the energy ledger are refused."""
+    """Stepping or reading the ledger before the connectors are set, mixed
+    precision, a pool and the wrong bond list are refused."""
     announce("error paths")
     L = _L()
     x, v = S.sinter_lattice()
@@ -107,13 +115,13 @@ def test_legacy_errors():
     g = L.System(p, x, v)
     with pytest.raises(L.HipgranError, match="connectors"):
         g.step(1e-13, 1)
+    with pytest.raises(L.HipgranError, match="connectors"):
+        g.energy(O.MASS, O.INERTIA)
     pairs, conn, _ = O.legacy_frames(x, R, 1e-9, unit_vectors(1))
     with pytest.raises(L.HipgranError):
         g.set_legacy_connectors(pairs[::-1], conn[::-1])
     g.set_legacy_connectors(pairs, conn)
     g.step(1e-13, 1)
-    with pytest.raises(L.HipgranError):
-        g.energy(O.MASS, O.INERTIA)
     pm = L.make_params(L.BINARY_SINTER_LEGACY, L.HAMAKER_NONE, pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=3.2 * R,
                        contact=c, sinter=s13, precision=L.PRECISION_MIXED)
     with pytest.raises(L.HipgranError, match="FP64"):

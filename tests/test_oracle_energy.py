@@ -136,3 +136,36 @@ def test_contact_collision_conserves_ledger():
     print(f"contact: total drift {drift:.3e} J of {e[0, 7]:.3e} J")
     assert drift < 1e-4 * e[0, 7]
     assert e[-1, 2] == 0.0 and abs(e[-1, 0] - e[0, 0]) < 1e-4 * e[0, 0]  # separated again, elastic
+
+
+def legacy_pair_energy(dt, T, v, w):
+    """Two touching spheres bonded by the legacy sinter_functor (undamped
+    connector springs, k_d = 1000), the ledger sampled 200 times over T."""
+    x = np.array([[0.0, 0.0, 0.0], [2.0 * R, 0.0, 0.0]])
+    s13 = undamped_sinter(dt=dt)
+    pairs, conn, _ = O.legacy_frames(x, R, 1.0e-9, np.array([[0.0, 0.6, 0.8]]))
+    o = O.OracleSystem(O.system_params(O.BIN_SINTER_LEGACY, O.HAM_NONE, contact=undamped_contact(dt=dt),
+                                       sinter=s13), x, v, w0=w)
+    o.set_legacy(pairs, conn)
+    n = int(round(T / dt))
+    e, _ = run(o, dt, n, n // 200)
+    return e
+
+
+def test_legacy_bond_ledger_converges_at_second_order():
+    """The legacy bond's five connector springs (sinter_bridge.h:244-256) are
+    forces of position only, moved rigidly with the particles by the step
+    handler, so the ledger's k_d/2 (|s| - L0)^2 per spring is their potential
+    and the total drifts only by the velocity-Verlet error: a quarter when dt
+    halves, in stretching and in a general motion with spin. The normal term
+    counted twice (a planted mistake) leaves a drift that does not shrink."""
+    v = np.array([[0.05, 0.02, 0.0], [-0.05, -0.02, 0.01]])
+    w = np.array([[0.0, 0.0, 2e6], [1e6, 0.0, 0.0]])
+    rows = [legacy_pair_energy(dt, 2e-10, v, w) for dt in (1e-14, 5e-15)]
+    assert rows[0][:, 2].max() > 0.5 * rows[0][0, 7]  # the bond took up most of the energy
+    d = [relative_drift(e, e[:, 7]) for e in rows]
+    print(f"legacy bond: relative drift {d[0]:.3e} at dt, {d[1]:.3e} at dt/2")
+    assert d[0] < 2e-5
+    assert 3.6 < d[0] / d[1] < 4.4
+    dp = [relative_drift(e, e[:, 7] + e[:, 2]) for e in rows]
+    assert dp[0] / dp[1] < 1.5
