@@ -85,12 +85,19 @@ def legacy_sinter_frames(x0, r_part, critical_separation, next_unit_vector):
     errors = []
 
     def trampoline(_user, out):
+        if errors:  # the source failed: end every further draw at once (x or y
+            # axis in turn, never both collinear with one normal); the frames
+            # are discarded and the error raised after the call
+            k = len(errors) % 2
+            errors.append(None)
+            out[0], out[1], out[2] = float(k == 0), float(k == 1), 0.0
+            return
         try:
             v = next_unit_vector()
             out[0], out[1], out[2] = float(v[0]), float(v[1]), float(v[2])
         except Exception as e:  # noqa: BLE001  (reported after the call)
             errors.append(e)
-            out[0] = out[1] = out[2] = 0.0
+            out[0], out[1], out[2] = 1.0, 0.0, 0.0
 
     cb = _UNIT_VECTOR_FN(trampoline)
     st = _abi.lib().hg_legacy_sinter_frames(x0.shape[0], _abi.dptr(x0), r_part, critical_separation,
