@@ -655,6 +655,10 @@ using namespace hg;
 struct hg_pool {
     hg_params p{};
     size_t P = 0, S = 0, MF = 0;
+    // per-particle energy-ledger terms (hg_pool_energy), grown to the largest
+    // system asked for
+    double *energy = nullptr;
+    size_t energy_cap = 0;
     hipStream_t stream = nullptr;
     double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
     uint32_t cap_n = 0, *nb = nullptr, *ncnt = nullptr;
@@ -1446,8 +1450,18 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
     const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, tab = Q->cap_s != 0;
     if (nl && M.rebuild)
         return fail(HG_ERR_INVALID, "hg_pool_energy: the system is listed by its first step; step it first");
-    double *terms = nullptr;
-    HG_HIP(hipMalloc(reinterpret_cast<void **>(&terms), size_t(8) * M.n * sizeof(double)));
+    if (M.n > Q->energy_cap) {
+        if (Q->energy) {
+            auto it = std::find(Q->allocations.begin(), Q->allocations.end(), static_cast<void *>(Q->energy));
+            if (it != Q->allocations.end()) Q->allocations.erase(it);
+            HG_HIP(hipFree(Q->energy));
+            Q->energy = nullptr;
+            Q->energy_cap = 0;
+        }
+        PG_TRY(palloc(Q, &Q->energy, size_t(8) * M.n));
+        Q->energy_cap = M.n;
+    }
+    double *terms = Q->energy;
     energy_view E{Q->x,
                   Q->v,
                   Q->w,
@@ -1468,9 +1482,7 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
     if (e == hipSuccess)
         e = hipMemcpyAsync(h.data(), terms, h.size() * sizeof(double), hipMemcpyDeviceToHost, Q->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(Q->stream);
-    const hipError_t ef = hipFree(terms);
     HG_HIP(e);
-    HG_HIP(ef);
     sum_energy(h.data(), M.n, out);
     return HG_OK;
 }
