@@ -167,6 +167,14 @@ hipError_t launch_final_kick(double *v, double *w, const double *a, const double
                              facet_dev *facets, int nf, double facet_dt, hipStream_t s);
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
 hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
+// nsteps steps of a system of at most 256 particles (FP64) in one workgroup
+// (see k_small_steps); `even`/`odd` are the views whose pair-state tables the
+// even/odd steps read.
+constexpr uint32_t kSmallSystemMax = 256;
+hipError_t launch_small_steps(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
+                              double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
+                              bool final_kick, double half, double dt, int nf_advance, double *lconn,
+                              const uint32_t *bcnt, hipStream_t s);
 // sinter_step_handler's connector moves after a drift (x and v already
 // advanced): each of particle i's connectors follows dx = v_i dt, then turns
 // by dtheta = w_i dt about the new x_i.

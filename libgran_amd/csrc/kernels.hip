@@ -27,9 +27,7 @@ namespace {
 // Neighbour-list acceleration pass: one lane per particle i, partners in the
 // list's ascending order, so the per-particle sum has the reference's order.
 template <int BIN, int HAM>
-__global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S.n) return;
+__device__ __forceinline__ void force_nl_one(const hg_params &P, const sys_view &S, uint32_t i) {
     const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     acc6 acc{zero3(), zero3()};
     walker_t<false> W;
@@ -44,6 +42,13 @@ __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
     st3(S.al, i, acc.ang);
+}
+
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    force_nl_one<BIN, HAM>(P, S, i);
 }
 
 // Mixed-precision neighbour-list pass (HG_PRECISION_MIXED, dem_mixed.h).
@@ -70,9 +75,16 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
 // order. Partner state is staged through LDS one tile per block-wide step;
 // all lanes read the same LDS word, a broadcast with no bank conflict.
 constexpr int kTile = 256;
+struct tile_lds {
+    double sx[3][kTile], sv[3][kTile], sw[3][kTile];
+};
+// The pass for particle i = blockIdx.x * blockDim.x + threadIdx.x; every
+// thread of the block takes part (it stages tiles), live or not.
 template <int BIN, int HAM>
-__global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
-    __shared__ double sx[3][kTile], sv[3][kTile], sw[3][kTile];
+__device__ __forceinline__ void force_all_block(const hg_params &P, const sys_view &S, tile_lds &T) {
+    auto &sx = T.sx;
+    auto &sv = T.sv;
+    auto &sw = T.sw;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < S.n;
     d3 xi = zero3(), vi = zero3(), wi = zero3();
@@ -116,16 +128,18 @@ __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
     st3(S.al, i, acc.ang);
 }
 
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
+    __shared__ tile_lds T;
+    force_all_block<BIN, HAM>(P, S, T);
+}
+
 // Kick (completing the previous step when `pending`), kick, drift:
 //   v += a*(dt/2); w += alpha*(dt/2); x += v*dt; theta += w*dt.
 // Lane 0 of block 0 advances the facets when a previous step completes here.
-__global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double *th, double *w, const double *a,
-                                                    const double *al, uint32_t n, int pending, double half_prev,
-                                                    double half, double dt, facet_dev *facets, int nf,
-                                                    double facet_dt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pending && i == 0) advance_facets(facets, nf, facet_dt);
-    if (i >= n) return;
+__device__ __forceinline__ void kick_drift_one(double *x, double *v, double *th, double *w, const double *a,
+                                               const double *al, uint32_t i, int pending, double half_prev,
+                                               double half, double dt) {
     d3 vi = ld3(v, i), wi = ld3(w, i);
     const d3 ai = ld3(a, i), ali = ld3(al, i);
     if (pending) {
@@ -140,14 +154,22 @@ __global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double
     st3(w, i, wi);
 }
 
+__global__ void __launch_bounds__(256) k_kick_drift(double *x, double *v, double *th, double *w, const double *a,
+                                                    const double *al, uint32_t n, int pending, double half_prev,
+                                                    double half, double dt, facet_dev *facets, int nf,
+                                                    double facet_dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pending && i == 0) advance_facets(facets, nf, facet_dt);
+    if (i >= n) return;
+    kick_drift_one(x, v, th, w, a, al, i, pending, half_prev, half, dt);
+}
+
 // sinter_step_handler (sinter_bridge.h:24-42, 55-75) after a drift: each of
 // particle i's connectors moves by dx = v_i dt (increment_x), then turns by
 // dtheta = w_i dt about the new centre (increment_theta). v, w and x are the
 // values the drift just used and wrote, so dx and dtheta are its increments.
-__global__ void __launch_bounds__(256) k_legacy_move(const double *x, const double *v, const double *w, uint32_t n,
-                                                     double dt, double *lconn, const uint32_t *bcnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void legacy_move_one(const double *x, const double *v, const double *w, uint32_t n,
+                                                uint32_t i, double dt, double *lconn, const uint32_t *bcnt) {
     const d3 dx = ld3(v, i) * dt, dth = ld3(w, i) * dt, xi = ld3(x, i);
     const uint32_t c = bcnt[i];
     for (uint32_t s = 0; s < c; s++) {
@@ -163,6 +185,13 @@ __global__ void __launch_bounds__(256) k_legacy_move(const double *x, const doub
     }
 }
 
+__global__ void __launch_bounds__(256) k_legacy_move(const double *x, const double *v, const double *w, uint32_t n,
+                                                     double dt, double *lconn, const uint32_t *bcnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    legacy_move_one(x, v, w, n, i, dt, lconn, bcnt);
+}
+
 __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const double *a, const double *al,
                                                     uint32_t n, double half, facet_dev *facets, int nf,
                                                     double facet_dt) {
@@ -171,6 +200,48 @@ __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const 
     if (i >= n) return;
     st3(v, i, ld3(v, i) + ld3(a, i) * half);
     st3(w, i, ld3(w, i) + ld3(al, i) * half);
+}
+
+// nsteps steps of a system of at most kTile particles in one workgroup, one
+// particle per thread: per step the kick/drift (with the legacy connector
+// moves), a barrier, the acceleration pass, a barrier, as the per-step
+// launches of k_kick_drift, k_legacy_move and k_force_nl / k_force_all do it,
+// through the same device code. The pass reads pair-state table `even` on
+// even steps and `odd` on odd ones (the two halves swap every pass).
+// pending_first: the first kick also completes a previous step; final_kick:
+// the last step's closing kick is applied (otherwise it is left pending, as
+// between the per-step launches).
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(kTile) k_small_steps(hg_params P, sys_view even, sys_view odd, double *x, double *v,
+                                                       double *th, double *w, facet_dev *facets, uint32_t nsteps,
+                                                       int pending_first, int final_kick, double half, double dt,
+                                                       int nf_advance, double *lconn, const uint32_t *bcnt) {
+    __shared__ tile_lds T;
+    const uint32_t i = threadIdx.x;
+    const uint32_t n = even.n;
+    const bool live = i < n;
+    for (uint32_t s = 0; s < nsteps; s++) {
+        const int pending = (s > 0 || pending_first) ? 1 : 0;
+        if (pending && i == 0) advance_facets(facets, nf_advance, dt);
+        if (live) {
+            kick_drift_one(x, v, th, w, even.a, even.al, i, pending, half, half, dt);
+            if (lconn) legacy_move_one(x, v, w, n, i, dt, lconn, bcnt);
+        }
+        __syncthreads();
+        if (P.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
+            if (live) force_nl_one<BIN, HAM>(P, (s & 1u) ? odd : even, i);
+        } else {
+            force_all_block<BIN, HAM>(P, (s & 1u) ? odd : even, T);
+        }
+        __syncthreads();
+    }
+    if (final_kick) {
+        if (i == 0) advance_facets(facets, nf_advance, dt);
+        if (live) {
+            st3(v, i, ld3(v, i) + ld3(even.a, i) * half);
+            st3(w, i, ld3(w, i) + ld3(even.al, i) * half);
+        }
+    }
 }
 
 // triangular_facet::update_positions with the driver's velocity (one lane).
@@ -500,6 +571,45 @@ hipError_t launch_force_t(const hg_params &p, const sys_view &sv, hipStream_t s)
 hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
 #define HG_CASE(B, H)                                                       \
     if (p.binary_model == B && p.hamaker_model == H) return launch_force_t<B, H>(p, sv, s);
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_NONE, HG_HAMAKER_SIMPLIFIED)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_CONTACT, HG_HAMAKER_SIMPLIFIED)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_SINTER, HG_HAMAKER_SIMPLIFIED)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_NONE)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_FULL)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_LIMITED)
+    HG_CASE(HG_BINARY_SINTER_LEGACY, HG_HAMAKER_SIMPLIFIED)
+#undef HG_CASE
+    return hipErrorInvalidValue;
+}
+
+template <int BIN, int HAM>
+hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
+                                double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
+                                bool final_kick, double half, double dt, int nf_advance, double *lconn,
+                                const uint32_t *bcnt, hipStream_t s) {
+    k_small_steps<BIN, HAM><<<1, kTile, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps, pending_first ? 1 : 0,
+                                                final_kick ? 1 : 0, half, dt, nf_advance, lconn, bcnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_small_steps(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
+                              double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
+                              bool final_kick, double half, double dt, int nf_advance, double *lconn,
+                              const uint32_t *bcnt, hipStream_t s) {
+    if (even.n == 0 || even.n > uint32_t(kTile) || p.precision != HG_PRECISION_FP64) return hipErrorInvalidValue;
+#define HG_CASE(B, H)                                                                                            \
+    if (p.binary_model == B && p.hamaker_model == H)                                                             \
+        return launch_small_steps_t<B, H>(p, even, odd, x, v, th, w, facets, nsteps, pending_first, final_kick, \
+                                          half, dt, nf_advance, lconn, bcnt, s);
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)

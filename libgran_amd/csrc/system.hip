@@ -490,6 +490,31 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
     const uint32_t n = uint32_t(S->n);
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     const int nf_advance = S->p.facet_advance ? 0 : S->nf;  // facets the step moves itself
+    if (n <= kSmallSystemMax && S->p.precision == HG_PRECISION_FP64 && !S->profiling) {
+        // small systems: the steps between list rebuilds in one launch of one
+        // workgroup instead of three dependent launches per step
+        if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
+            return fail(HG_ERR_INVALID, "the legacy sinter_functor's connectors are not set (hg_set_legacy_connectors)");
+        uint64_t s = 0;
+        while (s < nsteps) {
+            if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
+                HG_HIP(launch_neighbor_build(*S, S->stream));
+            uint64_t len = nsteps - s;
+            if (nl && rebuild_every) len = std::min<uint64_t>(len, rebuild_every - S->step_index % rebuild_every);
+            len = std::min<uint64_t>(len, uint64_t(1) << 30);
+            const sys_view even = make_view(S);
+            S->cur ^= 1;
+            const sys_view odd = make_view(S);
+            S->cur ^= 1;
+            HG_HIP(launch_small_steps(S->p, even, odd, S->x, S->v, S->th, S->w, S->facets, uint32_t(len), s > 0,
+                                      s + len == nsteps, half, dt, nf_advance, S->lconn, S->bcnt, S->stream));
+            if (S->p.binary_model != HG_BINARY_NONE) S->cur ^= int(len & 1u);
+            for (uint64_t k = 0; k < len; k++) S->t = S->t + dt;
+            S->step_index += len;
+            s += len;
+        }
+        return check_flags(S, "hg_step");
+    }
     for (uint64_t s = 0; s < nsteps; s++) {
         if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
             HG_HIP(timed(S, PHASE_REBUILD, [&] { return launch_neighbor_build(*S, S->stream); }));
