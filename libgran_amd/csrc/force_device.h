@@ -256,6 +256,52 @@ __device__ __forceinline__ void nl_partners(const IDS &ids, LOAD &&load, FOLD &&
     }
 }
 
+// The binary functor of one ordered pair: alt_sinter's bonded model for a
+// bonded pair, the contact functor otherwise; keep says whether the pair's
+// springs stay in the table; neck() gives the bond's neck strength (read only
+// when bonds break).
+template <int BIN, class NECK>
+__device__ __forceinline__ acc6 pair_binary(const hg_params &P, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj,
+                                            spring9 &sp, bool bonded, bool &keep, NECK &&neck) {
+    acc6 b;
+    if (BIN == HG_BINARY_SINTER && bonded) {
+        d3 ftot;
+        b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot);
+        // alt_breaking_sinter_bridge.h:117-119: (i, j) stops being bonded
+        // once its force exceeds the neck strength; dropping the entry
+        // leaves the contact functor a pair with zero springs, as the
+        // reference's untouched contact table holds.
+        keep = !(P.sinter_breaking && norm(ftot) > neck());
+    } else {
+        b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
+    }
+    return b;
+}
+
+template <int HAM>
+__device__ __forceinline__ acc6 pair_hamaker(const hg_params &P, d3 xi, d3 xj) {
+    acc6 h{zero3(), zero3()};
+    if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
+    if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
+    if constexpr (HAM == HG_HAMAKER_SIMPLIFIED) h = simplified_hamaker_eval(P.hamaker, xi, xj);
+    return h;
+}
+
+// The pair's term (the binary container's fold of its functors,
+// operator_overloads.h:10-20) added to the particle's sum.
+template <int BIN, int HAM>
+__device__ __forceinline__ acc6 pair_term(const acc6 &b, const acc6 &h) {
+    if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) return b + h;
+    else if constexpr (BIN != HG_BINARY_NONE) return b;
+    else return h;
+}
+template <int BIN, int HAM>
+__device__ __forceinline__ void fold_pair(acc6 &acc, const acc6 &b, const acc6 &h) {
+    const acc6 e = pair_term<BIN, HAM>(b, h);
+    acc.lin = acc.lin + e.lin;
+    acc.ang = acc.ang + e.ang;
+}
+
 // One ordered pair through the binary container fold
 // (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
 template <int BIN, int HAM, bool EAGER>
@@ -282,28 +328,11 @@ __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi,
         bool bonded = false;
         W.take(j, sp, bonded);
         bool keep;
-        if (BIN == HG_BINARY_SINTER && bonded) {
-            d3 ftot;
-            b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot);
-            // alt_breaking_sinter_bridge.h:117-119: (i, j) stops being bonded
-            // once its force exceeds the neck strength; dropping the entry
-            // leaves the contact functor a pair with zero springs, as the
-            // reference's untouched contact table holds.
-            keep = !(P.sinter_breaking && norm(ftot) > W.neck(j));
-        } else {
-            b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
-        }
+        b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, [&] { return W.neck(j); });
         if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
     }
-    if constexpr (HAM == HG_HAMAKER_FULL) h = hamaker_eval(P.hamaker, xi, xj);
-    if constexpr (HAM == HG_HAMAKER_LIMITED) h = lr_hamaker_eval(P.hamaker, xi, xj);
-    if constexpr (HAM == HG_HAMAKER_SIMPLIFIED) h = simplified_hamaker_eval(P.hamaker, xi, xj);
-    acc6 e;
-    if constexpr (BIN != HG_BINARY_NONE && HAM != HG_HAMAKER_NONE) e = b + h;
-    else if constexpr (BIN != HG_BINARY_NONE) e = b;
-    else e = h;
-    acc.lin = acc.lin + e.lin;
-    acc.ang = acc.ang + e.ang;
+    h = pair_hamaker<HAM>(P, xi, xj);
+    fold_pair<BIN, HAM>(acc, b, h);
 }
 
 // Unary container over facets (unary_force_container.h:20-32, a right fold),

@@ -129,6 +129,7 @@ struct hg_system_impl {
     // reductions
     double *red = nullptr;
     double *energy = nullptr;  // per-particle ledger terms, 8 per particle (allocated on first use)
+    double *tiny = nullptr;    // per-pair springs of the pair-parallel pass (n*n*kSpringRec, on first use)
     // host copies of the topology
     std::vector<std::pair<uint32_t, uint32_t>> bonds;
     size_t bonds_skipped = 0;
@@ -169,12 +170,14 @@ hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
 hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
 // nsteps steps of a system of at most 256 particles (FP64) in one workgroup
 // (see k_small_steps); `even`/`odd` are the views whose pair-state tables the
-// even/odd steps read.
-constexpr uint32_t kSmallSystemMax = 256;
+// even/odd steps read. All-pairs systems of at most kTinySystemMax particles
+// evaluate their pairs in parallel (k_tiny_steps) when `scratch` holds
+// n*n*kSpringRec doubles.
+constexpr uint32_t kSmallSystemMax = 256, kTinySystemMax = 27;
 hipError_t launch_small_steps(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
                               double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
                               bool final_kick, double half, double dt, int nf_advance, double *lconn,
-                              const uint32_t *bcnt, hipStream_t s);
+                              const uint32_t *bcnt, double *scratch, hipStream_t s);
 // sinter_step_handler's connector moves after a drift (x and v already
 // advanced): each of particle i's connectors follows dx = v_i dt, then turns
 // by dtheta = w_i dt about the new x_i.

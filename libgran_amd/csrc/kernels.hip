@@ -591,11 +591,116 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+// nsteps steps of a system of at most kTinyMax particles, all pairs, in one
+// workgroup of kTinyThreads with one thread per ordered pair: each pair's
+// functors evaluated at once (its old springs looked up in the pair-state
+// row of i), its term kept in LDS and its new springs in `scratch`
+// (kSpringRec doubles per pair: the 9 components, then keep | bonded << 1);
+// then one thread per particle adds its terms in ascending j and writes its
+// new row in ascending j, as the walker of the per-particle pass does. So the
+// sums and tables are those of k_force_all. Contact, alt_sinter (bonds that do
+// not break) or no binary functor.
+constexpr uint32_t kTinyMax = 27, kTinyThreads = 768;
+
+template <int BIN, int HAM>
+__global__ void __launch_bounds__(kTinyThreads)
+    k_tiny_steps(hg_params P, sys_view even, sys_view odd, double *x, double *v, double *th, double *w,
+                 facet_dev *facets, uint32_t nsteps, int pending_first, int final_kick, double half, double dt,
+                 int nf_advance, double *scratch) {
+    __shared__ double terms[kTinyMax * kTinyMax * 6];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = even.n;
+    for (uint32_t s = 0; s < nsteps; s++) {
+        const int pending = (s > 0 || pending_first) ? 1 : 0;
+        if (pending && t == 0) advance_facets(facets, nf_advance, dt);
+        if (t < n) kick_drift_one(x, v, th, w, even.a, even.al, t, pending, half, half, dt);
+        __syncthreads();
+        const sys_view &S = (s & 1u) ? odd : even;
+        if (t < n * n) {
+            const uint32_t i = t / n, j = t % n;
+            if (i != j) {
+                const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
+                const d3 xj = ld3(S.x, j), vj = ld3(S.v, j), wj = ld3(S.w, j);
+                acc6 b{zero3(), zero3()};
+                spring9 sp = zero_spring();
+                bool bonded = false, keep = false;
+                if constexpr (BIN != HG_BINARY_NONE) {
+                    const uint32_t cnt = S.pt.pcnt_old[i];
+                    for (uint32_t q = 0; q < cnt; q++) {
+                        const uint32_t tag = S.pt.pj_old[size_t(q) * n + i];
+                        if ((tag & kIdxMask) == j) {
+                            sp = ld_rec(S.pt.ps_old + (size_t(q) * n + i) * kSpringRec);
+                            bonded = (tag & kBondBit) != 0;
+                        }
+                    }
+                    b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, [] { return 0.0; });
+                }
+                const acc6 e = pair_term<BIN, HAM>(b, pair_hamaker<HAM>(P, xi, xj));
+                double *tm = terms + size_t(t) * 6;
+                tm[0] = e.lin.x; tm[1] = e.lin.y; tm[2] = e.lin.z;
+                tm[3] = e.ang.x; tm[4] = e.ang.y; tm[5] = e.ang.z;
+                if constexpr (BIN != HG_BINARY_NONE) {
+                    double *r = scratch + size_t(t) * kSpringRec;
+                    st_rec(r, sp);
+                    r[9] = double((keep ? 1 : 0) | (bonded ? 2 : 0));
+                }
+            }
+        }
+        __syncthreads();
+        if (t < n) {
+            const uint32_t i = t;
+            acc6 acc{zero3(), zero3()};
+            uint32_t q = 0;
+            for (uint32_t j = 0; j < n; j++) {
+                if (j == i) continue;
+                const double *tm = terms + size_t(i * n + j) * 6;
+                acc.lin = acc.lin + d3{tm[0], tm[1], tm[2]};
+                acc.ang = acc.ang + d3{tm[3], tm[4], tm[5]};
+                if constexpr (BIN != HG_BINARY_NONE) {
+                    const double *r = scratch + size_t(i * n + j) * kSpringRec;
+                    const int flags = int(r[9]);
+                    if (flags & 1) {
+                        if (q < S.pt.cap) {
+                            S.pt.pj_new[size_t(q) * n + i] = j | ((flags & 2) ? kBondBit : 0u);
+                            st_rec(S.pt.ps_new + (size_t(q) * n + i) * kSpringRec, ld_rec(r));
+                        }
+                        q++;
+                    }
+                }
+            }
+            if constexpr (BIN != HG_BINARY_NONE) {
+                S.pt.pcnt_new[i] = q < S.pt.cap ? q : S.pt.cap;
+                if (q > S.pt.cap) atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
+                atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(q));
+            }
+            if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, ld3(S.x, i), ld3(S.v, i), ld3(S.w, i));
+            st3(S.a, i, acc.lin);
+            st3(S.al, i, acc.ang);
+        }
+        __syncthreads();
+    }
+    if (final_kick) {
+        if (t == 0) advance_facets(facets, nf_advance, dt);
+        if (t < n) {
+            st3(v, t, ld3(v, t) + ld3(even.a, t) * half);
+            st3(w, t, ld3(w, t) + ld3(even.al, t) * half);
+        }
+    }
+}
+
 template <int BIN, int HAM>
 hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
                                 double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
                                 bool final_kick, double half, double dt, int nf_advance, double *lconn,
-                                const uint32_t *bcnt, hipStream_t s) {
+                                const uint32_t *bcnt, double *scratch, hipStream_t s) {
+    if constexpr (BIN != HG_BINARY_SINTER_LEGACY) {
+        if (scratch && even.n <= kTinyMax && p.pair_mode == HG_PAIRS_ALL && !p.sinter_breaking) {
+            k_tiny_steps<BIN, HAM><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
+                                                              pending_first ? 1 : 0, final_kick ? 1 : 0, half, dt,
+                                                              nf_advance, scratch);
+            return hipGetLastError();
+        }
+    }
     k_small_steps<BIN, HAM><<<1, kTile, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps, pending_first ? 1 : 0,
                                                 final_kick ? 1 : 0, half, dt, nf_advance, lconn, bcnt);
     return hipGetLastError();
@@ -604,12 +709,12 @@ hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const 
 hipError_t launch_small_steps(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
                               double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
                               bool final_kick, double half, double dt, int nf_advance, double *lconn,
-                              const uint32_t *bcnt, hipStream_t s) {
+                              const uint32_t *bcnt, double *scratch, hipStream_t s) {
     if (even.n == 0 || even.n > uint32_t(kTile) || p.precision != HG_PRECISION_FP64) return hipErrorInvalidValue;
 #define HG_CASE(B, H)                                                                                            \
     if (p.binary_model == B && p.hamaker_model == H)                                                             \
         return launch_small_steps_t<B, H>(p, even, odd, x, v, th, w, facets, nsteps, pending_first, final_kick, \
-                                          half, dt, nf_advance, lconn, bcnt, s);
+                                          half, dt, nf_advance, lconn, bcnt, scratch, s);
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_CASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)

@@ -495,6 +495,7 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
         // workgroup instead of three dependent launches per step
         if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
             return fail(HG_ERR_INVALID, "the legacy sinter_functor's connectors are not set (hg_set_legacy_connectors)");
+        if (!nl && n <= kTinySystemMax && !S->tiny) HG_TRY(dalloc(S, &S->tiny, size_t(n) * n * kSpringRec));
         uint64_t s = 0;
         while (s < nsteps) {
             if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
@@ -507,7 +508,8 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
             const sys_view odd = make_view(S);
             S->cur ^= 1;
             HG_HIP(launch_small_steps(S->p, even, odd, S->x, S->v, S->th, S->w, S->facets, uint32_t(len), s > 0,
-                                      s + len == nsteps, half, dt, nf_advance, S->lconn, S->bcnt, S->stream));
+                                      s + len == nsteps, half, dt, nf_advance, S->lconn, S->bcnt, S->tiny,
+                                      S->stream));
             if (S->p.binary_model != HG_BINARY_NONE) S->cur ^= int(len & 1u);
             for (uint64_t k = 0; k < len; k++) S->t = S->t + dt;
             S->step_index += len;
