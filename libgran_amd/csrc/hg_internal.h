@@ -170,9 +170,10 @@ hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
 hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
 // nsteps steps of a system of at most 256 particles (FP64) in one workgroup
 // (see k_small_steps); `even`/`odd` are the views whose pair-state tables the
-// even/odd steps read. All-pairs systems of at most kTinySystemMax particles
-// evaluate their pairs in parallel (k_tiny_steps) when `scratch` holds
-// n*n*kSpringRec doubles.
+// even/odd steps read. Systems of at most kTinySystemMax particles (all
+// pairs, or lists of capacity cap_n with n*cap_n <= 768) evaluate their pairs
+// in parallel (k_tiny_steps) when `scratch` holds n*max(n, cap_n)*kSpringRec
+// doubles.
 constexpr uint32_t kSmallSystemMax = 256, kTinySystemMax = 27;
 hipError_t launch_small_steps(const hg_params &p, const sys_view &even, const sys_view &odd, double *x, double *v,
                               double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,

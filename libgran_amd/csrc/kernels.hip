@@ -591,34 +591,47 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
-// nsteps steps of a system of at most kTinyMax particles, all pairs, in one
-// workgroup of kTinyThreads with one thread per ordered pair: each pair's
-// functors evaluated at once (its old springs looked up in the pair-state
-// row of i), its term kept in LDS and its new springs in `scratch`
-// (kSpringRec doubles per pair: the 9 components, then keep | bonded << 1);
-// then one thread per particle adds its terms in ascending j and writes its
-// new row in ascending j, as the walker of the per-particle pass does. So the
-// sums and tables are those of k_force_all. Contact, alt_sinter (bonds that do
-// not break) or no binary functor.
+// nsteps steps of a system of at most kTinyMax particles in one workgroup of
+// kTinyThreads, one thread per ordered pair (all pairs, or particle i's
+// listed partner slot k): each pair's functors evaluated at once (its old
+// springs looked up in the pair-state row of i), its term kept in LDS and its
+// new springs in `scratch` (kSpringRec doubles per pair: the 9 components,
+// then keep | bonded << 1). Then one thread per particle adds its terms in
+// partner order and writes its new row as the walker of the per-particle
+// pass does: old entries of partners not evaluated are carried in place, an
+// evaluated partner's entry is replaced by its kept springs. So the sums and
+// tables are those of k_force_all / k_force_nl. Contact, alt_sinter (bonds
+// that do not break) or no binary functor.
 constexpr uint32_t kTinyMax = 27, kTinyThreads = 768;
 
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool NL>
 __global__ void __launch_bounds__(kTinyThreads)
     k_tiny_steps(hg_params P, sys_view even, sys_view odd, double *x, double *v, double *th, double *w,
                  facet_dev *facets, uint32_t nsteps, int pending_first, int final_kick, double half, double dt,
                  int nf_advance, double *scratch) {
-    __shared__ double terms[kTinyMax * kTinyMax * 6];
+    __shared__ double terms[kTinyThreads * 6];
     const uint32_t t = threadIdx.x;
     const uint32_t n = even.n;
+    constexpr bool nl = NL;
+    const uint32_t slots = nl ? even.cap_n : n;  // pair threads per particle
     for (uint32_t s = 0; s < nsteps; s++) {
         const int pending = (s > 0 || pending_first) ? 1 : 0;
         if (pending && t == 0) advance_facets(facets, nf_advance, dt);
         if (t < n) kick_drift_one(x, v, th, w, even.a, even.al, t, pending, half, half, dt);
         __syncthreads();
         const sys_view &S = (s & 1u) ? odd : even;
-        if (t < n * n) {
-            const uint32_t i = t / n, j = t % n;
-            if (i != j) {
+        if (t < n * slots) {
+            const uint32_t i = t / slots, k = t % slots;
+            bool eval;
+            uint32_t j;
+            if (nl) {
+                eval = k < S.ncnt[i];
+                j = eval ? S.nb[size_t(k) * n + i] : 0u;
+            } else {
+                eval = k != i;
+                j = k;
+            }
+            if (eval) {
                 const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
                 const d3 xj = ld3(S.x, j), vj = ld3(S.v, j), wj = ld3(S.w, j);
                 acc6 b{zero3(), zero3()};
@@ -650,25 +663,40 @@ __global__ void __launch_bounds__(kTinyThreads)
         if (t < n) {
             const uint32_t i = t;
             acc6 acc{zero3(), zero3()};
-            uint32_t q = 0;
-            for (uint32_t j = 0; j < n; j++) {
-                if (j == i) continue;
-                const double *tm = terms + size_t(i * n + j) * 6;
+            uint32_t q = 0, p = 0;
+            const uint32_t cnt_old = BIN != HG_BINARY_NONE ? S.pt.pcnt_old[i] : 0u;
+            auto put = [&](uint32_t tag, const spring9 &sp) {
+                if (q < S.pt.cap) {
+                    S.pt.pj_new[size_t(q) * n + i] = tag;
+                    st_rec(S.pt.ps_new + (size_t(q) * n + i) * kSpringRec, sp);
+                }
+                q++;
+            };
+            auto carry_below = [&](uint32_t j) {  // old entries of partners before j
+                while (p < cnt_old && (S.pt.pj_old[size_t(p) * n + i] & kIdxMask) < j) {
+                    put(S.pt.pj_old[size_t(p) * n + i], ld_rec(S.pt.ps_old + (size_t(p) * n + i) * kSpringRec));
+                    p++;
+                }
+            };
+            const uint32_t m = nl ? S.ncnt[i] : n;
+            for (uint32_t k = 0; k < m; k++) {
+                const uint32_t j = nl ? S.nb[size_t(k) * n + i] : k;
+                if (!nl && j == i) continue;
+                const double *tm = terms + size_t(i * slots + k) * 6;
                 acc.lin = acc.lin + d3{tm[0], tm[1], tm[2]};
                 acc.ang = acc.ang + d3{tm[3], tm[4], tm[5]};
                 if constexpr (BIN != HG_BINARY_NONE) {
-                    const double *r = scratch + size_t(i * n + j) * kSpringRec;
-                    const int flags = int(r[9]);
-                    if (flags & 1) {
-                        if (q < S.pt.cap) {
-                            S.pt.pj_new[size_t(q) * n + i] = j | ((flags & 2) ? kBondBit : 0u);
-                            st_rec(S.pt.ps_new + (size_t(q) * n + i) * kSpringRec, ld_rec(r));
-                        }
-                        q++;
+                    if constexpr (NL) {  // with all pairs every partner is evaluated: nothing to carry
+                        carry_below(j);
+                        if (p < cnt_old && (S.pt.pj_old[size_t(p) * n + i] & kIdxMask) == j) p++;  // replaced
                     }
+                    const double *r = scratch + size_t(i * slots + k) * kSpringRec;
+                    const int flags = int(r[9]);
+                    if (flags & 1) put(j | ((flags & 2) ? kBondBit : 0u), ld_rec(r));
                 }
             }
             if constexpr (BIN != HG_BINARY_NONE) {
+                if constexpr (NL) carry_below(kNoEntry);
                 S.pt.pcnt_new[i] = q < S.pt.cap ? q : S.pt.cap;
                 if (q > S.pt.cap) atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
                 atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(q));
@@ -694,10 +722,16 @@ hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const 
                                 bool final_kick, double half, double dt, int nf_advance, double *lconn,
                                 const uint32_t *bcnt, double *scratch, hipStream_t s) {
     if constexpr (BIN != HG_BINARY_SINTER_LEGACY) {
-        if (scratch && even.n <= kTinyMax && p.pair_mode == HG_PAIRS_ALL && !p.sinter_breaking) {
-            k_tiny_steps<BIN, HAM><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
-                                                              pending_first ? 1 : 0, final_kick ? 1 : 0, half, dt,
-                                                              nf_advance, scratch);
+        const uint32_t slots = p.pair_mode == HG_PAIRS_NEIGHBOR_LIST ? even.cap_n : even.n;
+        if (scratch && even.n <= kTinyMax && even.n * slots <= kTinyThreads && !p.sinter_breaking) {
+            if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST)
+                k_tiny_steps<BIN, HAM, true><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
+                                                                        pending_first ? 1 : 0, final_kick ? 1 : 0,
+                                                                        half, dt, nf_advance, scratch);
+            else
+                k_tiny_steps<BIN, HAM, false><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
+                                                                         pending_first ? 1 : 0, final_kick ? 1 : 0,
+                                                                         half, dt, nf_advance, scratch);
             return hipGetLastError();
         }
     }

@@ -495,7 +495,7 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
         // workgroup instead of three dependent launches per step
         if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
             return fail(HG_ERR_INVALID, "the legacy sinter_functor's connectors are not set (hg_set_legacy_connectors)");
-        if (!nl && n <= kTinySystemMax && !S->tiny) HG_TRY(dalloc(S, &S->tiny, size_t(n) * n * kSpringRec));
+        if (n <= kTinySystemMax && !S->tiny) HG_TRY(dalloc(S, &S->tiny, size_t(n) * std::max(n, S->cap_n) * kSpringRec));
         uint64_t s = 0;
         while (s < nsteps) {
             if (nl && rebuild_every && (S->step_index % rebuild_every) == 0)
