@@ -160,6 +160,14 @@ hg_status hg_compute_accelerations(hg_system *sys);
  * rotational_velocity_verlet_half + rotational_step_handler
  * (test/contact.cpp:98-106). */
 hg_status hg_step(hg_system *sys, double dt, uint64_t nsteps, uint32_t rebuild_every);
+/* hg_step without waiting for the device: the steps are queued and the call
+ * returns. Table-capacity errors of the queued steps are reported by the next
+ * call that reads results or state (hg_get_state, hg_get_accelerations,
+ * hg_get_neighbors, hg_get_springs, hg_get_legacy_connectors, hg_diagnostics,
+ * hg_energy, hg_set_state, hg_step). Replaces do_step as a driver calls it,
+ * once per step with reads in between (Writerside/topics/Granular-system.md:65,
+ * test/contact.cpp:105). */
+hg_status hg_step_async(hg_system *sys, double dt, uint64_t nsteps, uint32_t rebuild_every);
 
 /* Copies the state out (any pointer may be NULL).
  * Replaces: get_x/get_v/get_theta/get_omega (Granular-system.md:65-226). */

@@ -194,6 +194,10 @@ class System:
     def step(self, dt, nsteps=1, rebuild_every=0):
         _abi.check(_abi.lib().hg_step(self._h, dt, nsteps, rebuild_every))
 
+    def step_async(self, dt, nsteps=1, rebuild_every=0):
+        """hg_step_async: queue the steps and return; errors surface at the next read."""
+        _abi.check(_abi.lib().hg_step_async(self._h, dt, nsteps, rebuild_every))
+
     def state(self):
         arrs = [np.zeros((self.n, 3)) for _ in range(4)]
         _abi.check(_abi.lib().hg_get_state(self._h, *[_abi.dptr(a) for a in arrs]))

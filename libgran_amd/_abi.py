@@ -86,6 +86,7 @@ SIGNATURES = {
     "hg_rebuild_neighbors": (C.c_int, [_sysp]),
     "hg_compute_accelerations": (C.c_int, [_sysp]),
     "hg_step": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
+    "hg_step_async": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
     "hg_get_state": (C.c_int, [_sysp, _dp, _dp, _dp, _dp]),
     "hg_set_state": (C.c_int, [_sysp, _dp, _dp, _dp, _dp]),
     "hg_get_accelerations": (C.c_int, [_sysp, _dp, _dp]),

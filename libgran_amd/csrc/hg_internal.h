@@ -95,6 +95,7 @@ struct hg_system_impl {
     // particle state, 4 doubles per particle per field
     double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
     bool acc_valid = false;
+    bool flags_unchecked = false;  // hg_step_async queued steps whose flags nobody has read yet
     double t = 0.0;
     uint64_t step_index = 0;
     // pair-state table (double buffered)

@@ -134,6 +134,7 @@ hg_status collect_profile(hg_system *S) {
 
 // Device flags -> status. Clears them for the next call.
 hg_status check_flags(hg_system *S, const char *where) {
+    S->flags_unchecked = false;
     HG_HIP(hipMemcpyAsync(S->flags_host, S->flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipStreamSynchronize(S->stream));
     if (!S->pending.empty()) {
@@ -153,6 +154,16 @@ hg_status check_flags(hg_system *S, const char *where) {
     }
     return HG_OK;
 }
+
+// Reports the flags of steps hg_step_async left unchecked, before a call
+// that reads results.
+hg_status settle(hg_system *S, const char *where) {
+    return S->flags_unchecked ? check_flags(S, where) : HG_OK;
+}
+
+// The steps of hg_step; `check` false leaves the flag read to the next call
+// that reads results (hg_step_async).
+hg_status step_impl(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_every, bool check);
 
 sys_view make_view(hg_system *S) {
     sys_view v;
@@ -481,6 +492,20 @@ hg_status hg_compute_accelerations(hg_system *S) {
 
 hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_every) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_step"));
+    return step_impl(S, dt, nsteps, rebuild_every, true);
+}
+
+hg_status hg_step_async(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_every) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    return step_impl(S, dt, nsteps, rebuild_every, false);
+}
+
+}  // extern "C"
+
+namespace hg {
+namespace {
+hg_status step_impl(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_every, bool check) {
     if (nsteps == 0) return HG_OK;
     if (!S->acc_valid) {
         HG_TRY(force_pass(S));
@@ -515,6 +540,10 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
             S->step_index += len;
             s += len;
         }
+        if (!check) {
+            S->flags_unchecked = true;
+            return HG_OK;
+        }
         return check_flags(S, "hg_step");
     }
     for (uint64_t s = 0; s < nsteps; s++) {
@@ -533,11 +562,20 @@ hg_status hg_step(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_eve
     HG_HIP(timed(S, PHASE_KICK, [&] {
         return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, nf_advance, dt, S->stream);
     }));
+    if (!check) {
+        S->flags_unchecked = true;
+        return HG_OK;
+    }
     return check_flags(S, "hg_step");
 }
+}  // namespace
+}  // namespace hg
+
+extern "C" {
 
 hg_status hg_get_state(hg_system *S, double *x, double *v, double *theta, double *omega) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_get_state"));
     HG_TRY(download3(S, S->x, x));
     HG_TRY(download3(S, S->v, v));
     HG_TRY(download3(S, S->th, theta));
@@ -547,6 +585,7 @@ hg_status hg_get_state(hg_system *S, double *x, double *v, double *theta, double
 
 hg_status hg_set_state(hg_system *S, const double *x, const double *v, const double *theta, const double *omega) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_set_state"));
     if (x) HG_TRY(upload3(S, S->x, x));
     if (v) HG_TRY(upload3(S, S->v, v));
     if (theta) HG_TRY(upload3(S, S->th, theta));
@@ -557,6 +596,7 @@ hg_status hg_set_state(hg_system *S, const double *x, const double *v, const dou
 
 hg_status hg_get_accelerations(hg_system *S, double *a, double *alpha) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_get_accelerations"));
     HG_TRY(download3(S, S->a, a));
     HG_TRY(download3(S, S->al, alpha));
     return HG_OK;
@@ -580,6 +620,7 @@ hg_status hg_get_time(hg_system *S, double *t, uint64_t *step_index) {
 
 hg_status hg_get_neighbors(hg_system *S, uint64_t *offsets, uint32_t *idx, size_t cap, size_t *total) {
     if (!S || !offsets) return fail(HG_ERR_INVALID, "system and offsets are required");
+    HG_TRY(settle(S, "hg_get_neighbors"));
     if (S->p.pair_mode != HG_PAIRS_NEIGHBOR_LIST) return fail(HG_ERR_INVALID, "system has no neighbour list");
     const size_t n = S->n;
     std::vector<uint32_t> nb(size_t(S->cap_n) * n), cnt(n);
@@ -650,6 +691,7 @@ hg_status hg_set_legacy_connectors(hg_system *S, size_t nbonds, const uint32_t *
 
 hg_status hg_get_legacy_connectors(hg_system *S, double *connectors, size_t cap, size_t *nbonds) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_get_legacy_connectors"));
     if (S->p.binary_model != HG_BINARY_SINTER_LEGACY)
         return fail(HG_ERR_INVALID, "hg_get_legacy_connectors: the system is not HG_BINARY_SINTER_LEGACY");
     if (nbonds) *nbonds = S->bonds.size();
@@ -680,6 +722,7 @@ hg_status hg_get_legacy_connectors(hg_system *S, double *connectors, size_t cap,
 
 hg_status hg_get_springs(hg_system *S, int which, uint32_t *ij, double *springs, size_t cap, size_t *total) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
+    HG_TRY(settle(S, "hg_get_springs"));
     if (total) *total = 0;
     if (S->p.binary_model == HG_BINARY_NONE) return HG_OK;
     const size_t n = S->n, cs = S->cap_s;
@@ -786,6 +829,7 @@ hg_status hg_set_neck_strengths(hg_system *S, const double *strengths) {
 
 hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out) {
     if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    HG_TRY(settle(S, "hg_diagnostics"));
     HG_HIP(launch_diagnostics(S->x, S->v, S->w, uint32_t(S->n), mass, inertia, S->red, S->stream));
     HG_HIP(hipMemcpyAsync(out, S->red, 9 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipStreamSynchronize(S->stream));
@@ -794,6 +838,7 @@ hg_status hg_diagnostics(hg_system *S, double mass, double inertia, double *out)
 
 hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
     if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    HG_TRY(settle(S, "hg_energy"));
     if (!S->energy) HG_TRY(dalloc(S, &S->energy, size_t(8) * S->n));
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, bin = S->p.binary_model != HG_BINARY_NONE;
     energy_view E{S->x,

@@ -139,7 +139,8 @@ public:
     }
 
     void do_step(real_t dt) {
-        check(hg_step(sys, double(dt), 1, 0), "do_step");
+        // queued: a capacity error surfaces at the next state read
+        check(hg_step_async(sys, double(dt), 1, 0), "do_step");
         t += dt;
         stale = true;
     }

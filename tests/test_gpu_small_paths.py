@@ -91,3 +91,31 @@ def test_small_paths_legacy_facets_and_larger_systems():
                            facet_hamaker=1, surface_contact=O.contact_params(mu=0.5),
                            surface_hamaker=O.hamaker_params(A=0.0, h0=1e-9)), xs, vs, 3000,
              facets=(verts, np.zeros((1, 3))))
+
+
+def test_step_async_equals_step_and_defers_errors():
+    """hg_step_async queued once per step, as a drop-in driver's do_step
+    does, gives the state of one hg_step over the same steps; a table
+    overflow that hg_step reports at once surfaces at the next state read."""
+    L = _L()
+    x, v = S.lattice_with_impactor()
+    p = L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=O.contact_params(),
+                      hamaker=O.hamaker_params(A=1e-20, h0=1e-9))
+    a, b = L.System(p, x, v), L.System(p, x, v)
+    a.step(1e-13, 400)
+    for _ in range(400):
+        b.step_async(1e-13, 1)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    x3 = np.array([[0.0, 0.0, 0.0], [1.9 * R, 0.0, 0.0], [-1.9 * R, 0.0, 0.0]])  # particle 0 touches two
+    p1 = L.make_params(L.BINARY_CONTACT, L.HAMAKER_NONE, contact=O.contact_params(), pair_capacity=1)
+    g = L.System(p1, x3, np.zeros_like(x3))
+    with pytest.raises(L.HipgranError) as e:
+        g.step(1e-13, 1)
+    assert e.value.status == 5  # HG_ERR_CAPACITY
+    g2 = L.System(p1, x3, np.zeros_like(x3))
+    g2.step_async(1e-13, 1)
+    with pytest.raises(L.HipgranError) as e:
+        g2.state()
+    assert e.value.status == 5
