@@ -236,6 +236,52 @@ def ledger_and_writer_leg(L, calls=20):
             "vtk_threads": min(16, os.cpu_count() or 1)}
 
 
+def small_systems_leg(L, steps=20000):
+    """The reference's own 19-particle scenario (test/hamaker.cpp: 18-particle
+    lattice and an impactor, contact + Hamaker, dt 1e-13 s) as a single FP64
+    system, all pairs and with the 5r neighbour list rebuilt every 20 steps
+    (test/hamaker_neighbor.cpp): microseconds per step with the whole call in
+    one launch per rebuild interval, with per-step launches (what hg_step
+    does while launch profiling is on), and with one queued step per call as
+    a drop-in driver's do_step makes them."""
+    import torch
+    x, v = [], []
+    for i in range(3):
+        y = -3.0 * R_PART + i * 2.0 * R_PART
+        for j in range(3):
+            xx = -3.0 * R_PART + j * 2.0 * R_PART
+            x += [[xx, y, 0.0], [xx, y, -2.0 * R_PART]]
+    x.append([0.0, 0.0, 3.0 * R_PART])
+    x = np.array(x)
+    v = np.zeros_like(x)
+    v[-1] = [1.0, 0.0, -10.0]
+    k, gn = 10000.0, 5.0e-9
+    contact = [k, gn, k, 0.2 * gn, 1.0, 1.0, k, 0.05 * gn, 0.1, 1.0, k, 0.05 * gn, 0.1, 1.0, R_PART, MASS, INERTIA, DT]
+    ham = [1.0e-20, 1.0e-9, 3.0 * R_PART, R_PART, MASS]
+    out = {"workload": "19-particle lattice + impactor, contact + Hamaker, FP64 single system", "steps": steps}
+    for mode, kw, every in (("all_pairs", {}, 0),
+                            ("neighbor_list", dict(pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=5.0 * R_PART), 20)):
+        p = L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=contact, hamaker=ham, **kw)
+        for name, profiled in (("per_step_launches_us", True), ("one_launch_us", False)):
+            s = L.System(p, x, v)
+            s.profile(profiled)
+            s.step(DT, 100, rebuild_every=every)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.step(DT, steps, rebuild_every=every)
+            torch.cuda.synchronize()
+            out[f"{mode}_{name}"] = round((time.perf_counter() - t0) / steps * 1e6, 2)
+            s.close()
+    s = L.System(L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=contact, hamaker=ham), x, v)
+    t0 = time.perf_counter()
+    for _ in range(steps // 4):
+        s.step_async(DT, 1)
+    s.state()
+    out["all_pairs_queued_step_per_call_us"] = round((time.perf_counter() - t0) / (steps // 4) * 1e6, 2)
+    s.close()
+    return out
+
+
 def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0, seed=7):
     """The serving stream: `count` plate tests (N log-uniform 100..20000, Df
     U(1.6, 2.2), plates at U(1, 10) m/s, compression or tension) arriving as
@@ -351,6 +397,9 @@ def main():
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
     ap.add_argument("--only-single", action="store_true", help="only the N=20,000 FP64 per-step path (profiling)")
+    ap.add_argument("--only-small", action="store_true",
+                    help="only the reference's 19-particle lattice scenarios (test/hamaker.cpp, hamaker_neighbor.cpp) "
+                         "as single systems: one launch per interval against per-step launches")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB device-to-device copy (known traffic for scaling HBM counters)")
     args = ap.parse_args()
@@ -375,6 +424,10 @@ def main():
     # as ROCm's, and the first one loaded serves the whole process.
     import torch  # noqa: F401  (torch.cuda.synchronize brackets the timed regions)
     import libgran_amd as L
+
+    if args.only_small:
+        print(json.dumps(small_systems_leg(L)))
+        return
 
     if args.only_single:
         # the plain FP64 per-step path on the N=20,000 aggregate alone (profiling runs)
