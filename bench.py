@@ -279,6 +279,33 @@ def small_systems_leg(L, steps=20000):
     s.state()
     out["all_pairs_queued_step_per_call_us"] = round((time.perf_counter() - t0) / (steps // 4) * 1e6, 2)
     s.close()
+    # test/sintered.cpp: the 3x3x3 touching lattice bonded by the legacy
+    # sinter_functor (54 bonds, five connectors each, k_d 1e4, undamped); the
+    # x_min column moves at +5 m/s in z, the x_max column at +5 m/s in y
+    xl, vl = [], []
+    for i in range(3):
+        for j in range(3):
+            xx, yy = (-1.0 + 2.0 * j) * R_PART, (-1.0 + 2.0 * i) * R_PART
+            xl += [[xx, yy, 0.0], [xx, yy, -2.0 * R_PART], [xx, yy, -4.0 * R_PART]]
+            vl += [[0.0, 0.0, 5.0] if j == 0 else ([0.0, 5.0, 0.0] if j == 2 else [0.0, 0.0, 0.0])] * 3
+    xl, vl = np.array(xl), np.array(vl)
+    vecs = np.random.default_rng(3).uniform(-1.0, 1.0, (20000, 3))
+    it = iter(vecs / np.linalg.norm(vecs, axis=1, keepdims=True))
+    pairs, conn = L.legacy_sinter_frames(xl, R_PART, 1.0e-9, lambda: next(it))
+    kd = 1.0e4
+    sinter = [kd, 0.0, kd, 0.0, kd, 0.0, kd, 0.0, R_PART, MASS, INERTIA, DT, 1.0e-9]
+    p = L.make_params(L.BINARY_SINTER_LEGACY, L.HAMAKER_NONE, contact=contact, sinter=sinter)
+    for name, profiled in (("per_step_launches_us", True), ("one_launch_us", False)):
+        s = L.System(p, xl, vl)
+        s.set_legacy_connectors(pairs, conn)
+        s.profile(profiled)
+        s.step(DT, 100)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(DT, steps)
+        torch.cuda.synchronize()
+        out[f"sintered27_legacy_{name}"] = round((time.perf_counter() - t0) / steps * 1e6, 2)
+        s.close()
     return out
 
 

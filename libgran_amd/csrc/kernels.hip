@@ -601,14 +601,15 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
 // pass does: old entries of partners not evaluated are carried in place, an
 // evaluated partner's entry is replaced by its kept springs. So the sums and
 // tables are those of k_force_all / k_force_nl. Contact, alt_sinter (bonds
-// that do not break) or no binary functor.
+// that do not break), the legacy sinter_functor (its connectors moved after
+// each drift, as k_legacy_move does) or no binary functor.
 constexpr uint32_t kTinyMax = 27, kTinyThreads = 768;
 
 template <int BIN, int HAM, bool NL>
 __global__ void __launch_bounds__(kTinyThreads)
     k_tiny_steps(hg_params P, sys_view even, sys_view odd, double *x, double *v, double *th, double *w,
                  facet_dev *facets, uint32_t nsteps, int pending_first, int final_kick, double half, double dt,
-                 int nf_advance, double *scratch) {
+                 int nf_advance, double *scratch, double *lconn, const uint32_t *bcnt) {
     __shared__ double terms[kTinyThreads * 6];
     const uint32_t t = threadIdx.x;
     const uint32_t n = even.n;
@@ -617,7 +618,10 @@ __global__ void __launch_bounds__(kTinyThreads)
     for (uint32_t s = 0; s < nsteps; s++) {
         const int pending = (s > 0 || pending_first) ? 1 : 0;
         if (pending && t == 0) advance_facets(facets, nf_advance, dt);
-        if (t < n) kick_drift_one(x, v, th, w, even.a, even.al, t, pending, half, half, dt);
+        if (t < n) {
+            kick_drift_one(x, v, th, w, even.a, even.al, t, pending, half, half, dt);
+            if constexpr (BIN == HG_BINARY_SINTER_LEGACY) legacy_move_one(x, v, w, n, t, dt, lconn, bcnt);
+        }
         __syncthreads();
         const sys_view &S = (s & 1u) ? odd : even;
         if (t < n * slots) {
@@ -637,7 +641,23 @@ __global__ void __launch_bounds__(kTinyThreads)
                 acc6 b{zero3(), zero3()};
                 spring9 sp = zero_spring();
                 bool bonded = false, keep = false;
-                if constexpr (BIN != HG_BINARY_NONE) {
+                uint32_t ls = kNoEntry;  // the legacy bond's slot in i's list
+                if constexpr (BIN == HG_BINARY_SINTER_LEGACY)
+                    for (uint32_t q = 0; q < S.pt.bcnt[i]; q++)
+                        if (S.pt.bj[size_t(q) * n + i] == j) ls = q;
+                if (ls != kNoEntry) {
+                    // sinter_bridge.h:222-223: its five connector springs; the
+                    // pair has no table entry (keep stays false)
+                    const double *ri = S.pt.lconn + (size_t(ls) * n + i) * kConnRec;
+                    const double *rj = S.pt.lconn + (size_t(S.pt.lslot[size_t(ls) * n + i]) * n + j) * kConnRec;
+                    d3 ci[5], cj[5];
+                    for (int q = 0; q < 5; q++) {
+                        ci[q] = d3{ri[3 * q], ri[3 * q + 1], ri[3 * q + 2]};
+                        cj[q] = d3{rj[3 * q], rj[3 * q + 1], rj[3 * q + 2]};
+                    }
+                    b = legacy_bonded_eval(P.sinter.k, P.sinter.gamma_n, P.contact.r_part, P.contact.mass,
+                                           P.contact.inertia, ci, cj, xi, xj, vi, vj, wi, wj);
+                } else if constexpr (BIN != HG_BINARY_NONE) {
                     const uint32_t cnt = S.pt.pcnt_old[i];
                     for (uint32_t q = 0; q < cnt; q++) {
                         const uint32_t tag = S.pt.pj_old[size_t(q) * n + i];
@@ -721,19 +741,17 @@ hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const 
                                 double *th, double *w, facet_dev *facets, uint32_t nsteps, bool pending_first,
                                 bool final_kick, double half, double dt, int nf_advance, double *lconn,
                                 const uint32_t *bcnt, double *scratch, hipStream_t s) {
-    if constexpr (BIN != HG_BINARY_SINTER_LEGACY) {
-        const uint32_t slots = p.pair_mode == HG_PAIRS_NEIGHBOR_LIST ? even.cap_n : even.n;
-        if (scratch && even.n <= kTinyMax && even.n * slots <= kTinyThreads && !p.sinter_breaking) {
-            if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST)
-                k_tiny_steps<BIN, HAM, true><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
-                                                                        pending_first ? 1 : 0, final_kick ? 1 : 0,
-                                                                        half, dt, nf_advance, scratch);
-            else
-                k_tiny_steps<BIN, HAM, false><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
-                                                                         pending_first ? 1 : 0, final_kick ? 1 : 0,
-                                                                         half, dt, nf_advance, scratch);
-            return hipGetLastError();
-        }
+    const uint32_t slots = p.pair_mode == HG_PAIRS_NEIGHBOR_LIST ? even.cap_n : even.n;
+    if (scratch && even.n <= kTinyMax && even.n * slots <= kTinyThreads && !p.sinter_breaking) {
+        if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST)
+            k_tiny_steps<BIN, HAM, true><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
+                                                                    pending_first ? 1 : 0, final_kick ? 1 : 0, half,
+                                                                    dt, nf_advance, scratch, lconn, bcnt);
+        else
+            k_tiny_steps<BIN, HAM, false><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
+                                                                     pending_first ? 1 : 0, final_kick ? 1 : 0, half,
+                                                                     dt, nf_advance, scratch, lconn, bcnt);
+        return hipGetLastError();
     }
     k_small_steps<BIN, HAM><<<1, kTile, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps, pending_first ? 1 : 0,
                                                 final_kick ? 1 : 0, half, dt, nf_advance, lconn, bcnt);
