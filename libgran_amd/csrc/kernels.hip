@@ -601,8 +601,9 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s) {
 // pass does: old entries of partners not evaluated are carried in place, an
 // evaluated partner's entry is replaced by its kept springs. So the sums and
 // tables are those of k_force_all / k_force_nl. Contact, alt_sinter (bonds
-// that do not break), the legacy sinter_functor (its connectors moved after
-// each drift, as k_legacy_move does) or no binary functor.
+// broken past their neck strength drop their entry), the legacy
+// sinter_functor (its connectors moved after each drift, as k_legacy_move
+// does) or no binary functor.
 constexpr uint32_t kTinyMax = 27, kTinyThreads = 768;
 
 template <int BIN, int HAM, bool NL>
@@ -666,7 +667,15 @@ __global__ void __launch_bounds__(kTinyThreads)
                             bonded = (tag & kBondBit) != 0;
                         }
                     }
-                    b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, [] { return 0.0; });
+                    // the bond's neck strength, as walker_t::neck finds it
+                    auto neck = [&] {
+                        double r = __builtin_huge_val();
+                        if (S.pt.bcnt)
+                            for (uint32_t q = 0; q < S.pt.bcnt[i]; q++)
+                                if (S.pt.bj[size_t(q) * n + i] == j) r = S.pt.bstr[size_t(q) * n + i];
+                        return r;
+                    };
+                    b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, neck);
                 }
                 const acc6 e = pair_term<BIN, HAM>(b, pair_hamaker<HAM>(P, xi, xj));
                 double *tm = terms + size_t(t) * 6;
@@ -742,7 +751,7 @@ hipError_t launch_small_steps_t(const hg_params &p, const sys_view &even, const 
                                 bool final_kick, double half, double dt, int nf_advance, double *lconn,
                                 const uint32_t *bcnt, double *scratch, hipStream_t s) {
     const uint32_t slots = p.pair_mode == HG_PAIRS_NEIGHBOR_LIST ? even.cap_n : even.n;
-    if (scratch && even.n <= kTinyMax && even.n * slots <= kTinyThreads && !p.sinter_breaking) {
+    if (scratch && even.n <= kTinyMax && even.n * slots <= kTinyThreads) {
         if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST)
             k_tiny_steps<BIN, HAM, true><<<1, kTinyThreads, 0, s>>>(p, even, odd, x, v, th, w, facets, nsteps,
                                                                     pending_first ? 1 : 0, final_kick ? 1 : 0, half,

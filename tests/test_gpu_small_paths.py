@@ -20,7 +20,7 @@ def _L():
     return libgran_amd
 
 
-def run_both(params, x, v, steps, rebuild_every=0, legacy=False, facets=None):
+def run_both(params, x, v, steps, rebuild_every=0, legacy=False, facets=None, necks=None):
     L = _L()
     out = []
     for profiled in (True, False):  # profiled: per-step launches
@@ -31,6 +31,8 @@ def run_both(params, x, v, steps, rebuild_every=0, legacy=False, facets=None):
             s.set_legacy_connectors(pairs, conn)
         if facets is not None:
             s.set_facets(*facets)
+        if necks is not None:
+            s.set_neck_strengths(necks)
         s.profile(profiled)
         s.step(1e-13, steps, rebuild_every=rebuild_every)
         st = s.state()
@@ -68,6 +70,21 @@ def test_tiny_sinter_lattice_and_hamaker_only():
     assert len(st["springs1"]) == 2 * 26
     x, v = S.lattice_with_impactor()
     run_both(L.make_params(L.BINARY_NONE, L.HAMAKER_FULL, hamaker=O.hamaker_params(A=1e-19, h0=1e-9)), x, v, 2000)
+
+
+@pytest.mark.parametrize("pairs", ["all", "list"])
+def test_tiny_breaking_sinter_lattice(pairs):
+    """alt_breaking_sinter on the bonded 27-particle lattice with neck
+    strengths drawn log-uniformly around its bond forces: part of the bonds
+    break, their entries leave the table and the pairs go on as contacts."""
+    L = _L()
+    x, v = S.sinter_lattice()
+    kw = dict(pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=4.0 * R) if pairs == "list" else {}
+    p = L.make_params(L.BINARY_SINTER, L.HAMAKER_NONE, contact=O.contact_params(dt=5e-14),
+                      sinter=O.sinter_params(k_bond=1000.0, d_crit=1.0e-9, dt=5e-14), sinter_breaking=1, **kw)
+    necks = np.exp(np.random.default_rng(1).uniform(np.log(3e-9), np.log(3e-7), 54))
+    st = run_both(p, x, v, 6000, rebuild_every=50 if pairs == "list" else 0, necks=necks)
+    assert 0 < len(st["springs1"]) < 2 * 54  # some bonds broke, not all
 
 
 def test_small_paths_legacy_facets_and_larger_systems():
