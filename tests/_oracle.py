@@ -1,5 +1,4 @@
-"""ctypes access to the CPU oracle (oracle/liboracle.so) and, when it was
-built, the reference harness (oracle/_ref/libref.so).
+"""ctypes access to the CPU oracle (oracle/liboracle.so).
 
 Test infrastructure only: the oracle is the checker, never the product.
 """
@@ -11,7 +10,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 
 _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
@@ -106,7 +104,6 @@ def system_params(binary_model, hamaker_model, neighbor_mode=0, r_verlet=0.0, co
 
 # ------------------------------------------------------------- loading ------
 _lib = None
-_ref = None
 
 
 def lib():
@@ -159,27 +156,6 @@ def lib():
     return _lib
 
 
-def ref():
-    """The reference harness, or None when it was not built (no reference tree)."""
-    global _ref
-    if _ref is None and os.path.exists(REF_SO):
-        R = C.CDLL(REF_SO)
-        R.ref_contact_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
-        R.ref_sinter_seq.argtypes = [_dp, _dp, C.c_size_t, _dp, _dp, _dp, C.POINTER(C.c_int)]
-        R.ref_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
-        R.ref_lr_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
-        R.ref_simplified_hamaker_seq.argtypes = [_dp, C.c_size_t, _dp, _dp]
-        R.ref_fold_seq.argtypes = [_dp, _dp, _dp, C.c_size_t, _dp, _dp]
-        R.ref_facet_seq.argtypes = [_dp, _dp, _dp, _dp, C.c_double, C.c_size_t, _dp, _dp]
-        R.ref_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
-        R.ref_legacy_seq.argtypes = [C.c_size_t, _dp, _dp, _dp, _dp, _dp, C.c_size_t, C.c_size_t, _dp, _dp,
-                                     C.POINTER(C.c_size_t)]
-        R.ref_sinter_topology.restype = C.c_size_t
-        R.ref_sinter_topology.argtypes = [C.c_size_t, _dp, _dp, _dp, _u32p, C.c_size_t]
-        _ref = R
-    return _ref
-
-
 # ------------------------------------------------------ sequence drivers ----
 def c64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
@@ -219,10 +195,6 @@ def oracle_breaking_seq(sinter, contact, strength, states):
     """alt_breaking_sinter on the ordered pair (0, 1): (a, alpha) per step and
     whether the bond has broken after that step."""
     return _breaking(lib().or_breaking_seq, sinter, contact, strength, states)
-
-
-def ref_breaking_seq(sinter, contact, strength, states):
-    return _breaking(ref().ref_breaking_seq, sinter, contact, strength, states)
 
 
 def oracle_hamaker_seq(mode, ham, states):
@@ -267,58 +239,6 @@ def oracle_topology(x0, r_part, d_crit):
     return pairs[:nb].copy(), sk.value
 
 
-def ref_contact_seq(params, states):
-    states = c64(states)
-    out = np.zeros((states.shape[0], 12))
-    ref().ref_contact_seq(dptr(c64(params)), states.shape[0], dptr(states), dptr(out))
-    return out
-
-
-def ref_sinter_seq(sinter, contact, states):
-    states = c64(states)
-    n = states.shape[0]
-    out = np.zeros((n, 12))
-    springs = np.zeros((n, 18))
-    b = C.c_int(0)
-    ref().ref_sinter_seq(dptr(c64(sinter)), dptr(c64(contact)), n, dptr(states), dptr(out), dptr(springs),
-                         C.byref(b))
-    return out, springs, b.value
-
-
-def ref_hamaker_seq(mode, ham, states):
-    states = c64(states)
-    out = np.zeros((states.shape[0], 12))
-    fn = {HAM_FULL: ref().ref_hamaker_seq, HAM_LIMITED: ref().ref_lr_hamaker_seq,
-          HAM_SIMPLIFIED: ref().ref_simplified_hamaker_seq}[mode]
-    fn(dptr(c64(ham)), states.shape[0], dptr(states), dptr(out))
-    return out
-
-
-def ref_fold_seq(sinter, contact, ham, states):
-    states = c64(states)
-    out = np.zeros((states.shape[0], 12))
-    ref().ref_fold_seq(dptr(c64(sinter)), dptr(c64(contact)), dptr(c64(ham)), states.shape[0], dptr(states),
-                       dptr(out))
-    return out
-
-
-def ref_facet_seq(verts, vf, contact, ham, dt_adv, states):
-    states = c64(states)
-    out = np.zeros((states.shape[0], 6))
-    ref().ref_facet_seq(dptr(c64(verts)), dptr(c64(vf)), dptr(c64(contact)), dptr(c64(ham)), dt_adv,
-                        states.shape[0], dptr(states), dptr(out))
-    return out
-
-
-def ref_topology(x0, sinter, contact):
-    x0 = c64(x0)
-    n = x0.shape[0]
-    cap = 16 * n + 16
-    pairs = np.zeros((cap, 2), dtype=np.uint32)
-    nb = ref().ref_sinter_topology(n, dptr(x0), dptr(c64(sinter)), dptr(c64(contact)), u32ptr(pairs), cap)
-    return pairs[:nb].copy()
-
-
 # -------------------------------------------------------- system driver -----
 def legacy_frames(x0, r_part, d_crit, vecs):
     """sinter_functor's constructor (or_legacy_frames): bonds (i < j, row
@@ -355,10 +275,6 @@ def _legacy_seq(fn, n, x0, th0, sinter13, contact18, vecs, incs, with_bonds):
 def oracle_legacy_seq(n, x0, th0, sinter13, contact18, vecs, incs):
     """or_legacy_seq: incs is (nsteps, n, 12) = dx, dtheta, v, omega."""
     return _legacy_seq(lib().or_legacy_seq, n, x0, th0, sinter13, contact18, vecs, incs, True)
-
-
-def ref_legacy_seq(n, x0, th0, sinter13, contact18, vecs, incs):
-    return _legacy_seq(ref().ref_legacy_seq, n, x0, th0, sinter13, contact18, vecs, incs, False)
 
 
 class OracleSystem:

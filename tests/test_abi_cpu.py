@@ -12,7 +12,7 @@ import _oracle as O
 libgran_amd = pytest.importorskip("libgran_amd")
 from libgran_amd import _abi  # noqa: E402
 
-GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "functors.npz"))
+AGG150 = np.load(os.path.join(os.path.dirname(__file__), "golden", "aggregate150.npz"))["x_in_r"] * O.R_PART
 
 
 def test_library_exports_every_declared_symbol():
@@ -30,13 +30,20 @@ def test_params_struct_layout_matches_header():
     assert C.sizeof(_abi.Params) == 48 + 8 * (18 + 13 + 5 + 18 + 5)
 
 
-def test_topology_matches_reference_fixtures():
-    pairs, sk = libgran_amd.sinter_topology(GOLD["agg150_x"], O.R_PART, 1e-3 * O.R_PART)
-    np.testing.assert_array_equal(pairs, GOLD["agg150_bonds"])
-    assert sk == 0
-    pairs, sk = libgran_amd.sinter_topology(GOLD["lattice_x"], O.R_PART, 1.0e-9)
-    np.testing.assert_array_equal(pairs, GOLD["lattice_bonds"])
-    assert sk == 28
+def test_topology_matches_reference_geometries():
+    """The host bond search on the reference's shipped aggregate
+    (resources/aggregate.h, a tree: 149 bonds) and the 3x3x3 lattice of
+    test/alt_sintered.cpp (26 bonds, 28 cycle-closing pairs skipped), against
+    the oracle's, which test_oracle_pins.py holds to an independent union-find
+    scan."""
+    import _scenarios as S
+    pairs, sk = libgran_amd.sinter_topology(AGG150, O.R_PART, 1e-3 * O.R_PART)
+    np.testing.assert_array_equal(pairs, O.oracle_topology(AGG150, O.R_PART, 1e-3 * O.R_PART)[0])
+    assert len(pairs) == 149 and sk == 0
+    xl, _ = S.sinter_lattice()
+    pairs, sk = libgran_amd.sinter_topology(xl, O.R_PART, 1.0e-9)
+    np.testing.assert_array_equal(pairs, O.oracle_topology(xl, O.R_PART, 1.0e-9)[0])
+    assert len(pairs) == 26 and sk == 28
 
 
 @pytest.mark.parametrize("n,seed,df", [(200, 0, 1.8), (1000, 3, 2.2), (3000, 1, 1.6)])

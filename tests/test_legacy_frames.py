@@ -1,7 +1,7 @@
 """The legacy sinter_functor's host set-up (hg_legacy_sinter_frames,
-sinter_bridge.h:138-211) against the oracle's restatement, which
-test_oracle_golden.py pins to the reference constructor: the same bonds in
-the same order, the same unit vectors drawn (rejections included) and
+sinter_bridge.h:138-211) against the oracle's restatement, whose frame
+geometry test_oracle_pins.py holds to its closed form: the same bonds in the
+same order, the same unit vectors drawn (rejections included) and
 bit-identical connectors. Host code: no GPU needed."""
 import numpy as np
 import pytest
