@@ -44,6 +44,17 @@ INERTIA = 2.0 / 5.0 * MASS * R_PART ** 2
 DT = 1e-13
 REBUILD = 50
 HBM_PEAK_GBS = 8000.0
+T_START = time.perf_counter()
+
+
+def log(msg):
+    """Progress of the legs, on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
+# The pool's autotuner scores each launch split over two rebuild intervals: 13
+# candidates on this batch. With fewer warm-up steps the split it settles on
+# (wide threshold 4096, room for CUs/16) is set directly.
+TUNER_STEPS = 26 * REBUILD
+PROFILED_SPLIT = (4096, 16)
 
 
 def aggregate_params(k=1.0e4, zeta=0.5, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.0 * R_PART, d_crit=1e-3 * R_PART):
@@ -182,11 +193,16 @@ def energy_drift(L, precision, steps=20000, every=1000):
 def legacy_sinter_leg(L, steps=2000):
     """The legacy sinter_functor (five connector springs per bond, moved by
     the step handler) with limited-range Hamaker on the N=20,000 aggregate,
-    FP64 neighbour list 3.2r rebuilt every 50 steps: particle-steps/s."""
+    FP64 neighbour list 3.2r rebuilt every 50 steps: particle-steps/s. The
+    connectors are undamped with k_d = 1e4 N/m, the reference's own legacy
+    scenario (test/sintered.cpp:66-67): damping at the batch's ratio on all
+    five connectors of every bond drives explicit velocity Verlet unstable on
+    particles with several bonds (the state leaves the finite range within
+    60 steps, oracle and device alike)."""
     import torch
-    c, s, h = aggregate_params()
-    s = list(s)
-    s[12] = 1e-3 * R_PART
+    c, _, h = aggregate_params()
+    kd = 1.0e4
+    s = [kd, 0.0, kd, 0.0, kd, 0.0, kd, 0.0, R_PART, MASS, INERTIA, DT, 1e-3 * R_PART]
     x = L.generate_aggregate(20000, R_PART, 1.8, 1.3, 1)
     v0 = np.random.default_rng(1).normal(0.0, 1.0, x.shape)
     vecs = np.random.default_rng(2).uniform(-1.0, 1.0, (200000, 3))
@@ -204,7 +220,7 @@ def legacy_sinter_leg(L, steps=2000):
     el = time.perf_counter() - t0
     sys_.close()
     return {"value": round(20000 * steps / el, 1), "unit": "particle-steps/s", "bonds": int(len(pairs)),
-            "steps": steps}
+            "steps": steps, "connectors": "k_d 1e4 N/m, undamped (test/sintered.cpp:66-67)"}
 
 
 def ledger_and_writer_leg(L, calls=20):
@@ -352,9 +368,10 @@ COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counte
 VALU_COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_valu_counters.txt")
 
 
-def measured_traffic(tuning, count):
+def measured_traffic(tuning, count, steps_per_launch):
     """HBM bytes per chunk launch from the committed FETCH_SIZE/WRITE_SIZE
-    profile (copy-calibrated). Valid when the run's wide threshold is the
+    profile (copy-calibrated), taken at 50 steps per launch and scaled to this
+    run's steps per launch. Valid when the run's wide threshold is the
     profile's: the threshold decides which systems the chunk kernel steps, the
     CU room left for the wide launches only its grid."""
     try:
@@ -367,15 +384,17 @@ def measured_traffic(tuning, count):
         return None, "counter profile unreadable"
     if count != 4096 or tuning["wide_min"] != int(split.group(1)):
         return None, f"counter profile was measured at wide threshold {split.group(1)}, this run differs"
-    return (int(m.group(1).replace(",", "")),
+    scale = steps_per_launch / REBUILD
+    return (int(int(m.group(1).replace(",", "")) * scale),
             f"profiles/pool_chunk_mixed_batch4096_hbm_counters.txt (measured at split {split.group(1)}:"
-            f"{split.group(2)})")
+            f"{split.group(2)}, 50 steps per launch; scaled by {scale:.3g} to this run's steps per launch)")
 
 
-def measured_valu(tuning, count, launch_us):
+def measured_valu(tuning, count, launch_us, steps_per_launch):
     """FP32 and FP64 VALU throughput of the chunk kernel: FLOPs per launch from
-    the committed SQ_INSTS_VALU_* profile (valid at its wide threshold) over
-    this run's average launch duration, against MI355X's vector peaks."""
+    the committed SQ_INSTS_VALU_* profile (valid at its wide threshold, taken
+    at 50 steps per launch and scaled to this run's) over this run's average
+    launch duration, against MI355X's vector peaks."""
     try:
         txt = open(VALU_COUNTERS).read()
     except OSError:
@@ -387,29 +406,53 @@ def measured_valu(tuning, count, launch_us):
         return None
     out = {"source": "profiles/pool_chunk_mixed_batch4096_valu_counters.txt"}
     for name, m, peak in (("fp32", f32, 157.3), ("fp64", f64, 78.6)):
-        tf = int(m.group(1).replace(",", "")) / (launch_us * 1e-6) / 1e12
+        tf = int(m.group(1).replace(",", "")) * steps_per_launch / REBUILD / (launch_us * 1e-6) / 1e12
         out[name] = {"achieved": round(tf, 2), "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4)}
     return out
 
 
-def cpu_baseline(x, v0, steps):
-    """The CPU oracle (a restatement of the reference, OpenMP over particles)
-    on the N=20,000 aggregate, timed on this host."""
+def host_cpu():
+    """Model name from lscpu and the CPUs this process may run on."""
+    import subprocess
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
+        model = m.group(1).strip() if m else model
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(x, v0, steps, runs=3):
+    """The CPU oracle (a restatement of the reference, OpenMP over particles,
+    threads bound close) on the N=20,000 aggregate, timed on this host: one
+    warm-up run, then the median of `runs` timed runs of `steps` steps."""
+    # the OpenMP runtime reads these when the oracle library is loaded
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     c, s, h = aggregate_params()
     p = O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R_PART, contact=c, sinter=s,
                         hamaker=h)
-    osys = O.OracleSystem(p, x, v0)
-    osys.step(DT, 20, rebuild_every=REBUILD)
-    t0 = time.perf_counter()
-    osys.step(DT, steps, rebuild_every=REBUILD)
-    el = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    times = []
+    for r in range(runs + 1):
+        osys = O.OracleSystem(p, x, v0)
+        osys.step(DT, 20, rebuild_every=REBUILD)
+        t0 = time.perf_counter()
+        osys.step(DT, steps, rebuild_every=REBUILD)
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    el = float(np.median(times))
+    model, cpus = host_cpu()
     return {"value": round(x.shape[0] * steps / el, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "cpus_available": cpus, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "sample": f"N=20,000 aggregate of the batch's parameters (seed 1, Df 1.8), {steps} FP64 steps after 20 "
-                      f"warm-up steps, list rebuilt every 50 steps, OpenMP over particles: {el:.1f} s; the batch "
-                      "is independent aggregates, so its CPU rate per particle-step is the same"}
+                      f"warm-up steps, list rebuilt every 50 steps, OpenMP over particles on {threads} threads; one "
+                      f"warm-up run, median of {runs} ({', '.join(f'{t:.2f}' for t in times)} s); the batch is "
+                      "independent aggregates, so its CPU rate per particle-step is the same"}
 
 
 def main():
@@ -418,7 +461,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=1500)
     ap.add_argument("--count", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=6000)
+    ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--serving-requests", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
@@ -443,9 +486,6 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the batch shards by aggregate "
               "across GPUs with no communication; see the design notes)", file=sys.stderr)
         sys.exit(2)
-    if args.steps % REBUILD or args.warmup % REBUILD:
-        print("--steps and --warmup must be multiples of 50 (the rebuild interval)", file=sys.stderr)
-        sys.exit(2)
 
     # torch first: it bundles its own HIP runtime library with the same soname
     # as ROCm's, and the first one loaded serves the whole process.
@@ -466,41 +506,56 @@ def main():
                           "phases": ["force", "kick", "rebuild", "reserved"]}))
         return
 
+    log(f"generating the {args.count}-aggregate batch")
     t_gen = time.perf_counter()
     sizes, xs, vs = batch_workload(L, args.count)
     t_gen = time.perf_counter() - t_gen
     total = int(sizes.sum())
+    log(f"batch: {total} particles in {t_gen:.1f} s; mixed pool, {args.warmup} warm-up + {args.steps} timed steps")
 
     split = tuple(int(q) for q in args.split.split(":")) if args.split else None
+    if split is None and args.warmup < TUNER_STEPS:
+        # too few warm-up steps for the autotuner: the split it picks on this
+        # batch, at which the committed counter profiles were taken
+        split = PROFILED_SPLIT
     pool, el_mx, prof_mx, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps,
                                                                  args.warmup, split)
     pool.close()
     value = total * args.steps / el_mx
-    chunk_us = prof_mx["chunk_ms"] * 1e3 / max(prof_mx["chunks"], 1)
+    log(f"mixed pool: {value:.4g} particle-steps/s")
+    chunks = max(prof_mx["chunks"], 1)
+    chunk_us = prof_mx["chunk_ms"] * 1e3 / chunks
+    steps_per_launch = prof_mx["chunk_steps"] / chunks
     bpps = state_bytes_per_particle_step(kbar, cbar)
-    bytes_launch = bpps * narrow * REBUILD
+    bytes_launch = bpps * narrow * steps_per_launch
     achieved = bytes_launch / (chunk_us * 1e-6) / 1e9
-    traffic, traffic_note = measured_traffic(tuning, args.count)
-    valu = measured_valu(tuning, args.count, chunk_us)
+    traffic, traffic_note = measured_traffic(tuning, args.count, steps_per_launch)
+    valu = measured_valu(tuning, args.count, chunk_us, steps_per_launch)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
-                "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (50 steps of every aggregate below the "
-                          "wide threshold per launch)",
+                "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (every aggregate below the wide threshold, "
+                          "the steps of one rebuild interval or of the part of one the timed steps cover, per launch)",
                 "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
+                "steps_per_launch": round(steps_per_launch, 2),
                 "particles_per_launch": narrow,
                 "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
                 "kbar": round(kbar, 3), "cbar": round(cbar, 3),
-                "rebuild_us_per_interval": round(prof_mx["rebuild_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1),
-                "wide_us_per_interval": round(prof_mx["wide_ms"] * 1e3 / max(prof_mx["chunks"], 1), 1),
+                "rebuilds_timed": sum(1 for k in range(args.warmup, args.warmup + args.steps) if k % REBUILD == 0),
+                "rebuild_ms_timed": round(prof_mx["rebuild_ms"], 3),
+                "wide_ms_timed": round(prof_mx["wide_ms"], 3),
                 "valu": valu}
     extra = {"aggregates": len(xs), "particles": total, "generation_s": round(t_gen, 1),
              "launch_split": {"wide_min": tuning["wide_min"], "wide_cus": tuning["wide_cus"],
                               "chunk_particles": narrow, "wide_particles": total - narrow}}
     cpu = None
     if not args.only_batched:
-        pool, el_64, _, _, _, tuning64, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup)
+        log("FP64 pool")
+        pool, el_64, _, _, _, tuning64, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup,
+                                                     split)
         pool.close()
+        log("plain FP64 path on a sample of the batch")
         plain, plain_note = plain_fp64_sample(L, sizes, xs, vs, 200)
+        log("single N=20,000 aggregate, FP64 and mixed")
         single64, x20, v20, launches, ms = single_aggregate(L, L.PRECISION_FP64, 5000, 200)
         single_mx = single_aggregate(L, L.PRECISION_MIXED, 5000, 200)[0]
         extra.update({
@@ -511,16 +566,22 @@ def main():
             "single_n20000_fp64": round(single64, 1), "single_n20000_mixed": round(single_mx, 1),
             "single_n20000_force_kernel_us": round(float(ms[0]) * 1e3 / max(int(launches[0]), 1), 2),
         })
+        log("energy ledger drift, FP64 and mixed")
         extra["energy_drift_undamped_n20000"] = {
             "note": "the aggregate above with normal springs (bonds, contacts) and limited-range Hamaker only, no "
                     "damping; largest ledger total drift over 20,000 steps relative to the ledger scale, dt 1e-13 s",
             "fp64": energy_drift(L, L.PRECISION_FP64), "mixed": energy_drift(L, L.PRECISION_MIXED)}
+        log("legacy sinter_functor")
         extra["legacy_sinter_n20000_fp64"] = legacy_sinter_leg(L)
+        log("energy ledger and VTK writer cost")
         extra["ledger_and_vtk_n20000"] = ledger_and_writer_leg(L)
         if args.serving_requests > 0:
+            log(f"serving stream, {args.serving_requests} requests")
             extra["serving"] = serving_leg(L, args.serving_requests)
         if not args.no_cpu_baseline:
+            log("CPU oracle baseline")
             cpu = cpu_baseline(x20, v20, args.cpu_steps)
+    log("done")
     out = {"metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_mx * 1e3 / args.steps, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",

@@ -156,6 +156,10 @@ hg_status hg_compute_accelerations(hg_system *sys);
  * and the system is in neighbour-list mode, the list is rebuilt before every
  * step whose global index is a multiple of rebuild_every. Facets advance by
  * v_facet*dt after each step (triangular_facet.h:62-66).
+ * Returns HG_ERR_NUMERIC when the closing kick leaves a velocity or angular
+ * velocity NaN/Inf, the overflow and invalid-operation conditions the
+ * reference's tests trap with feenableexcept (test/contact.cpp:9-18); the
+ * state is left as the steps made it.
  * Replaces: do_step(dt) of the libtimestep system with
  * rotational_velocity_verlet_half + rotational_step_handler
  * (test/contact.cpp:98-106). */
@@ -270,11 +274,24 @@ hg_status hg_pool_add(hg_pool *pool, size_t n, const double *x0, const double *v
 hg_status hg_pool_remove(hg_pool *pool, int32_t slot);
 /* Facets of one system (nf <= max_facets), as hg_set_facets. */
 hg_status hg_pool_set_facets(hg_pool *pool, int32_t slot, size_t nf, const double *verts, const double *v_facet);
-/* Advances every active system by nsteps of size dt. In neighbour-list mode
- * with rebuild_every > 0, nsteps must be a multiple of rebuild_every and each
- * system's list is rebuilt before every step whose index (the system's own
- * step count) is a multiple of rebuild_every. */
+/* Advances every active, unfaulted system by nsteps of size dt (any count).
+ * In neighbour-list mode with rebuild_every > 0 each system's list is rebuilt
+ * before every step whose index (the system's own step count) is a multiple
+ * of rebuild_every, as hg_step does; systems at different phases of that
+ * cadence share the launches up to the nearest rebuild of any of them.
+ * A system whose neighbour list or pair-state table overflows, or whose
+ * velocities become NaN/Inf, is marked faulted (hg_pool_fault) at the end of
+ * the rebuild interval in which it happened and is not stepped again; the
+ * other systems complete all nsteps unaffected. The call then returns
+ * HG_ERR_NUMERIC if any system went non-finite, else HG_ERR_CAPACITY, and
+ * hg_last_error() names the slots. Remove a faulted system with
+ * hg_pool_remove; its state stays readable until then. */
 hg_status hg_pool_step(hg_pool *pool, double dt, uint64_t nsteps, uint32_t rebuild_every);
+/* Why a system stopped (0: it has not): HG_FAULT_* bits. Not in the
+ * reference, whose tests stop the whole process on a floating-point trap
+ * (test/contact.cpp:9-18) or an allocation failure (contact_force.h:53). */
+enum { HG_FAULT_NEIGHBOR_CAPACITY = 1, HG_FAULT_PAIR_CAPACITY = 2, HG_FAULT_NUMERIC = 4 };
+hg_status hg_pool_fault(hg_pool *pool, int32_t slot, int *fault);
 /* Neck strengths of one system, as hg_set_neck_strengths. */
 hg_status hg_pool_set_neck_strengths(hg_pool *pool, int32_t slot, const double *strengths);
 /* Per-system copies, as the hg_system getters. */
@@ -295,8 +312,11 @@ hg_status hg_pool_info(hg_pool *pool, int32_t slot, size_t *n, double *t, uint64
  * the admission unit of an HBM budget. */
 size_t hg_pool_bytes_per_particle(const hg_params *params, size_t max_facets);
 /* Device time of the last hg_pool_step's chunk launches (milliseconds, HIP
- * events on the pool's stream) and the number of launches. */
-hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *rebuild_ms, uint64_t *chunks);
+ * events on the pool's stream), the number of launches and the steps they ran
+ * in total (a launch runs one rebuild interval, or the part of one that the
+ * call covers). */
+hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *rebuild_ms, uint64_t *chunks,
+                                    uint64_t *chunk_steps);
 /* Systems of at least wide_min particles (default 2048; 0: none) are stepped
  * by per-step launches over all their particles, one thread each, on a second
  * stream beside the chunk kernel, instead of by one workgroup: a large system

@@ -375,11 +375,18 @@ class Pool:
         return pairs[:nb.value], sk.value
 
     def last_step_profile(self):
-        c, r, k = C.c_double(0), C.c_double(0), C.c_uint64(0)
-        _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k)))
+        c, r, k, ks = C.c_double(0), C.c_double(0), C.c_uint64(0), C.c_uint64(0)
+        _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k), C.byref(ks)))
         w, wl = C.c_double(0), C.c_uint64(0)
         _abi.check(_abi.lib().hg_pool_last_wide_profile(self._h, C.byref(w), C.byref(wl)))
-        return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value, wide_ms=w.value, wide_launches=wl.value)
+        return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value, chunk_steps=ks.value, wide_ms=w.value,
+                    wide_launches=wl.value)
+
+    def fault(self, slot):
+        """HG_FAULT_* bits of a system that stopped (0 while it runs)."""
+        f = C.c_int(0)
+        _abi.check(_abi.lib().hg_pool_fault(self._h, slot, C.byref(f)))
+        return f.value
 
     def set_wide_threshold(self, wide_min):
         """Systems of at least wide_min particles (0: none) use the wide launches."""

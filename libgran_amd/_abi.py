@@ -60,6 +60,7 @@ class ServerStats(C.Structure):
 
 
 HG_REQ_QUEUED, HG_REQ_RUNNING, HG_REQ_DONE, HG_REQ_FAILED = range(4)
+HG_FAULT_NEIGHBOR_CAPACITY, HG_FAULT_PAIR_CAPACITY, HG_FAULT_NUMERIC = 1, 2, 4
 
 
 def fill(struct, values):
@@ -125,7 +126,8 @@ SIGNATURES = {
     "hg_pool_get_bonds": (C.c_int, [_sysp, C.c_int32, _u32p, C.c_size_t, _szp, _szp]),
     "hg_pool_info": (C.c_int, [_sysp, C.c_int32, _szp, _dp, _u64p]),
     "hg_pool_bytes_per_particle": (C.c_size_t, [C.POINTER(Params), C.c_size_t]),
-    "hg_pool_last_step_profile": (C.c_int, [_sysp, _dp, _dp, _u64p]),
+    "hg_pool_last_step_profile": (C.c_int, [_sysp, _dp, _dp, _u64p, _u64p]),
+    "hg_pool_fault": (C.c_int, [_sysp, C.c_int32, C.POINTER(C.c_int)]),
     "hg_pool_set_wide_threshold": (C.c_int, [_sysp, C.c_size_t]),
     "hg_pool_last_wide_profile": (C.c_int, [_sysp, _dp, _u64p]),
     "hg_pool_set_wide_cus": (C.c_int, [_sysp, C.c_int]),

@@ -160,6 +160,8 @@ __device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, d
 #pragma clang fp contract(fast)
     const d3 d = xi - xj;
     const double d2 = dot(d, d);
+    // coincident particles (d2 == 0) get a zero normal, as normalized() of a
+    // zero vector gives in the reference (Eigen) and in the FP64 path
     const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
     dist = d2 * inv;
     n64 = d * inv;
@@ -236,6 +238,8 @@ __device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf,
 #pragma clang fp contract(fast)
     const d3 d = xi - xf;
     const double d2 = dot(d, d);
+    // a centre on its facet point (d2 == 0) gets a zero normal, as normalized()
+    // of a zero vector gives in the reference (Eigen) and in the FP64 path
     const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
     const double dist = d2 * inv;
     const double ovd = m.sc_rd - dist;
@@ -267,6 +271,8 @@ __device__ __forceinline__ f3 surface_hamaker_mx(const mixed_consts &m, d3 xf, d
 #pragma clang fp contract(fast)
     const d3 r = xf - xi;
     const double d2 = dot(r, r);
+    // a centre on its facet point (d2 == 0) gets a zero normal, as normalized()
+    // of a zero vector gives in the reference (Eigen) and in the FP64 path
     const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
     const double dist = d2 * inv;
     double hd = dist - m.sh_r;

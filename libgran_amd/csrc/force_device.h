@@ -404,14 +404,26 @@ __device__ __forceinline__ accf facet_fold_mx(const hg_params &P, const mixed_co
     return s;
 }
 
+// Closes particle i's new pair-state row. An overflowing row raises the
+// flags (and, in a pool, its system's fault word `sys_fault`, may be null).
 template <int BIN, bool EAGER>
-__device__ __forceinline__ void close_table(walker_t<EAGER> &W, uint32_t *pcnt_new, uint32_t i, int *flags) {
+__device__ __forceinline__ void close_table(walker_t<EAGER> &W, uint32_t *pcnt_new, uint32_t i, int *flags,
+                                            int *sys_fault = nullptr) {
     if constexpr (BIN != HG_BINARY_NONE) {
         W.finish();
         pcnt_new[i] = W.q < W.cap ? W.q : W.cap;
-        if (W.q > W.cap) atomicOr(&flags[FLAG_PAIR_OVERFLOW], 1);
-        atomicMax(&flags[FLAG_PAIR_MAXCOUNT], int(W.q));
+        if (W.q > W.cap) {
+            atomicOr(&flags[FLAG_PAIR_OVERFLOW], 1);
+            atomicMax(&flags[FLAG_PAIR_MAXCOUNT], int(W.q));
+            if (sys_fault) atomicOr(sys_fault, HG_FAULT_PAIR_CAPACITY);
+        }
     }
+}
+
+// True when any component of the closing kick's v or omega is NaN or infinite.
+__device__ __forceinline__ bool nonfinite(d3 v, d3 w) {
+    return !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z) &&
+             __builtin_isfinite(w.x) && __builtin_isfinite(w.y) && __builtin_isfinite(w.z));
 }
 
 // Facet vertices advance by v_facet*dt (triangular_facet.h:62-66).

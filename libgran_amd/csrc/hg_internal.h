@@ -20,7 +20,17 @@ constexpr uint32_t kIdxMask = 0x7fffffffu;
 constexpr uint32_t kNoEntry = 0xffffffffu;  // (kNoEntry & kIdxMask) exceeds every valid j
 
 // Device flag words (int32 each) written by kernels, read back by the host.
-enum : int { FLAG_NL_OVERFLOW = 0, FLAG_PAIR_OVERFLOW = 1, FLAG_NL_MAXCOUNT = 2, FLAG_PAIR_MAXCOUNT = 3, FLAG_COUNT = 4 };
+// The MAXCOUNT words are raised only by lists that overflow, so they hold the
+// count the overflow needed. FLAG_NUMERIC: a velocity or angular velocity
+// left a closing kick non-finite.
+enum : int {
+    FLAG_NL_OVERFLOW = 0,
+    FLAG_PAIR_OVERFLOW = 1,
+    FLAG_NL_MAXCOUNT = 2,
+    FLAG_PAIR_MAXCOUNT = 3,
+    FLAG_NUMERIC = 4,
+    FLAG_COUNT = 5
+};
 
 constexpr int kMaxFacets = 16;
 // Bonded partners per particle in the neck-strength lists (touching equal
@@ -165,8 +175,9 @@ hipError_t launch_force(const hg_params &p, const sys_view &sv, hipStream_t s);
 hipError_t launch_kick_drift(double *x, double *v, double *th, double *w, const double *a, const double *al,
                              uint32_t n, bool pending, double half_prev, double half, double dt, facet_dev *facets,
                              int nf, double facet_dt, hipStream_t s);
+// Closing kick; raises flags[FLAG_NUMERIC] when a result is not finite.
 hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
-                             facet_dev *facets, int nf, double facet_dt, hipStream_t s);
+                             facet_dev *facets, int nf, double facet_dt, int *flags, hipStream_t s);
 hipError_t launch_neighbor_build(hg_system_impl &S, hipStream_t s);
 hipError_t launch_advance_facet(facet_dev *f, const double *v, double dt, hipStream_t s);
 // nsteps steps of a system of at most 256 particles (FP64) in one workgroup

@@ -192,14 +192,22 @@ __global__ void __launch_bounds__(256) k_legacy_move(const double *x, const doub
     legacy_move_one(x, v, w, n, i, dt, lconn, bcnt);
 }
 
+// Closing kick of particle i; a non-finite result raises FLAG_NUMERIC.
+__device__ __forceinline__ void final_kick_one(double *v, double *w, const double *a, const double *al, uint32_t i,
+                                               double half, int *flags) {
+    const d3 vi = ld3(v, i) + ld3(a, i) * half, wi = ld3(w, i) + ld3(al, i) * half;
+    st3(v, i, vi);
+    st3(w, i, wi);
+    if (nonfinite(vi, wi)) atomicOr(&flags[FLAG_NUMERIC], 1);
+}
+
 __global__ void __launch_bounds__(256) k_final_kick(double *v, double *w, const double *a, const double *al,
                                                     uint32_t n, double half, facet_dev *facets, int nf,
-                                                    double facet_dt) {
+                                                    double facet_dt, int *flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) advance_facets(facets, nf, facet_dt);
     if (i >= n) return;
-    st3(v, i, ld3(v, i) + ld3(a, i) * half);
-    st3(w, i, ld3(w, i) + ld3(al, i) * half);
+    final_kick_one(v, w, a, al, i, half, flags);
 }
 
 // nsteps steps of a system of at most kTile particles in one workgroup, one
@@ -237,10 +245,7 @@ __global__ void __launch_bounds__(kTile) k_small_steps(hg_params P, sys_view eve
     }
     if (final_kick) {
         if (i == 0) advance_facets(facets, nf_advance, dt);
-        if (live) {
-            st3(v, i, ld3(v, i) + ld3(even.a, i) * half);
-            st3(w, i, ld3(w, i) + ld3(even.al, i) * half);
-        }
+        if (live) final_kick_one(v, w, even.a, even.al, i, half, even.flags);
     }
 }
 
@@ -422,8 +427,10 @@ __global__ void __launch_bounds__(256) k_nl_fill(const double *x, uint32_t n, co
                 }
             }
     ncnt[i] = cnt < cap ? cnt : cap;
-    if (cnt > cap) atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
-    atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+    if (cnt > cap) {
+        atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
+        atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+    }
 }
 
 // ---------------------------------------------------------- diagnostics -----
@@ -727,8 +734,10 @@ __global__ void __launch_bounds__(kTinyThreads)
             if constexpr (BIN != HG_BINARY_NONE) {
                 if constexpr (NL) carry_below(kNoEntry);
                 S.pt.pcnt_new[i] = q < S.pt.cap ? q : S.pt.cap;
-                if (q > S.pt.cap) atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
-                atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(q));
+                if (q > S.pt.cap) {
+                    atomicOr(&S.flags[FLAG_PAIR_OVERFLOW], 1);
+                    atomicMax(&S.flags[FLAG_PAIR_MAXCOUNT], int(q));
+                }
             }
             if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, ld3(S.x, i), ld3(S.v, i), ld3(S.w, i));
             st3(S.a, i, acc.lin);
@@ -738,10 +747,7 @@ __global__ void __launch_bounds__(kTinyThreads)
     }
     if (final_kick) {
         if (t == 0) advance_facets(facets, nf_advance, dt);
-        if (t < n) {
-            st3(v, t, ld3(v, t) + ld3(even.a, t) * half);
-            st3(w, t, ld3(w, t) + ld3(even.al, t) * half);
-        }
+        if (t < n) final_kick_one(v, w, even.a, even.al, t, half, even.flags);
     }
 }
 
@@ -813,9 +819,9 @@ hipError_t launch_legacy_move(const double *x, const double *v, const double *w,
 }
 
 hipError_t launch_final_kick(double *v, double *w, const double *a, const double *al, uint32_t n, double half,
-                             facet_dev *facets, int nf, double facet_dt, hipStream_t s) {
+                             facet_dev *facets, int nf, double facet_dt, int *flags, hipStream_t s) {
     unsigned nb = blocks_for(n > 0 ? n : 1, 256);
-    k_final_kick<<<nb, 256, 0, s>>>(v, w, a, al, n, half, facets, nf, facet_dt);
+    k_final_kick<<<nb, 256, 0, s>>>(v, w, a, al, n, half, facets, nf, facet_dt, flags);
     return hipGetLastError();
 }
 

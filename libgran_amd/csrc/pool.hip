@@ -29,10 +29,12 @@
 #include "force_device.h"
 #include "hg_internal.h"
 
+
 namespace hg {
 
+// slot: the system's own index, naming its fault word in pool_view::sflags.
 struct pool_meta {
-    uint32_t offset, n, cur, acc_valid, nf, rebuild, pad0, pad1;
+    uint32_t offset, n, cur, acc_valid, nf, rebuild, slot, pad1;
 };
 
 struct pool_view {
@@ -51,6 +53,7 @@ struct pool_view {
     const pool_meta *meta;
     size_t stride;
     int *flags;
+    int *sflags;  // per-system fault words (HG_FAULT_* bits), indexed by slot
 };
 
 namespace {
@@ -200,7 +203,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
                     pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             }
-            close_table<BIN>(W, T.pcnt_new, g, V.flags);
+            close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
             if (M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             store_acc<LDS>(V, M.offset, g, acc.lin, acc.ang);
         } else {
@@ -217,7 +220,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
                     pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
                 }
             }
-            close_table<BIN>(W, T.pcnt_new, g, V.flags);
+            close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
             if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             store_acc<LDS>(V, M.offset, g, tod(acc.lin), tod(acc.ang));
         }
@@ -261,24 +264,34 @@ __device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_m
     }
 }
 
-// Final kick; a system kept wholly in LDS is written back here.
+// Final kick; a system kept wholly in LDS is written back here. A non-finite
+// velocity or angular velocity faults the system (HG_FAULT_NUMERIC).
 template <int LDS>
 __device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_meta &M, double half,
                                                 uint32_t g_first, uint32_t g_end, uint32_t g_step) {
+    bool bad = false;
     for (uint32_t g = g_first; g < g_end; g += g_step) {
+        d3 vi, wi;
         if constexpr (LDS == STATE_FULL) {
             const double *q = lds_rec<LDS>(M.offset, g);
             const d3 a = lds3(q + 12), al = lds3(q + 15);
+            vi = lds3(q + 3) + a * half;
+            wi = lds3(q + 6) + al * half;
             st3(V.x, g, lds3(q));
-            st3(V.v, g, lds3(q + 3) + a * half);
-            st3(V.w, g, lds3(q + 6) + al * half);
             st3(V.th, g, lds3(q + 9));
             st3(V.a, g, a);
             st3(V.al, g, al);
         } else {
-            st3(V.v, g, ld3(V.v, g) + ld3(V.a, g) * half);
-            st3(V.w, g, ld3(V.w, g) + ld3(V.al, g) * half);
+            vi = ld3(V.v, g) + ld3(V.a, g) * half;
+            wi = ld3(V.w, g) + ld3(V.al, g) * half;
         }
+        st3(V.v, g, vi);
+        st3(V.w, g, wi);
+        bad |= nonfinite(vi, wi);
+    }
+    if (bad) {
+        atomicOr(&V.flags[FLAG_NUMERIC], 1);
+        atomicOr(&V.sflags[M.slot], HG_FAULT_NUMERIC);
     }
 }
 
@@ -479,7 +492,7 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
                                                       uint32_t hw, double cs, uint32_t mask, double r_verlet,
                                                       const uint32_t *bucket_count, const uint32_t *bucket_start,
                                                       const uint32_t *cell_items, uint32_t *nb, uint32_t *ncnt,
-                                                      size_t stride, uint32_t cap, int *flags) {
+                                                      size_t stride, uint32_t cap, int *flags, int *sflags) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= hw || cell_of[g] == kNoEntry) return;
     const uint32_t slot = owner[g];
@@ -552,8 +565,11 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
             if (s < cnt && s < cap) nb[size_t(s) * stride + g] = r[s];
     }
     ncnt[g] = cnt < cap ? cnt : cap;
-    if (cnt > cap) atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
-    atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+    if (cnt > cap) {
+        atomicOr(&flags[FLAG_NL_OVERFLOW], 1);
+        atomicMax(&flags[FLAG_NL_MAXCOUNT], int(cnt));
+        atomicOr(&sflags[slot], HG_FAULT_NEIGHBOR_CAPACITY);
+    }
 }
 
 // Zeroes `rows` strided rows [offset, offset+n) of an array with row stride.
@@ -654,6 +670,7 @@ using namespace hg;
 
 struct hg_pool {
     hg_params p{};
+    int device = 0;  // the device current at hg_pool_create
     size_t P = 0, S = 0, MF = 0;
     // per-particle energy-ledger terms (hg_pool_energy), grown to the largest
     // system asked for
@@ -705,12 +722,17 @@ struct hg_pool {
     uint32_t hash_size = 0, *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr,
              *bucket_fill = nullptr, *cell_items = nullptr, *scan_tmp = nullptr;
     int *flags = nullptr, *flags_host = nullptr;
+    // per-system fault words set by the kernels (device, cleared after every
+    // read) and the faults recorded so far (host, per slot)
+    int *sflags = nullptr, *sflags_host = nullptr;
+    std::vector<int> fault;
+    int fault_nl_need = 0, fault_pair_need = 0;  // counts the last overflows needed
     std::map<uint32_t, uint32_t> free_ranges;  // offset -> length
     uint32_t high_water = 0;
     std::vector<void *> allocations;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double last_chunk_ms = 0.0, last_rebuild_ms = 0.0;
-    uint64_t last_chunks = 0;
+    uint64_t last_chunks = 0, last_chunk_steps = 0;
     int grid = 0;
     uint32_t lds_cap = 0;
     size_t lds_bytes_per_wg = 0;
@@ -750,18 +772,28 @@ uint32_t pool_cap_s(const hg_params &p) {
     return std::max<uint32_t>(p.pair_capacity ? p.pair_capacity : 24u, kTagPrefetch);
 }
 
-hg_status check_pool_flags(hg_pool *Q, const char *where) {
+// Reads the flag words of the launches queued so far. When one is raised, the
+// per-system fault words say which systems tripped it: they are added to
+// Q->fault (those systems are no longer stepped) and pushed onto `faulted`,
+// and both the flags and the fault words are cleared on the device.
+hg_status collect_faults(hg_pool *Q, std::vector<int32_t> &faulted) {
     HG_HIP(hipMemcpyAsync(Q->flags_host, Q->flags, FLAG_COUNT * sizeof(int), hipMemcpyDeviceToHost, Q->stream));
     HG_HIP(hipStreamSynchronize(Q->stream));
-    if (Q->flags_host[FLAG_NL_OVERFLOW] || Q->flags_host[FLAG_PAIR_OVERFLOW]) {
-        const bool nl = Q->flags_host[FLAG_NL_OVERFLOW] != 0;
-        const int need = Q->flags_host[nl ? FLAG_NL_MAXCOUNT : FLAG_PAIR_MAXCOUNT];
-        HG_HIP(hipMemsetAsync(Q->flags, 0, FLAG_COUNT * sizeof(int), Q->stream));
-        HG_HIP(hipStreamSynchronize(Q->stream));
-        return fail(HG_ERR_CAPACITY, std::string(where) + (nl ? ": neighbour list needs " : ": pair-state table needs ") +
-                                         std::to_string(need) + " slots per particle, capacity is " +
-                                         std::to_string(nl ? Q->cap_n : Q->cap_s));
+    const int *f = Q->flags_host;
+    if (!f[FLAG_NL_OVERFLOW] && !f[FLAG_PAIR_OVERFLOW] && !f[FLAG_NUMERIC]) return HG_OK;
+    if (f[FLAG_NL_OVERFLOW]) Q->fault_nl_need = std::max(Q->fault_nl_need, f[FLAG_NL_MAXCOUNT]);
+    if (f[FLAG_PAIR_OVERFLOW]) Q->fault_pair_need = std::max(Q->fault_pair_need, f[FLAG_PAIR_MAXCOUNT]);
+    HG_HIP(hipMemcpyAsync(Q->sflags_host, Q->sflags, Q->S * sizeof(int), hipMemcpyDeviceToHost, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    for (size_t s = 0; s < Q->S; s++) {
+        if (!Q->sflags_host[s] || !Q->active[s]) continue;
+        if (!Q->fault[s]) faulted.push_back(int32_t(s));
+        Q->fault[s] |= Q->sflags_host[s];
+        Q->order_dirty = true;
     }
+    HG_HIP(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->flags), FLAG_COUNT, Q->stream));
+    HG_HIP(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->sflags), Q->S, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
     return HG_OK;
 }
 
@@ -784,6 +816,7 @@ pool_view make_pool_view(hg_pool *Q) {
     V.meta = Q->meta_d;
     V.stride = Q->P;
     V.flags = Q->flags;
+    V.sflags = Q->sflags;
     return V;
 }
 
@@ -830,9 +863,29 @@ hg_status rebuild_due(hg_pool *Q) {
     HG_HIP(hipGetLastError());
     k_pool_nl_fill<<<nblk(hw, 64), 64, 0, Q->stream>>>(Q->x, Q->owner, Q->cell_of, hw, cs, mask, Q->p.r_verlet,
                                                       Q->bucket_count, Q->bucket_start, Q->cell_items, Q->nb,
-                                                      Q->ncnt, Q->P, Q->cap_n, Q->flags);
+                                                      Q->ncnt, Q->P, Q->cap_n, Q->flags, Q->sflags);
     HG_HIP(hipGetLastError());
     return HG_OK;
+}
+
+// Inputs of the energy ledger of one system of the pool.
+energy_view pool_energy_view(hg_pool *Q, const pool_meta &M) {
+    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, tab = Q->cap_s != 0;
+    return energy_view{Q->x,
+                       Q->v,
+                       Q->w,
+                       M.offset,
+                       M.n,
+                       Q->P,
+                       nl ? Q->nb : nullptr,
+                       nl ? Q->ncnt : nullptr,
+                       tab ? Q->pj[M.cur] : nullptr,
+                       tab ? Q->pcnt[M.cur] : nullptr,
+                       tab ? Q->ps[M.cur] : nullptr,
+                       nullptr,
+                       nullptr,
+                       nullptr,
+                       nullptr};
 }
 
 }  // namespace
@@ -899,6 +952,7 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
             return bail(fail(HG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
     } while (0)
     Q->p = *params;
+    Q->device = dev;
     Q->P = max_particles;
     Q->S = max_systems;
     Q->MF = max_facets;
@@ -963,12 +1017,16 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     PG_TRYC(palloc(Q, &Q->wide_map_d, Q->wide_map_cap));
     PG_TRYC(palloc(Q, &Q->counter_d, 1));
     PG_TRYC(palloc(Q, &Q->flags, FLAG_COUNT));
+    PG_TRYC(palloc(Q, &Q->sflags, Q->S));
     PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->flags_host), FLAG_COUNT * sizeof(int)));
-    PG_HIPC(hipMemsetAsync(Q->flags, 0, FLAG_COUNT * sizeof(int), Q->stream));
+    PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->sflags_host), Q->S * sizeof(int)));
+    PG_HIPC(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->flags), FLAG_COUNT, Q->stream));
+    PG_HIPC(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->sflags), Q->S, Q->stream));
     Q->meta.assign(Q->S, pool_meta{0, 0, 0, 0, 0, 0, 0, 0});
     Q->step_index.assign(Q->S, 0);
     Q->t.assign(Q->S, 0.0);
     Q->active.assign(Q->S, 0);
+    Q->fault.assign(Q->S, 0);
     Q->bonds.assign(Q->S, {});
     Q->skipped.assign(Q->S, 0);
     Q->free_ranges[0] = uint32_t(P);
@@ -991,6 +1049,7 @@ hg_status hg_pool_destroy(hg_pool *Q) {
     if (Q->wg_exec) (void)hipGraphExecDestroy(Q->wg_exec);
     if (Q->stream_w) (void)hipStreamDestroy(Q->stream_w);
     if (Q->flags_host) (void)hipHostFree(Q->flags_host);
+    if (Q->sflags_host) (void)hipHostFree(Q->sflags_host);
     if (Q->stream) (void)hipStreamDestroy(Q->stream);
     delete Q;
     return HG_OK;
@@ -1066,10 +1125,11 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
         HG_HIP(hipStreamSynchronize(Q->stream));
     }
     if (Q->cap_n) HG_HIP(launch_zero_u32(Q->ncnt + offset, n, Q->stream));
-    Q->meta[slot] = pool_meta{offset, uint32_t(n), 0, 0, 0, 1, 0, 0};
+    Q->meta[slot] = pool_meta{offset, uint32_t(n), 0, 0, 0, 1, uint32_t(slot), 0};
     Q->step_index[slot] = 0;
     Q->t[slot] = 0.0;
     Q->active[slot] = 1;
+    Q->fault[slot] = 0;
     Q->bonds[slot] = std::move(bonds);
     Q->skipped[slot] = skipped;
     Q->order_dirty = true;
@@ -1118,6 +1178,7 @@ hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
         HG_HIP(hipStreamSynchronize(Q->stream));
     }
     Q->active[slot] = 0;
+    Q->fault[slot] = 0;
     Q->meta[slot] = pool_meta{0, 0, 0, 0, 0, 0, 0, 0};
     Q->bonds[slot].clear();
     Q->order_dirty = true;
@@ -1177,7 +1238,7 @@ namespace {
 hg_status refresh_order(hg_pool *Q) {
     Q->order.clear();
     for (size_t s = 0; s < Q->S; s++)
-        if (Q->active[s]) Q->order.push_back(uint32_t(s));
+        if (Q->active[s] && !Q->fault[s]) Q->order.push_back(uint32_t(s));
     std::stable_sort(Q->order.begin(), Q->order.end(),
                      [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
     std::vector<wide_block> map;
@@ -1248,20 +1309,25 @@ void tune_record(hg_pool *Q, double seconds_per_step) {
 hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
     Q->last_chunk_ms = Q->last_rebuild_ms = Q->last_wide_ms = 0.0;
-    Q->last_chunks = Q->last_wide_launches = 0;
+    Q->last_chunks = Q->last_wide_launches = Q->last_chunk_steps = 0;
     if (nsteps == 0) return HG_OK;
     const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
-    if (nl && rebuild_every && nsteps % rebuild_every)
-        return fail(HG_ERR_INVALID, "hg_pool_step: nsteps must be a multiple of rebuild_every");
-    const uint64_t chunk = (nl && rebuild_every) ? rebuild_every : std::min<uint64_t>(nsteps, 2000);
+    const bool cadence = nl && rebuild_every;
+    const uint64_t chunk = cadence ? rebuild_every : 2000;
     const pool_view V = make_pool_view(Q);
     const double half = dt / 2.0;
+    std::vector<int32_t> faulted;  // systems that stopped during this call
     if (Q->tune_phase == TUNE_START) tune_begin(Q);
     for (uint64_t done = 0; done < nsteps;) {
-        const uint32_t len = uint32_t(std::min<uint64_t>(chunk, nsteps - done));
         if (Q->tune_phase == TUNE_RUNNING) tune_select(Q);
         if (Q->order_dirty) PG_TRY(refresh_order(Q));
-        if (Q->order.empty()) return HG_OK;
+        if (Q->order.empty()) break;
+        // one launch runs every system up to the nearest step at which any of
+        // them is due for a rebuild, as hg_step does per system
+        uint64_t len64 = std::min<uint64_t>(chunk, nsteps - done);
+        if (cadence)
+            for (uint32_t s : Q->order) len64 = std::min<uint64_t>(len64, rebuild_every - Q->step_index[s] % rebuild_every);
+        const uint32_t len = uint32_t(len64);
         const uint32_t count = Q->narrow_count;
         const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
         bool any_wide_initial = false;
@@ -1270,7 +1336,7 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
         if (nl) {
             bool any = false;
             for (uint32_t s : Q->order) {
-                const bool due = Q->meta[s].rebuild || (rebuild_every && Q->step_index[s] % rebuild_every == 0);
+                const bool due = Q->meta[s].rebuild || (cadence && Q->step_index[s] % rebuild_every == 0);
                 Q->meta[s].rebuild = due ? 1u : 0u;
                 any |= due;
             }
@@ -1355,20 +1421,9 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
             HG_HIP(hipEventRecord(Q->ev[3], sc));
         }
         if (split) HG_HIP(hipStreamWaitEvent(Q->stream, Q->wev[1], 0));
-        PG_TRY(check_pool_flags(Q, "hg_pool_step"));
-        float ms = 0.f;
-        if (count) {
-            HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
-            Q->last_chunk_ms += ms;
-            Q->last_chunks++;
-        }
-        if (Q->wide_blocks) {
-            HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
-            Q->last_wide_ms += ms;
-        }
-        if (Q->tune_phase == TUNE_RUNNING)
-            tune_record(Q, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_interval).count() / len);
-        // host mirror of what the kernels did to each system
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        // host mirror of what the kernels did to each system, before any
+        // fault is read, so that the systems that go on keep a true mirror
         for (uint32_t s : Q->order) {
             pool_meta &M = Q->meta[s];
             if (Q->cap_s) {
@@ -1379,9 +1434,50 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
             Q->step_index[s] += len;
             for (uint32_t k = 0; k < len; k++) Q->t[s] = Q->t[s] + dt;
         }
+        Q->meta_dirty = true;
+        PG_TRY(collect_faults(Q, faulted));
+        float ms = 0.f;
+        if (count) {
+            HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
+            Q->last_chunk_ms += ms;
+            Q->last_chunks++;
+            Q->last_chunk_steps += len;
+        }
+        if (Q->wide_blocks) {
+            HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
+            Q->last_wide_ms += ms;
+        }
+        if (Q->tune_phase == TUNE_RUNNING)
+            tune_record(Q, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_interval).count() / len);
         done += len;
     }
-    return upload_meta(Q);
+    PG_TRY(upload_meta(Q));
+    if (faulted.empty()) return HG_OK;
+    bool numeric = false;
+    std::string msg = "hg_pool_step: stopped system";
+    msg += faulted.size() > 1 ? "s" : "";
+    for (int32_t s : faulted) {
+        const int f = Q->fault[s];
+        numeric |= (f & HG_FAULT_NUMERIC) != 0;
+        msg += " " + std::to_string(s) + " (";
+        std::string why;
+        if (f & HG_FAULT_NEIGHBOR_CAPACITY)
+            why += "neighbour list needs " + std::to_string(Q->fault_nl_need) + " slots per particle, capacity is " +
+                   std::to_string(Q->cap_n);
+        if (f & HG_FAULT_PAIR_CAPACITY)
+            why += std::string(why.empty() ? "" : "; ") + "pair-state table needs " +
+                   std::to_string(Q->fault_pair_need) + " slots per particle, capacity is " + std::to_string(Q->cap_s);
+        if (f & HG_FAULT_NUMERIC) why += std::string(why.empty() ? "" : "; ") + "velocity became NaN or infinite";
+        msg += why + ")";
+    }
+    return fail(numeric ? HG_ERR_NUMERIC : HG_ERR_CAPACITY, msg + "; the other systems completed the steps");
+}
+
+hg_status hg_pool_fault(hg_pool *Q, int32_t slot, int *fault) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!fault) return fail(HG_ERR_INVALID, "hg_pool_fault: fault is required");
+    *fault = Q->fault[slot];
+    return HG_OK;
 }
 
 hg_status hg_pool_set_wide_threshold(hg_pool *Q, size_t wide_min) {
@@ -1447,7 +1543,7 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
     PG_TRY(valid_slot(Q, slot));
     if (!out) return fail(HG_ERR_INVALID, "hg_pool_energy: out is required");
     const pool_meta &M = Q->meta[slot];
-    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST, tab = Q->cap_s != 0;
+    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     if (nl && M.rebuild)
         return fail(HG_ERR_INVALID, "hg_pool_energy: the system is listed by its first step; step it first");
     if (M.n > Q->energy_cap) {
@@ -1462,21 +1558,7 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
         Q->energy_cap = M.n;
     }
     double *terms = Q->energy;
-    energy_view E{Q->x,
-                  Q->v,
-                  Q->w,
-                  M.offset,
-                  M.n,
-                  Q->P,
-                  nl ? Q->nb : nullptr,
-                  nl ? Q->ncnt : nullptr,
-                  tab ? Q->pj[M.cur] : nullptr,
-                  tab ? Q->pcnt[M.cur] : nullptr,
-                  tab ? Q->ps[M.cur] : nullptr,
-                  nullptr,
-                  nullptr,
-                  nullptr,
-                  nullptr};
+    const energy_view E = pool_energy_view(Q, M);
     std::vector<double> h(size_t(8) * M.n);
     hipError_t e = launch_energy(Q->p, E, mass, inertia, terms, Q->stream);
     if (e == hipSuccess)
@@ -1576,11 +1658,13 @@ hg_status hg_pool_info(hg_pool *Q, int32_t slot, size_t *n, double *t, uint64_t 
     return HG_OK;
 }
 
-hg_status hg_pool_last_step_profile(hg_pool *Q, double *chunk_ms, double *rebuild_ms, uint64_t *chunks) {
+hg_status hg_pool_last_step_profile(hg_pool *Q, double *chunk_ms, double *rebuild_ms, uint64_t *chunks,
+                                    uint64_t *chunk_steps) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
     if (chunk_ms) *chunk_ms = Q->last_chunk_ms;
     if (rebuild_ms) *rebuild_ms = Q->last_rebuild_ms;
     if (chunks) *chunks = Q->last_chunks;
+    if (chunk_steps) *chunk_steps = Q->last_chunk_steps;
     return HG_OK;
 }
 

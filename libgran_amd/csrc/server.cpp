@@ -118,7 +118,7 @@ void hg_server::run() {
         hg_status st = hg_pool_step(pool, cfg.dt, cfg.quantum, cfg.rebuild_every);
         double chunk_ms = 0, rebuild_ms = 0;
         uint64_t launches = 0;
-        if (st == HG_OK) st = hg_pool_last_step_profile(pool, &chunk_ms, &rebuild_ms, &launches);
+        if (st == HG_OK) st = hg_pool_last_step_profile(pool, &chunk_ms, &rebuild_ms, &launches, nullptr);
         std::lock_guard<std::mutex> lk(mu);
         if (st != HG_OK) {
             fatal = hg_last_error();

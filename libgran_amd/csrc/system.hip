@@ -143,14 +143,18 @@ hg_status check_flags(hg_system *S, const char *where) {
     }
     const int nl = S->flags_host[FLAG_NL_OVERFLOW], pr = S->flags_host[FLAG_PAIR_OVERFLOW];
     const int nlmax = S->flags_host[FLAG_NL_MAXCOUNT], prmax = S->flags_host[FLAG_PAIR_MAXCOUNT];
-    if (nl || pr) {
-        HG_HIP(hipMemsetAsync(S->flags, 0, FLAG_COUNT * sizeof(int), S->stream));
+    const int numeric = S->flags_host[FLAG_NUMERIC];
+    if (nl || pr || numeric) {
+        HG_HIP(launch_zero_u32(reinterpret_cast<uint32_t *>(S->flags), FLAG_COUNT, S->stream));
         HG_HIP(hipStreamSynchronize(S->stream));
+        // a truncated table is reported first: it can be what made the state non-finite
         if (nl)
             return fail(HG_ERR_CAPACITY, std::string(where) + ": neighbour list needs " + std::to_string(nlmax) +
                                              " slots per particle, capacity is " + std::to_string(S->cap_n));
-        return fail(HG_ERR_CAPACITY, std::string(where) + ": pair-state table needs " + std::to_string(prmax) +
-                                         " slots per particle, capacity is " + std::to_string(S->cap_s));
+        if (pr)
+            return fail(HG_ERR_CAPACITY, std::string(where) + ": pair-state table needs " + std::to_string(prmax) +
+                                             " slots per particle, capacity is " + std::to_string(S->cap_s));
+        return fail(HG_ERR_NUMERIC, std::string(where) + ": a velocity or angular velocity became NaN or infinite");
     }
     return HG_OK;
 }
@@ -560,7 +564,7 @@ hg_status step_impl(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_e
         S->step_index++;
     }
     HG_HIP(timed(S, PHASE_KICK, [&] {
-        return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, nf_advance, dt, S->stream);
+        return launch_final_kick(S->v, S->w, S->a, S->al, n, half, S->facets, nf_advance, dt, S->flags, S->stream);
     }));
     if (!check) {
         S->flags_unchecked = true;
