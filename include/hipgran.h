@@ -298,6 +298,20 @@ hg_status hg_pool_set_neck_strengths(hg_pool *pool, int32_t slot, const double *
 hg_status hg_pool_get_state(hg_pool *pool, int32_t slot, double *x, double *v, double *theta, double *omega);
 /* hg_energy for one system of the pool. */
 hg_status hg_pool_energy(hg_pool *pool, int32_t slot, double mass, double inertia, double *out);
+/* hg_pool_energy without allocating anything, for callers that cannot
+ * allocate: serving loops that must not hipMalloc/hipFree between steps (the
+ * ledger of each request as it retires), code that runs where allocation is
+ * forbidden. No counterpart in the reference. hg_pool_energy allocates and
+ * grows a per-pool buffer for the per-particle terms; this one writes them to
+ * a fixed device array of the library (one per GPU, 8 MiB, held until the
+ * process ends) and sums them on the device in particle order, so its result
+ * equals hg_pool_energy's bit for bit. Systems of at most
+ * HG_NOALLOC_MAX_PARTICLES particles (HG_ERR_CAPACITY beyond). All calls on
+ * one GPU share that array: calls that overlap on one GPU (from two threads,
+ * or on pools whose streams run concurrently) race, and the caller must
+ * serialise them. */
+enum { HG_NOALLOC_MAX_PARTICLES = 131072 };
+hg_status hg_pool_energy_noalloc(hg_pool *pool, int32_t slot, double mass, double inertia, double *out);
 hg_status hg_pool_get_accelerations(hg_pool *pool, int32_t slot, double *a, double *alpha);
 hg_status hg_pool_get_neighbors(hg_pool *pool, int32_t slot, uint64_t *offsets, uint32_t *idx, size_t cap,
                                 size_t *total);

@@ -61,6 +61,7 @@ class ServerStats(C.Structure):
 
 HG_REQ_QUEUED, HG_REQ_RUNNING, HG_REQ_DONE, HG_REQ_FAILED = range(4)
 HG_FAULT_NEIGHBOR_CAPACITY, HG_FAULT_PAIR_CAPACITY, HG_FAULT_NUMERIC = 1, 2, 4
+HG_NOALLOC_MAX_PARTICLES = 131072
 
 
 def fill(struct, values):
@@ -120,6 +121,7 @@ SIGNATURES = {
     "hg_pool_step": (C.c_int, [_sysp, C.c_double, C.c_uint64, C.c_uint32]),
     "hg_pool_get_state": (C.c_int, [_sysp, C.c_int32, _dp, _dp, _dp, _dp]),
     "hg_pool_energy": (C.c_int, [_sysp, C.c_int32, C.c_double, C.c_double, _dp]),
+    "hg_pool_energy_noalloc": (C.c_int, [_sysp, C.c_int32, C.c_double, C.c_double, _dp]),
     "hg_pool_get_accelerations": (C.c_int, [_sysp, C.c_int32, _dp, _dp]),
     "hg_pool_get_neighbors": (C.c_int, [_sysp, C.c_int32, _u64p, _u32p, C.c_size_t, _szp]),
     "hg_pool_get_springs": (C.c_int, [_sysp, C.c_int32, C.c_int, _u32p, _dp, C.c_size_t, _szp]),

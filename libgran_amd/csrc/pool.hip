@@ -23,12 +23,22 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "force_device.h"
 #include "hg_internal.h"
 
+// Device scratch of hg_pool_energy_noalloc, the entry point that allocates
+// nothing: the per-particle ledger terms of one system (8 doubles per
+// particle) followed by their 8 sums, for systems of up to
+// HG_NOALLOC_MAX_PARTICLES particles (8 MiB). Every GPU that runs the library
+// holds its own copy until the process ends. Host code reaches it only through
+// the address hipGetSymbolAddress returns for a device (noalloc_scratch),
+// never through this name.
+constexpr size_t kNoAllocParticles = HG_NOALLOC_MAX_PARTICLES;
+static __device__ __align__(256) double scratch[8 * kNoAllocParticles + 8];
 
 namespace hg {
 
@@ -442,6 +452,26 @@ __global__ void __launch_bounds__(kWideBlock) k_pool_wide_final_kick(pool_view V
     const uint32_t g = M.offset + b.start + threadIdx.x;
     if (g >= M.offset + M.n) return;
     pool_final_kick<STATE_GLOBAL>(V, M, half, g, g + 1, 1);
+}
+
+// Sums the per-particle ledger terms in particle order, the order sum_energy
+// uses on the host: lane c < 7 adds term c of every particle, then lane 0 adds
+// the seven sums; out[0..7].
+__global__ void __launch_bounds__(64) k_energy_sum(const double *terms, uint32_t n, double *out) {
+    __shared__ double part[7];
+    const uint32_t c = threadIdx.x;
+    if (c < 7) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < n; i++) s += terms[8 * size_t(i) + c];
+        part[c] = s;
+    }
+    __syncthreads();
+    if (c < 7) out[c] = part[c];
+    if (c == 0) {
+        double t = 0.0;
+        for (int q = 0; q < 7; q++) t += part[q];
+        out[7] = t;
+    }
 }
 
 // ------------------------------------------------------ batched rebuild -----
@@ -886,6 +916,30 @@ energy_view pool_energy_view(hg_pool *Q, const pool_meta &M) {
                        nullptr,
                        nullptr,
                        nullptr};
+}
+
+// The device address of `scratch` on device dev: looked up once per device,
+// with that device current, the first time a call reaches it; later calls
+// reuse it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+hg_status noalloc_scratch(int dev, double **out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(HG_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        int cur = 0;
+        HG_HIP(hipGetDevice(&cur));
+        if (cur != dev) HG_HIP(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        HG_HIP(e);
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return HG_OK;
 }
 
 }  // namespace
@@ -1566,6 +1620,29 @@ hg_status hg_pool_energy(hg_pool *Q, int32_t slot, double mass, double inertia, 
     if (e == hipSuccess) e = hipStreamSynchronize(Q->stream);
     HG_HIP(e);
     sum_energy(h.data(), M.n, out);
+    return HG_OK;
+}
+
+hg_status hg_pool_energy_noalloc(hg_pool *Q, int32_t slot, double mass, double inertia, double *out) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!out) return fail(HG_ERR_INVALID, "hg_pool_energy_noalloc: out is required");
+    const pool_meta &M = Q->meta[slot];
+    if (M.n > kNoAllocParticles)
+        return fail(HG_ERR_CAPACITY, "hg_pool_energy_noalloc: " + std::to_string(M.n) + " particles exceed its " +
+                                         std::to_string(kNoAllocParticles) + "; use hg_pool_energy");
+    if (Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST && M.rebuild)
+        return fail(HG_ERR_INVALID, "hg_pool_energy_noalloc: the system is listed by its first step; step it first");
+    double *scr = nullptr;
+    PG_TRY(noalloc_scratch(Q->device, &scr));
+    double *sums = scr + 8 * kNoAllocParticles;
+    hipError_t e = launch_energy(Q->p, pool_energy_view(Q, M), mass, inertia, scr, Q->stream);
+    if (e == hipSuccess) {
+        k_energy_sum<<<1, 64, 0, Q->stream>>>(scr, M.n, sums);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, sums, 8 * sizeof(double), hipMemcpyDeviceToHost, Q->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(Q->stream);
+    HG_HIP(e);
     return HG_OK;
 }
 
