@@ -257,9 +257,10 @@ def small_systems_leg(L, steps=20000):
     lattice and an impactor, contact + Hamaker, dt 1e-13 s) as a single FP64
     system, all pairs and with the 5r neighbour list rebuilt every 20 steps
     (test/hamaker_neighbor.cpp): microseconds per step with the whole call in
-    one launch per rebuild interval, with per-step launches (what hg_step
-    does while launch profiling is on), and with one queued step per call as
-    a drop-in driver's do_step makes them."""
+    one launch per rebuild interval, with per-step launches (the path of
+    every larger system, forced by hg_set_per_step_launches, no event
+    timing), and with one queued step per call as a drop-in driver's do_step
+    makes them."""
     import torch
     x, v = [], []
     for i in range(3):
@@ -278,9 +279,9 @@ def small_systems_leg(L, steps=20000):
     for mode, kw, every in (("all_pairs", {}, 0),
                             ("neighbor_list", dict(pair_mode=L.PAIRS_NEIGHBOR_LIST, r_verlet=5.0 * R_PART), 20)):
         p = L.make_params(L.BINARY_CONTACT, L.HAMAKER_FULL, contact=contact, hamaker=ham, **kw)
-        for name, profiled in (("per_step_launches_us", True), ("one_launch_us", False)):
+        for name, per_step in (("per_step_launches_us", True), ("one_launch_us", False)):
             s = L.System(p, x, v)
-            s.profile(profiled)
+            s.per_step_launches(per_step)
             s.step(DT, 100, rebuild_every=every)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -311,10 +312,10 @@ def small_systems_leg(L, steps=20000):
     kd = 1.0e4
     sinter = [kd, 0.0, kd, 0.0, kd, 0.0, kd, 0.0, R_PART, MASS, INERTIA, DT, 1.0e-9]
     p = L.make_params(L.BINARY_SINTER_LEGACY, L.HAMAKER_NONE, contact=contact, sinter=sinter)
-    for name, profiled in (("per_step_launches_us", True), ("one_launch_us", False)):
+    for name, per_step in (("per_step_launches_us", True), ("one_launch_us", False)):
         s = L.System(p, xl, vl)
         s.set_legacy_connectors(pairs, conn)
-        s.profile(profiled)
+        s.per_step_launches(per_step)
         s.step(DT, 100)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -325,17 +326,11 @@ def small_systems_leg(L, steps=20000):
     return out
 
 
-def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0, seed=7):
-    """The serving stream: `count` plate tests (N log-uniform 100..20000, Df
-    U(1.6, 2.2), plates at U(1, 10) m/s, compression or tension) arriving as
-    a Poisson stream at `rate` per second, each run for `steps` steps through
-    the continuous-batching server with an HBM budget of `budget_gib` GiB for
-    resident requests. Latency = result ready - arrival."""
-    from libgran_amd import workloads as W
-    reqs = W.serving_stream(count, rate, R_PART, seed=seed)
-    p = params(L, L.PRECISION_MIXED, facets=True)
-    srv = L.Server(p, budget_bytes=int(budget_gib * (1 << 30)), max_requests=count, max_facets=4, quantum=quantum,
-                   rebuild_every=REBUILD, dt=DT)
+def serve_stream(L, p, reqs, steps, budget_gib, quantum):
+    """Submits `reqs` at their arrival times to a continuous-batching server
+    and waits for every result. Returns latencies and the server's figures."""
+    srv = L.Server(p, budget_bytes=int(budget_gib * (1 << 30)), max_requests=len(reqs), max_facets=4,
+                   quantum=quantum, rebuild_every=REBUILD, dt=DT)
     ids = []
     t0 = time.perf_counter()
     for r in reqs:
@@ -350,18 +345,56 @@ def serving_leg(L, count, rate=200.0, steps=20000, quantum=1000, budget_gib=4.0,
     lat = times[:, 2] - times[:, 0]
     queue = times[:, 1] - times[:, 0]
     span = times[:, 2].max() - times[:, 0].min()
+    # device-busy share of the time in which requests were resident
+    active = times[:, 2].max() - times[:, 1].min()
     ps = sum(r["n"] for r in reqs) * steps
-    return {"requests": count, "arrival_rate_per_s": rate, "steps_per_request": steps,
-            "particles_mean": round(float(np.mean([r["n"] for r in reqs])), 1),
-            "budget_gib": budget_gib, "quantum_steps": quantum,
-            "throughput": round(ps / span, 1), "unit": "particle-steps/s", "makespan_s": round(float(span), 3),
+    rate = (len(reqs) - 1) / max(reqs[-1]["arrival"], 1e-9)
+    mean_n = float(np.mean([r["n"] for r in reqs]))
+    return {"requests": len(reqs), "arrival_rate_per_s": round(rate, 2),
+            "offered_load": round(rate * mean_n * steps, 1),
+            "throughput": round(ps / span, 1), "makespan_s": round(float(span), 3),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 3),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
             "queue_wait_p50_s": round(float(np.percentile(queue, 50)), 3),
             "queue_wait_p99_s": round(float(np.percentile(queue, 99)), 3),
-            "offered_load": round(rate * float(np.mean([r["n"] for r in reqs])) * steps, 1),
-            "device_ms": round(stats["device_ms"], 1), "rounds": int(stats["rounds"]),
-            "precision": "mixed (FP32 forces, FP64 state/springs)"}
+            "device_busy_s": round(stats["device_ms"] / 1e3, 3),
+            "device_busy_frac": round(stats["device_ms"] / 1e3 / active, 4), "rounds": int(stats["rounds"])}
+
+
+def serving_leg(L, count, rate=200.0, steps=20000, quantum=REBUILD, budget_gib=4.0, seed=7):
+    """The serving stream: plate tests (N log-uniform 100..20000, Df
+    U(1.6, 2.2), plates at U(1, 10) m/s, compression or tension) arriving as
+    a Poisson stream, each run for `steps` steps through the
+    continuous-batching server with an HBM budget of `budget_gib` GiB for
+    resident requests; latency = result ready - arrival. Three runs:
+    `count` requests at `rate` per second; the same requests as one static
+    batch in a pool (all present from the start: the throughput a server
+    cannot beat); and `count // 2` further requests at 70% of the measured
+    server throughput, so that the queue stays short."""
+    from libgran_amd import workloads as W
+    p = params(L, L.PRECISION_MIXED, facets=True)
+    reqs = W.serving_stream(count, rate, R_PART, seed=seed)
+    mean_n = float(np.mean([r["n"] for r in reqs]))
+    loaded = serve_stream(L, p, reqs, steps, budget_gib, quantum)
+    # the static batch of the same requests
+    total = sum(r["n"] for r in reqs)
+    pool = L.Pool(p, max_particles=total + 64, max_systems=len(reqs), max_facets=4)
+    for r in reqs:
+        pool.set_facets(pool.add(r["x"]), *r["facets"])
+    pool.autotune(True)
+    t0 = time.perf_counter()
+    pool.step(DT, steps, REBUILD)
+    static = total * steps / (time.perf_counter() - t0)
+    pool.close()
+    # below capacity
+    low_rate = 0.7 * loaded["throughput"] / (mean_n * steps)
+    reqs_low = W.serving_stream(max(count // 2, 2), low_rate, R_PART, seed=seed + 1)
+    light = serve_stream(L, p, reqs_low, steps, budget_gib, quantum)
+    return {"steps_per_request": steps, "budget_gib": budget_gib, "quantum_steps": quantum,
+            "unit": "particle-steps/s", "precision": "mixed (FP32 forces, FP64 state/springs)",
+            "at_200_per_s": loaded, "static_batch_same_requests": round(static, 1),
+            "served_vs_static_batch": round(loaded["throughput"] / static, 4),
+            "below_capacity": light}
 
 
 COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counters.txt")
@@ -462,7 +495,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1500)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=3000)
-    ap.add_argument("--serving-requests", type=int, default=256)
+    ap.add_argument("--serving-requests", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")

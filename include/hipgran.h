@@ -188,6 +188,22 @@ hg_status hg_reset_accelerations(hg_system *sys);
 /* Simulation time and number of steps taken. */
 hg_status hg_get_time(hg_system *sys, double *t, uint64_t *step_index);
 
+/* Snapshot of the whole system (state, accelerations, the current pair-state
+ * table, the neighbour list, bonds, facets with their springs, neck strengths,
+ * legacy connectors, time and step count) as a self-checking byte string:
+ * hg_load builds a system that continues exactly as the saved one would
+ * (bit for bit). *size receives the byte count; the bytes are written when
+ * buf is given and cap suffices (HG_ERR_CAPACITY otherwise). Not in the
+ * reference, whose only dump is the VTK file of positions and orientation
+ * axes (writer.cpp:51-118), which cannot restart a run. The format is
+ * described in libgran_amd/csrc/snapshot.h; snapshots of pool systems
+ * (hg_pool_save) load here too. */
+hg_status hg_save(hg_system *sys, void *buf, size_t cap, size_t *size);
+/* Creates a system on the current device from hg_save / hg_pool_save bytes,
+ * with the parameters they carry. HG_ERR_INVALID when the bytes are damaged
+ * (checksum) or inconsistent. */
+hg_status hg_load(const void *buf, size_t size, hg_system **out);
+
 /* Neighbour list as CSR: offsets[n+1]; idx written when cap suffices;
  * *total = number of entries. Each row is ascending. */
 hg_status hg_get_neighbors(hg_system *sys, uint64_t *offsets, uint32_t *idx, size_t cap, size_t *total);
@@ -294,6 +310,14 @@ enum { HG_FAULT_NEIGHBOR_CAPACITY = 1, HG_FAULT_PAIR_CAPACITY = 2, HG_FAULT_NUME
 hg_status hg_pool_fault(hg_pool *pool, int32_t slot, int *fault);
 /* Neck strengths of one system, as hg_set_neck_strengths. */
 hg_status hg_pool_set_neck_strengths(hg_pool *pool, int32_t slot, const double *strengths);
+/* hg_save for one system of the pool: saving, removing and later reloading a
+ * request (hg_pool_load, into this pool, another pool with the same params,
+ * or as a single system with hg_load) pre-empts it without changing its
+ * result. */
+hg_status hg_pool_save(hg_pool *pool, int32_t slot, void *buf, size_t cap, size_t *size);
+/* Adds a system from hg_save / hg_pool_save bytes taken with this pool's
+ * params (HG_ERR_INVALID otherwise); HG_ERR_CAPACITY when it does not fit. */
+hg_status hg_pool_load(hg_pool *pool, const void *buf, size_t size, int32_t *slot);
 /* Per-system copies, as the hg_system getters. */
 hg_status hg_pool_get_state(hg_pool *pool, int32_t slot, double *x, double *v, double *theta, double *omega);
 /* hg_energy for one system of the pool. */
@@ -356,6 +380,11 @@ hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
 hg_status hg_pool_autotune(hg_pool *pool, int enable);
 /* Current split and whether the autotuner has finished (tuned = 1). */
 hg_status hg_pool_get_tuning(hg_pool *pool, size_t *wide_min, int *wide_cus, int *tuned);
+/* Device-busy time of the last hg_pool_step (milliseconds): over its
+ * launches, the time from an interval's first device operation to its last,
+ * both streams joined (HIP events); the rest of the call's wall time is host
+ * work between intervals. */
+hg_status hg_pool_last_busy(hg_pool *pool, double *busy_ms);
 /* Device time of the last hg_pool_step's wide launches (second stream, HIP
  * events; milliseconds) and their number. */
 hg_status hg_pool_last_wide_profile(hg_pool *pool, double *wide_ms, uint64_t *launches);
@@ -369,11 +398,17 @@ hg_status hg_pool_last_wide_profile(hg_pool *pool, double *wide_ms, uint64_t *la
  * facet springs) and one worker thread. Requests (initial state, facets, step
  * count) are submitted by one producer thread and queue in arrival order. The
  * worker admits the head of the queue as soon as its particles fit a free
- * range of the arena and a slot is free, advances all resident requests by
- * `quantum` steps per round (one hg_pool_step), and retires each request when
- * it has run its steps: the final state is copied to host memory and its
- * range freed for the next one. A request's result does not depend on the
- * other requests on the device. Callers are the serving stream's producers.
+ * range of the arena and a slot is free. Each worker round queues one
+ * interval of the resident requests (at most `quantum` steps, and no further
+ * than the nearest rebuild or the end of a request) and, while it runs on the
+ * device, reads back the requests that finished in the previous round and
+ * uploads the admitted ones on a second stream; the host work of an admission
+ * (bond search, pair-state rows) is done by hg_server_submit on the
+ * producer's thread. A request whose state becomes NaN or infinite, or whose
+ * neighbour list or pair-state table overflows, fails alone (hg_server_wait
+ * returns HG_ERR_NUMERIC or HG_ERR_CAPACITY) and the others go on. A
+ * request's result does not depend on the other requests on the device.
+ * Callers are the serving stream's producers.
  * ------------------------------------------------------------------------ */
 typedef struct hg_server hg_server;
 
@@ -381,16 +416,16 @@ typedef struct {
     uint64_t budget_bytes;  /* device memory for the pool arena */
     uint32_t max_requests;  /* requests resident on the device at once */
     uint32_t max_facets;    /* facets per request (0..16) */
-    uint32_t quantum;       /* steps per worker round; every request's nsteps is a multiple */
-    uint32_t rebuild_every; /* neighbour-list cadence; divides quantum */
+    uint32_t quantum;       /* most steps per worker round (any nsteps is accepted) */
+    uint32_t rebuild_every; /* neighbour-list cadence */
     double dt;              /* time step of every request */
 } hg_server_config;
 
 typedef struct {
     uint64_t submitted, queued, running, done, failed;
-    uint64_t rounds;         /* worker rounds (hg_pool_step calls) */
-    uint64_t particle_steps; /* sum over rounds of resident particles x quantum */
-    double device_ms;        /* device time of the rounds' launches */
+    uint64_t rounds;         /* worker rounds that stepped requests */
+    uint64_t particle_steps; /* sum over rounds of stepped particles x the round's steps */
+    double device_ms;        /* device-busy time of the rounds (first to last operation, HIP events) */
     double elapsed_s;        /* seconds since hg_server_create */
 } hg_server_stats;
 
@@ -410,7 +445,8 @@ hg_status hg_server_status(hg_server *srv, uint64_t id, int *state);
 /* Blocks until the request has finished (timeout_s < 0: no limit; on time-out
  * HG_ERR_CAPACITY and the request stays), copies its final x, v, theta, omega
  * (any may be NULL) and times[3] = {submitted, admitted, done} in seconds
- * since hg_server_create, and releases it. */
+ * since hg_server_create, and releases it. A failed request is released too
+ * and returns its own status (HG_ERR_NUMERIC, HG_ERR_CAPACITY, HG_ERR_HIP). */
 hg_status hg_server_wait(hg_server *srv, uint64_t id, double timeout_s, double *x, double *v, double *theta,
                          double *omega, double *times);
 hg_status hg_server_get_stats(hg_server *srv, hg_server_stats *out);
@@ -424,6 +460,13 @@ hg_status hg_server_get_stats(hg_server *srv, hg_server_stats *out);
  * final kick, [2] neighbour rebuild (all of its kernels), [3] reserved. */
 hg_status hg_profile_enable(hg_system *sys, int enable);
 hg_status hg_profile_read(hg_system *sys, uint64_t *launches, double *device_ms);
+/* Small FP64 systems (up to 256 particles) step through one launch per
+ * rebuild interval; enable = 1 makes hg_step use the per-step launches (the
+ * path every larger system takes) without the event timing of
+ * hg_profile_enable, enable = 0 restores the choice by size. Results are the
+ * same either way. Not in the reference (one code path per functor there):
+ * the per-step baseline against which the one-launch path is timed. */
+hg_status hg_set_per_step_launches(hg_system *sys, int enable);
 
 /* Seeded tunable-dimension cluster-cluster aggregate of n touching spheres of
  * radius r_part (fractal dimension df, prefactor kf). x_out = 3*n doubles.

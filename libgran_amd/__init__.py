@@ -164,6 +164,25 @@ class System:
         _abi.check(L.hg_create(C.byref(params), self.n, _abi.dptr(x0), _abi.dptr(_abi.c64(v0)),
                                _abi.dptr(_abi.c64(theta0)), _abi.dptr(_abi.c64(omega0)), C.byref(self._h)))
 
+    @classmethod
+    def load(cls, snapshot):
+        """A system rebuilt from save() (or Pool.save()) bytes (hg_load)."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        buf = bytes(snapshot)
+        _abi.check(_abi.lib().hg_load(buf, len(buf), C.byref(self._h)))
+        self.params = None
+        self.n = int(np.frombuffer(buf, dtype=np.uint64, count=1, offset=16 + C.sizeof(Params))[0])
+        return self
+
+    def save(self):
+        """The system's snapshot as bytes (hg_save)."""
+        size = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_save(self._h, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _abi.check(_abi.lib().hg_save(self._h, buf, size.value, C.byref(size)))
+        return buf.raw[:size.value]
+
     def close(self):
         # at interpreter exit the module globals may already be gone
         if getattr(self, "_h", None) and self._h.value and _abi is not None and C is not None:
@@ -267,8 +286,49 @@ class System:
         _abi.check(_abi.lib().hg_energy(self._h, mass, inertia, _abi.dptr(out)))
         return dict(zip(ENERGY_TERMS, out))
 
+    def run(self, dt, nsteps, mass, inertia, thermo_period, thermo_path="energies.dat", rebuild_every=0,
+            dump_period=0, dump_dir=None, r_part=None):
+        """The driver loop of main.cpp:142-161: before the steps n with
+        n % thermo_period == 0, one line of t, E_trs, E_rot, E_tot, |P|, |L|
+        (tab-separated, six significant digits as a default C++ stream writes
+        them) into `thermo_path`, under the header "t E_trs E_rot E_tot P L";
+        with dump_period, a VTK particle file dump_dir/particles_<k>.vtk before
+        the steps n with n % dump_period == 0 (writer.cpp:51-54). The steps in
+        between run as one hg_step call each, so a thermo period costs one
+        device reduction, not a host loop. t is n * dt, as the reference
+        prints it. Returns the number of thermo lines."""
+        marks = set()
+        if thermo_period:
+            marks.update(range(0, nsteps, thermo_period))
+        if dump_period:
+            if dump_dir is None or r_part is None:
+                raise ValueError("dump_period needs dump_dir and r_part")
+            marks.update(range(0, nsteps, dump_period))
+        marks = sorted(marks) + [nsteps]
+        lines = dumps = 0
+        with open(thermo_path, "w") as ofs:
+            ofs.write("t\tE_trs\tE_rot\tE_tot\tP\tL\n")
+            for a, b in zip(marks[:-1], marks[1:]):
+                if dump_period and a % dump_period == 0:
+                    st = self.state()
+                    write_vtk_particles(os.path.join(dump_dir, f"particles_{dumps}.vtk"), st["x"], st["theta"],
+                                        r_part)
+                    dumps += 1
+                if thermo_period and a % thermo_period == 0:
+                    d = self.diagnostics(mass, inertia)
+                    ofs.write("\t".join(f"{q:g}" for q in (float(a) * dt, d["ke_trans"], d["ke_rot"], d["ke"],
+                                                           d["P"], d["L"])) + "\n")
+                    lines += 1
+                self.step(dt, b - a, rebuild_every)
+        return lines
+
     def profile(self, enable=True):
         _abi.check(_abi.lib().hg_profile_enable(self._h, 1 if enable else 0))
+
+    def per_step_launches(self, enable=True):
+        """Per-step kernels for a small FP64 system instead of its one
+        launch per rebuild interval (same results; a timing baseline)."""
+        _abi.check(_abi.lib().hg_set_per_step_launches(self._h, 1 if enable else 0))
 
     def profile_read(self):
         launches = np.zeros(4, dtype=np.uint64)
@@ -379,8 +439,25 @@ class Pool:
         _abi.check(_abi.lib().hg_pool_last_step_profile(self._h, C.byref(c), C.byref(r), C.byref(k), C.byref(ks)))
         w, wl = C.c_double(0), C.c_uint64(0)
         _abi.check(_abi.lib().hg_pool_last_wide_profile(self._h, C.byref(w), C.byref(wl)))
+        b = C.c_double(0)
+        _abi.check(_abi.lib().hg_pool_last_busy(self._h, C.byref(b)))
         return dict(chunk_ms=c.value, rebuild_ms=r.value, chunks=k.value, chunk_steps=ks.value, wide_ms=w.value,
-                    wide_launches=wl.value)
+                    wide_launches=wl.value, busy_ms=b.value)
+
+    def save(self, slot):
+        """One system's snapshot as bytes (hg_pool_save)."""
+        size = C.c_size_t(0)
+        _abi.check(_abi.lib().hg_pool_save(self._h, slot, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _abi.check(_abi.lib().hg_pool_save(self._h, slot, buf, size.value, C.byref(size)))
+        return buf.raw[:size.value]
+
+    def load(self, snapshot):
+        """Adds a system from save() bytes (hg_pool_load); returns its slot."""
+        buf = bytes(snapshot)
+        slot = C.c_int32(-1)
+        _abi.check(_abi.lib().hg_pool_load(self._h, buf, len(buf), C.byref(slot)))
+        return slot.value
 
     def fault(self, slot):
         """HG_FAULT_* bits of a system that stopped (0 while it runs)."""
@@ -457,9 +534,11 @@ class Server:
         n = self._sizes[rid]
         arrs = [np.zeros((n, 3)) for _ in range(4)]
         times = np.zeros(3)
-        _abi.check(_abi.lib().hg_server_wait(self._h, rid, float(timeout), *[_abi.dptr(a) for a in arrs],
-                                             _abi.dptr(times)))
-        del self._sizes[rid]
+        st = _abi.lib().hg_server_wait(self._h, rid, float(timeout), *[_abi.dptr(a) for a in arrs],
+                                       _abi.dptr(times))
+        if st != _abi.HG_ERR_CAPACITY or b"timed out" not in _abi.lib().hg_last_error():
+            del self._sizes[rid]  # finished (done or failed): the server has released it
+        _abi.check(st)
         out = dict(zip(("x", "v", "theta", "omega"), arrs))
         out["times"] = times
         return out

@@ -104,6 +104,7 @@ SIGNATURES = {
     "hg_breaking_sinter_topology": (C.c_int, [C.c_size_t, _dp, C.c_double, C.c_double, _u32p, C.c_size_t, _szp]),
     "hg_pool_set_neck_strengths": (C.c_int, [_sysp, C.c_int32, _dp]),
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
+    "hg_set_per_step_launches": (C.c_int, [_sysp, C.c_int]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
     "hg_generate_aggregates": (C.c_int, [C.c_size_t, _u32p, _dp, C.c_double, _u64p, C.c_double, _dp, C.c_int]),
@@ -130,8 +131,13 @@ SIGNATURES = {
     "hg_pool_bytes_per_particle": (C.c_size_t, [C.POINTER(Params), C.c_size_t]),
     "hg_pool_last_step_profile": (C.c_int, [_sysp, _dp, _dp, _u64p, _u64p]),
     "hg_pool_fault": (C.c_int, [_sysp, C.c_int32, C.POINTER(C.c_int)]),
+    "hg_save": (C.c_int, [_sysp, C.c_void_p, C.c_size_t, _szp]),
+    "hg_load": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(_sysp)]),
+    "hg_pool_save": (C.c_int, [_sysp, C.c_int32, C.c_void_p, C.c_size_t, _szp]),
+    "hg_pool_load": (C.c_int, [_sysp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int32)]),
     "hg_pool_set_wide_threshold": (C.c_int, [_sysp, C.c_size_t]),
     "hg_pool_last_wide_profile": (C.c_int, [_sysp, _dp, _u64p]),
+    "hg_pool_last_busy": (C.c_int, [_sysp, _dp]),
     "hg_pool_set_wide_cus": (C.c_int, [_sysp, C.c_int]),
     "hg_pool_set_wide_graph": (C.c_int, [_sysp, C.c_int]),
     "hg_pool_autotune": (C.c_int, [_sysp, C.c_int]),

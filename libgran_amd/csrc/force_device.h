@@ -239,10 +239,22 @@ struct lds_id_list {
 // they fetch the last partner again, from cache): a conditional load leaves
 // two values for the compiler to merge, and the merge waits for the load and
 // every memory operation issued before it.
-// IDS is id_list (global memory) or lds_id_list.
-template <class IDS, class LOAD, class FOLD>
+// IDS is id_list (global memory) or lds_id_list. PREFETCH = false loads each
+// partner's state when its pair comes (fewer registers held across a pair).
+template <bool PREFETCH = true, class IDS, class LOAD, class FOLD>
 __device__ __forceinline__ void nl_partners(const IDS &ids, LOAD &&load, FOLD &&fold) {
     if (ids.n == 0) return;
+    if constexpr (!PREFETCH) {
+        uint32_t jn = ids.at(0);
+        for (uint32_t k = 0; k < ids.n; k++) {
+            const uint32_t j = jn;
+            d3 xj, vj, wj;
+            load(j, xj, vj, wj);
+            jn = ids.at_clamped(k + 1);
+            fold(j, xj, vj, wj);
+        }
+        return;
+    }
     uint32_t jn = ids.at(0), jn2 = ids.at_clamped(1);
     d3 xn, vn, wn;
     load(jn, xn, vn, wn);

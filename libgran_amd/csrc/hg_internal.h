@@ -147,6 +147,7 @@ struct hg_system_impl {
     std::vector<void *> allocations;
     // launch profiling (hg_profile_enable)
     bool profiling = false;
+    bool per_step = false;  // hg_set_per_step_launches: no one-launch small path
     struct timed_range {
         int phase;
         hipEvent_t begin, end;

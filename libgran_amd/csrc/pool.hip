@@ -29,6 +29,8 @@
 
 #include "force_device.h"
 #include "hg_internal.h"
+#include "pool_internal.h"
+#include "snapshot.h"
 
 // Device scratch of hg_pool_energy_noalloc, the entry point that allocates
 // nothing: the per-particle ledger terms of one system (8 doubles per
@@ -78,6 +80,13 @@ namespace {
 //                partner gathers read LDS; the state itself stays global;
 //   STATE_GLOBAL everything in global memory.
 enum { STATE_GLOBAL = 0, STATE_MIRROR = 1, STATE_FULL = 2 };
+#ifndef HG_CHUNK_EAGER
+#define HG_CHUNK_EAGER 1
+#endif
+#ifndef HG_CHUNK_PREFETCH
+#define HG_CHUNK_PREFETCH 1
+#endif
+constexpr bool kChunkEager = HG_CHUNK_EAGER, kChunkPrefetch = HG_CHUNK_PREFETCH;
 extern __shared__ double pool_lds[];
 constexpr uint32_t kMirrorStride = 9, kFullStride = 18;
 
@@ -164,7 +173,7 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *s_red) {
 // of system M (a workgroup strides over the whole system; a wide launch gives
 // each thread one particle). With LIDS the partner ids come from the system's
 // LDS copy `lids` (lds_ids_fill).
-template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false>
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false, bool FAC = true>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                                  const pool_meta &M, uint32_t cur, const facet_dev *F,
                                                  uint32_t g_first, uint32_t g_end, uint32_t g_step,
@@ -185,7 +194,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
     for (uint32_t g = g_first; g < g_end; g += g_step) {
         d3 xi, vi, wi;
         load_state<LDS>(V, M.offset, g, xi, vi, wi);
-        walker_t<PREC == HG_PRECISION_MIXED> W;
+        walker_t<PREC == HG_PRECISION_MIXED && kChunkEager> W;
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
         id_list ids;
         if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST && !LIDS) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
@@ -194,9 +203,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
             auto ld = [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); };
             if constexpr (LIDS) {
                 const uint32_t t = g - M.offset;
-                nl_partners(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ld, fold);
+                nl_partners<kChunkPrefetch>(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ld, fold);
             } else {
-                nl_partners(ids, ld, fold);
+                nl_partners<kChunkPrefetch>(ids, ld, fold);
             }
         };
         if constexpr (PREC == HG_PRECISION_FP64) {
@@ -214,7 +223,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
-            if (M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
+            if (FAC && M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             store_acc<LDS>(V, M.offset, g, acc.lin, acc.ang);
         } else {
             accf acc{zerof(), zerof()};
@@ -231,7 +240,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
-            if (M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
+            if (FAC && M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             store_acc<LDS>(V, M.offset, g, tod(acc.lin), tod(acc.ang));
         }
     }
@@ -308,7 +317,7 @@ __device__ __forceinline__ void pool_final_kick(const pool_view &V, const pool_m
 // All steps of one rebuild interval for one system, by one workgroup. With
 // LIDS the system's partner ids (kmax slots per particle) are copied to LDS
 // after its state records.
-template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS>
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS, bool FAC>
 __device__ __forceinline__ void run_system(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                            const pool_meta &M, facet_dev *F, uint32_t nsteps, double dt,
                                            uint32_t kmax) {
@@ -331,7 +340,7 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
             if (s > 0 && threadIdx.x == 0) advance_facets(F, int(M.nf), dt);
             __syncthreads();
         }
-        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS, LIDS>(P, MC, V, M, cur, F, g0, g_end, gs, lids, kmax);
+        pool_force_phase<BIN, HAM, PAIRS, PREC, LDS, LIDS, FAC>(P, MC, V, M, cur, F, g0, g_end, gs, lids, kmax);
         cur ^= 1u;
         __syncthreads();
     }
@@ -351,8 +360,10 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
 
 // One rebuild interval of every queued system. Exit: every workgroup leaves
 // once the shared counter passes `count`. lds_bytes of dynamic LDS per
-// workgroup decide each system's state placement.
-template <int BIN, int HAM, int PAIRS, int PREC>
+// workgroup decide each system's state placement. FAC = false leaves the
+// facet fold out (every queued system has nf = 0): it holds the kernel's
+// largest register set, and without it more wavefronts fit each SIMD.
+template <int BIN, int HAM, int PAIRS, int PREC, bool FAC>
 // The model constants come by value: the scalar registers they spill to are
 // re-read with v_readlane, which measured faster than reloading them through
 // a pointer inside the pair loop (scalar loads add memory latency there).
@@ -378,20 +389,20 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
             kmax = block_max_u32(kmax, s_red);
             const size_t ids = M.n <= 65535u ? lds_ids_bytes(M.n, kmax) : ~size_t(0) / 2;
             if (need * kFullStride + ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true>(P, MC, V, M, F, nsteps, dt, kmax);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else if (need * kMirrorStride + ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true>(P, MC, V, M, F, nsteps, dt, kmax);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else if (ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true>(P, MC, V, M, F, nsteps, dt, kmax);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false>(P, MC, V, M, F, nsteps, dt, 0);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
         } else {
             if (need * kFullStride <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, false>(P, MC, V, M, F, nsteps, dt, 0);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
             else if (need * kMirrorStride <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, false>(P, MC, V, M, F, nsteps, dt, 0);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
             else
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false>(P, MC, V, M, F, nsteps, dt, 0);
+                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
         }
     }
 }
@@ -611,39 +622,46 @@ __global__ void k_zero_rows_f64(double *p, size_t stride, size_t offset, size_t 
 
 inline unsigned nblk(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
-template <int BIN, int HAM, int PAIRS, int PREC>
+template <int BIN, int HAM, int PAIRS, int PREC, bool FAC>
 hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
                           hipStream_t s) {
     const size_t lds_bytes = lds_cap;
     if (lds_bytes > 0) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pool_chunk<BIN, HAM, PAIRS, PREC>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
         if (e != hipSuccess) return e;
     }
-    k_pool_chunk<BIN, HAM, PAIRS, PREC><<<grid, 256, lds_bytes, s>>>(p, make_mixed_consts(p), V, order, count, counter,
-                                                                      nsteps, dt, lds_cap);
+    k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC><<<grid, 256, lds_bytes, s>>>(p, make_mixed_consts(p), V, order, count,
+                                                                           counter, nsteps, dt, lds_cap);
     return hipGetLastError();
 }
 
 template <int BIN, int HAM, int PAIRS>
 hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
-                          hipStream_t s) {
-    if (p.precision == HG_PRECISION_MIXED)
-        return launch_chunk_k<BIN, HAM, PAIRS, HG_PRECISION_MIXED>(p, V, order, count, counter, nsteps, dt, grid,
-                                                                   lds_cap, s);
-    return launch_chunk_k<BIN, HAM, PAIRS, HG_PRECISION_FP64>(p, V, order, count, counter, nsteps, dt, grid, lds_cap,
-                                                              s);
+                          bool facets, hipStream_t s) {
+#define HG_KCASE(PREC, FAC)                                                                                  \
+    if (p.precision == PREC && facets == FAC)                                                               \
+        return launch_chunk_k<BIN, HAM, PAIRS, PREC, FAC>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, \
+                                                           s);
+    HG_KCASE(HG_PRECISION_MIXED, true)
+    HG_KCASE(HG_PRECISION_MIXED, false)
+    HG_KCASE(HG_PRECISION_FP64, true)
+    HG_KCASE(HG_PRECISION_FP64, false)
+#undef HG_KCASE
+    return hipErrorInvalidValue;
 }
 
+// facets: whether any queued system has facets (else the kernel without the
+// facet fold runs).
 template <int PAIRS>
 hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
-                          hipStream_t s) {
+                          bool facets, hipStream_t s) {
 #define HG_PCASE(B, H) \
     if (p.binary_model == B && p.hamaker_model == H) \
-        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, s);
+        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, facets, s);
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
@@ -721,6 +739,7 @@ struct hg_pool {
     std::vector<uint64_t> step_index;
     std::vector<double> t;
     std::vector<char> active;
+    std::vector<char> parked;  // out of the launch order while its state is read back
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> bonds;
     std::vector<size_t> skipped;
     uint32_t *order_d = nullptr, *counter_d = nullptr;
@@ -734,6 +753,7 @@ struct hg_pool {
     // with wide systems present the wide launches run on stream_w beside the
     // chunk kernel, whose grid then leaves room worth wide_cus CUs
     hipStream_t stream_w = nullptr;
+    hipStream_t stream_c = nullptr;  // copies that overlap an interval (pool_commit, pool_read_state)
     int n_cus = 0, wide_cus = 0;
     bool wide_graph = true;  // replay the wide launches of an interval as a HIP graph
     int tune_phase = 0;      // TUNE_*
@@ -761,6 +781,17 @@ struct hg_pool {
     uint32_t high_water = 0;
     std::vector<void *> allocations;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t evb[2] = {nullptr, nullptr};  // start and end of an interval's device work
+    pool_meta *meta_pinned = nullptr;        // staging of the per-interval meta upload
+    double last_busy_ms = 0.0;
+    // the interval queued by pool_begin and not yet closed by pool_end
+    struct {
+        bool active = false;
+        uint32_t len = 0, count = 0;
+        bool rebuilt = false, wide = false;
+        double dt = 0.0;
+        std::chrono::steady_clock::time_point t0;
+    } fl;
     double last_chunk_ms = 0.0, last_rebuild_ms = 0.0;
     uint64_t last_chunks = 0, last_chunk_steps = 0;
     int grid = 0;
@@ -1022,10 +1053,12 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream, hipStreamNonBlocking));
     for (auto &e : Q->ev) PG_HIPC(hipEventCreate(&e));
     for (auto &e : Q->wev) PG_HIPC(hipEventCreate(&e));
+    for (auto &e : Q->evb) PG_HIPC(hipEventCreate(&e));
     Q->n_cus = prop.multiProcessorCount;
     Q->wide_min = kDefaultWideMin;
     Q->wide_cus = int(std::lround(kDefaultWideCuFraction * Q->n_cus));
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream_w, hipStreamNonBlocking));
+    PG_HIPC(hipStreamCreateWithFlags(&Q->stream_c, hipStreamNonBlocking));
     const size_t P = Q->P;
     double **fields[6] = {&Q->x, &Q->v, &Q->th, &Q->w, &Q->a, &Q->al};
     for (auto f : fields) PG_TRYC(palloc(Q, f, 4 * P));
@@ -1074,12 +1107,14 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     PG_TRYC(palloc(Q, &Q->sflags, Q->S));
     PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->flags_host), FLAG_COUNT * sizeof(int)));
     PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->sflags_host), Q->S * sizeof(int)));
+    PG_HIPC(hipHostMalloc(reinterpret_cast<void **>(&Q->meta_pinned), Q->S * sizeof(pool_meta)));
     PG_HIPC(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->flags), FLAG_COUNT, Q->stream));
     PG_HIPC(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->sflags), Q->S, Q->stream));
     Q->meta.assign(Q->S, pool_meta{0, 0, 0, 0, 0, 0, 0, 0});
     Q->step_index.assign(Q->S, 0);
     Q->t.assign(Q->S, 0.0);
     Q->active.assign(Q->S, 0);
+    Q->parked.assign(Q->S, 0);
     Q->fault.assign(Q->S, 0);
     Q->bonds.assign(Q->S, {});
     Q->skipped.assign(Q->S, 0);
@@ -1095,13 +1130,18 @@ hg_status hg_pool_destroy(hg_pool *Q) {
     if (!Q) return HG_OK;
     if (Q->stream) (void)hipStreamSynchronize(Q->stream);
     if (Q->stream_w) (void)hipStreamSynchronize(Q->stream_w);
+    if (Q->stream_c) (void)hipStreamSynchronize(Q->stream_c);
     for (void *p : Q->allocations) (void)hipFree(p);
     for (auto e : Q->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto e : Q->wev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : Q->evb)
+        if (e) (void)hipEventDestroy(e);
+    if (Q->meta_pinned) (void)hipHostFree(Q->meta_pinned);
     if (Q->wg_exec) (void)hipGraphExecDestroy(Q->wg_exec);
     if (Q->stream_w) (void)hipStreamDestroy(Q->stream_w);
+    if (Q->stream_c) (void)hipStreamDestroy(Q->stream_c);
     if (Q->flags_host) (void)hipHostFree(Q->flags_host);
     if (Q->sflags_host) (void)hipHostFree(Q->sflags_host);
     if (Q->stream) (void)hipStreamDestroy(Q->stream);
@@ -1109,10 +1149,51 @@ hg_status hg_pool_destroy(hg_pool *Q) {
     return HG_OK;
 }
 
-hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, const double *theta0,
-                      const double *omega0, int32_t *slot_out) {
-    if (!Q || !x0 || !slot_out) return fail(HG_ERR_INVALID, "hg_pool_add: pool, x0 and slot are required");
+}  // extern "C"
+
+namespace hg {
+
+hg_status pool_prepare(const hg_params &p, size_t n, const double *x0, const double *v0, const double *theta0,
+                       const double *omega0, const std::vector<std::pair<uint32_t, uint32_t>> *given_bonds,
+                       size_t given_skipped, pool_prep &out) {
+    if (!x0) return fail(HG_ERR_INVALID, "hg_pool_add: x0 is required");
     if (n == 0) return fail(HG_ERR_INVALID, "hg_pool_add: empty system");
+    out.n = n;
+    out.bonds.clear();
+    out.skipped = 0;
+    out.rows.assign(p.binary_model == HG_BINARY_SINTER ? n : 0, {});
+    if (p.binary_model == HG_BINARY_SINTER) {
+        if (given_bonds) {
+            out.bonds = *given_bonds;
+            out.skipped = given_skipped;
+        } else {
+            sinter_topology_host(n, x0, p.sinter.r_part, p.sinter.d_crit, out.bonds, out.skipped, !p.sinter_breaking);
+        }
+        for (auto &b : out.bonds) {
+            out.rows[b.first].push_back(b.second | kBondBit);
+            out.rows[b.second].push_back(b.first | kBondBit);
+        }
+        for (auto &r : out.rows)
+            std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return (a & kIdxMask) < (b & kIdxMask); });
+    }
+    const double *src[4] = {x0, v0, theta0, omega0};
+    for (int f = 0; f < 4; f++) {
+        out.q4[f].assign(4 * n, 0.0);
+        if (src[f])
+            for (size_t i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) out.q4[f][4 * i + c] = src[f][3 * i + c];
+    }
+    return HG_OK;
+}
+
+hg_status pool_commit(hg_pool *Q, pool_prep &prep, int32_t *slot_out, hipStream_t cs) {
+    if (!cs && Q) cs = Q->stream;
+    if (!Q || !slot_out) return fail(HG_ERR_INVALID, "hg_pool_add: pool and slot are required");
+    const size_t n = prep.n;
+    for (auto &r : prep.rows)
+        if (r.size() > Q->cap_s || (Q->bcnt && r.size() > kMaxBondsPerParticle))
+            return fail(HG_ERR_CAPACITY, "hg_pool_add: a particle has " + std::to_string(r.size()) +
+                                             " bonds, pair capacity is " + std::to_string(Q->cap_s));
     int32_t slot = -1;
     for (size_t s = 0; s < Q->S; s++)
         if (!Q->active[s]) {
@@ -1125,78 +1206,88 @@ hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, 
         if (it->second >= n) break;
     if (it == Q->free_ranges.end()) return fail(HG_ERR_CAPACITY, "hg_pool_add: no free particle range of that size");
     const uint32_t offset = it->first;
-    // topology and the initial pair-state rows before committing anything
-    std::vector<std::pair<uint32_t, uint32_t>> bonds;
-    size_t skipped = 0;
-    std::vector<std::vector<uint32_t>> rows(n);
-    if (Q->p.binary_model == HG_BINARY_SINTER) {
-        sinter_topology_host(n, x0, Q->p.sinter.r_part, Q->p.sinter.d_crit, bonds, skipped, !Q->p.sinter_breaking);
-        for (auto &b : bonds) {
-            rows[b.first].push_back((offset + b.second) | kBondBit);
-            rows[b.second].push_back((offset + b.first) | kBondBit);
-        }
-        for (auto &r : rows) std::sort(r.begin(), r.end(), [](uint32_t a, uint32_t b) { return (a & kIdxMask) < (b & kIdxMask); });
-        for (auto &r : rows)
-            if (r.size() > Q->cap_s || (Q->bcnt && r.size() > kMaxBondsPerParticle))
-                return fail(HG_ERR_CAPACITY, "hg_pool_add: a particle has " + std::to_string(r.size()) +
-                                                 " bonds, pair capacity is " + std::to_string(Q->cap_s));
-    }
     const uint32_t len = it->second;
     Q->free_ranges.erase(it);
     if (len > n) Q->free_ranges[offset + uint32_t(n)] = len - uint32_t(n);
     Q->high_water = std::max<uint32_t>(Q->high_water, offset + uint32_t(n));
 
-    std::vector<double> buf(4 * n, 0.0);
-    const double *src[4] = {x0, v0, theta0, omega0};
     double *dst[4] = {Q->x, Q->v, Q->th, Q->w};
-    for (int f = 0; f < 4; f++) {
-        std::fill(buf.begin(), buf.end(), 0.0);
-        if (src[f])
-            for (size_t i = 0; i < n; i++)
-                for (int c = 0; c < 3; c++) buf[4 * i + c] = src[f][3 * i + c];
-        HG_HIP(hipMemcpyAsync(dst[f] + 4 * size_t(offset), buf.data(), buf.size() * 8, hipMemcpyHostToDevice, Q->stream));
-        HG_HIP(hipStreamSynchronize(Q->stream));
-    }
-    HG_HIP(launch_zero_f64(Q->a + 4 * size_t(offset), 4 * n, Q->stream));
-    HG_HIP(launch_zero_f64(Q->al + 4 * size_t(offset), 4 * n, Q->stream));
-    {
-        std::vector<uint32_t> own(n, uint32_t(slot));
-        HG_HIP(hipMemcpyAsync(Q->owner + offset, own.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
-        HG_HIP(hipStreamSynchronize(Q->stream));
-    }
+    for (int f = 0; f < 4; f++)
+        HG_HIP(hipMemcpyAsync(dst[f] + 4 * size_t(offset), prep.q4[f].data(), prep.q4[f].size() * 8,
+                              hipMemcpyHostToDevice, cs));
+    HG_HIP(launch_zero_f64(Q->a + 4 * size_t(offset), 4 * n, cs));
+    HG_HIP(launch_zero_f64(Q->al + 4 * size_t(offset), 4 * n, cs));
+    std::vector<uint32_t> own(n, uint32_t(slot));
+    HG_HIP(hipMemcpyAsync(Q->owner + offset, own.data(), n * 4, hipMemcpyHostToDevice, cs));
+    std::vector<uint32_t> pj, cnt;
     if (Q->cap_s) {
-        std::vector<uint32_t> pj(size_t(Q->cap_s) * n, kNoEntry), cnt(n);
-        for (size_t i = 0; i < n; i++) {
-            cnt[i] = uint32_t(rows[i].size());
-            for (size_t s = 0; s < rows[i].size(); s++) pj[s * n + i] = rows[i][s];
+        pj.assign(size_t(Q->cap_s) * n, kNoEntry);
+        cnt.assign(n, 0u);
+        for (size_t i = 0; i < prep.rows.size(); i++) {
+            cnt[i] = uint32_t(prep.rows[i].size());
+            for (size_t s = 0; s < prep.rows[i].size(); s++)
+                pj[s * n + i] = (prep.rows[i][s] & kBondBit) | (offset + (prep.rows[i][s] & kIdxMask));
         }
         HG_HIP(hipMemcpy2DAsync(Q->pj[0] + offset, Q->P * 4, pj.data(), n * 4, n * 4, Q->cap_s,
-                                hipMemcpyHostToDevice, Q->stream));
-        HG_HIP(hipMemcpyAsync(Q->pcnt[0] + offset, cnt.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
-        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ps[0], size_t(Q->P) * kSpringRec, size_t(offset) * kSpringRec,
+                                hipMemcpyHostToDevice, cs));
+        HG_HIP(hipMemcpyAsync(Q->pcnt[0] + offset, cnt.data(), n * 4, hipMemcpyHostToDevice, cs));
+        k_zero_rows_f64<<<256, 256, 0, cs>>>(Q->ps[0], size_t(Q->P) * kSpringRec, size_t(offset) * kSpringRec,
                                                      n * kSpringRec, Q->cap_s);
         HG_HIP(hipGetLastError());
-        HG_HIP(hipStreamSynchronize(Q->stream));
     }
-    if (Q->cap_n) HG_HIP(launch_zero_u32(Q->ncnt + offset, n, Q->stream));
+    if (Q->cap_n) HG_HIP(launch_zero_u32(Q->ncnt + offset, n, cs));
+    // the host buffers above are read by the copies: wait before they go
+    HG_HIP(hipStreamSynchronize(cs));
     Q->meta[slot] = pool_meta{offset, uint32_t(n), 0, 0, 0, 1, uint32_t(slot), 0};
     Q->step_index[slot] = 0;
     Q->t[slot] = 0.0;
     Q->active[slot] = 1;
     Q->fault[slot] = 0;
-    Q->bonds[slot] = std::move(bonds);
-    Q->skipped[slot] = skipped;
+    Q->bonds[slot] = std::move(prep.bonds);
+    Q->skipped[slot] = prep.skipped;
     Q->order_dirty = true;
     Q->meta_dirty = true;
     *slot_out = slot;
     if (Q->bcnt) {  // neck strengths start infinite
         std::vector<double> inf(Q->bonds[slot].size(), std::numeric_limits<double>::infinity());
-        PG_TRY(hg_pool_set_neck_strengths(Q, slot, inf.data()));
+        PG_TRY(pool_set_necks(Q, slot, inf.data(), cs));
     }
     return HG_OK;
 }
 
+}  // namespace hg
+
+namespace {
+// hg_pool_add; `given_bonds` (with `given_skipped`) replaces the bond search
+// on x0 (a system reloaded from a snapshot keeps the bonds it was built with).
+hg_status pool_add_impl(hg_pool *Q, size_t n, const double *x0, const double *v0, const double *theta0,
+                        const double *omega0, const std::vector<std::pair<uint32_t, uint32_t>> *given_bonds,
+                        size_t given_skipped, int32_t *slot_out) {
+    if (!Q || !x0 || !slot_out) return fail(HG_ERR_INVALID, "hg_pool_add: pool, x0 and slot are required");
+    pool_prep prep;
+    PG_TRY(pool_prepare(Q->p, n, x0, v0, theta0, omega0, given_bonds, given_skipped, prep));
+    return pool_commit(Q, prep, slot_out, Q->stream);
+}
+}  // namespace
+
+extern "C" {
+
+hg_status hg_pool_add(hg_pool *Q, size_t n, const double *x0, const double *v0, const double *theta0,
+                      const double *omega0, int32_t *slot_out) {
+    return pool_add_impl(Q, n, x0, v0, theta0, omega0, nullptr, 0, slot_out);
+}
+
 hg_status hg_pool_set_neck_strengths(hg_pool *Q, int32_t slot, const double *strengths) {
+    return pool_set_necks(Q, slot, strengths, Q ? Q->stream : nullptr);
+}
+
+}  // extern "C"
+
+namespace hg {
+
+// hg_pool_set_neck_strengths with the copies on stream cs (the copy stream
+// while an interval runs: the slot is not in the launch order).
+hg_status pool_set_necks(hg_pool *Q, int32_t slot, const double *strengths, hipStream_t cs) {
     PG_TRY(valid_slot(Q, slot));
     if (!strengths) return fail(HG_ERR_INVALID, "strengths is null");
     if (!Q->bcnt) return fail(HG_ERR_INVALID, "hg_pool_set_neck_strengths: the pool's bonds do not break "
@@ -1214,24 +1305,28 @@ hg_status hg_pool_set_neck_strengths(hg_pool *Q, int32_t slot, const double *str
         }
     }
     HG_HIP(hipMemcpy2DAsync(Q->bj + off, Q->P * 4, bj.data(), n * 4, n * 4, kMaxBondsPerParticle,
-                            hipMemcpyHostToDevice, Q->stream));
+                            hipMemcpyHostToDevice, cs));
     HG_HIP(hipMemcpy2DAsync(Q->bstr + off, Q->P * 8, bs.data(), n * 8, n * 8, kMaxBondsPerParticle,
-                            hipMemcpyHostToDevice, Q->stream));
-    HG_HIP(hipMemcpyAsync(Q->bcnt + off, cnt.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
-    HG_HIP(hipStreamSynchronize(Q->stream));
+                            hipMemcpyHostToDevice, cs));
+    HG_HIP(hipMemcpyAsync(Q->bcnt + off, cnt.data(), n * 4, hipMemcpyHostToDevice, cs));
+    HG_HIP(hipStreamSynchronize(cs));
     return HG_OK;
 }
 
-hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
+}  // namespace hg
+
+namespace hg {
+
+hg_status pool_release(hg_pool *Q, int32_t slot, hipStream_t cs) {
     PG_TRY(valid_slot(Q, slot));
     const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
-    HG_HIP(hipStreamSynchronize(Q->stream));
     {
         std::vector<uint32_t> none(n, kNoEntry);
-        HG_HIP(hipMemcpyAsync(Q->owner + off, none.data(), n * 4, hipMemcpyHostToDevice, Q->stream));
-        HG_HIP(hipStreamSynchronize(Q->stream));
+        HG_HIP(hipMemcpyAsync(Q->owner + off, none.data(), n * 4, hipMemcpyHostToDevice, cs));
+        HG_HIP(hipStreamSynchronize(cs));
     }
     Q->active[slot] = 0;
+    Q->parked[slot] = 0;
     Q->fault[slot] = 0;
     Q->meta[slot] = pool_meta{0, 0, 0, 0, 0, 0, 0, 0};
     Q->bonds[slot].clear();
@@ -1257,7 +1352,41 @@ hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
     return HG_OK;
 }
 
-hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *verts, const double *v_facet) {
+hg_status pool_read_state(hg_pool *Q, int32_t slot, double *x, double *v, double *theta, double *omega,
+                          hipStream_t cs) {
+    PG_TRY(valid_slot(Q, slot));
+    const uint32_t off = Q->meta[slot].offset, n = Q->meta[slot].n;
+    std::vector<double> buf(4 * size_t(n));
+    const double *src[4] = {Q->x, Q->v, Q->th, Q->w};
+    double *dst[4] = {x, v, theta, omega};
+    for (int f = 0; f < 4; f++) {
+        if (!dst[f]) continue;
+        HG_HIP(hipMemcpyAsync(buf.data(), src[f] + 4 * size_t(off), buf.size() * 8, hipMemcpyDeviceToHost, cs));
+        HG_HIP(hipStreamSynchronize(cs));
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) dst[f][3 * i + c] = buf[4 * i + c];
+    }
+    return HG_OK;
+}
+
+hipStream_t pool_copy_stream(hg_pool *Q) { return Q->stream_c; }
+
+}  // namespace hg
+
+extern "C" {
+
+hg_status hg_pool_remove(hg_pool *Q, int32_t slot) {
+    PG_TRY(valid_slot(Q, slot));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    return pool_release(Q, slot, Q->stream);
+}
+
+}  // extern "C"
+
+namespace hg {
+
+hg_status pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *verts, const double *v_facet,
+                          hipStream_t cs) {
     PG_TRY(valid_slot(Q, slot));
     if (nf > Q->MF) return fail(HG_ERR_INVALID, "hg_pool_set_facets: more facets than the pool's max_facets");
     if (nf > 0 && (!verts || !v_facet)) return fail(HG_ERR_INVALID, "hg_pool_set_facets: verts and v_facet required");
@@ -1273,16 +1402,24 @@ hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *
     }
     if (nf > 0) {
         HG_HIP(hipMemcpyAsync(Q->facets + size_t(slot) * Q->MF, F.data(), nf * sizeof(facet_dev),
-                              hipMemcpyHostToDevice, Q->stream));
-        k_zero_rows_f64<<<256, 256, 0, Q->stream>>>(Q->ss, Q->P, Q->meta[slot].offset, Q->meta[slot].n,
+                              hipMemcpyHostToDevice, cs));
+        k_zero_rows_f64<<<256, 256, 0, cs>>>(Q->ss, Q->P, Q->meta[slot].offset, Q->meta[slot].n,
                                                     uint32_t(nf * 9));
         HG_HIP(hipGetLastError());
     }
-    HG_HIP(hipStreamSynchronize(Q->stream));
+    HG_HIP(hipStreamSynchronize(cs));
     Q->meta[slot].nf = uint32_t(nf);
     Q->meta[slot].acc_valid = 0;
     Q->meta_dirty = true;
     return HG_OK;
+}
+
+}  // namespace hg
+
+extern "C" {
+
+hg_status hg_pool_set_facets(hg_pool *Q, int32_t slot, size_t nf, const double *verts, const double *v_facet) {
+    return pool_set_facets(Q, slot, nf, verts, v_facet, Q ? Q->stream : nullptr);
 }
 
 namespace {
@@ -1292,7 +1429,7 @@ namespace {
 hg_status refresh_order(hg_pool *Q) {
     Q->order.clear();
     for (size_t s = 0; s < Q->S; s++)
-        if (Q->active[s] && !Q->fault[s]) Q->order.push_back(uint32_t(s));
+        if (Q->active[s] && !Q->fault[s] && !Q->parked[s]) Q->order.push_back(uint32_t(s));
     std::stable_sort(Q->order.begin(), Q->order.end(),
                      [&](uint32_t a, uint32_t b) { return Q->meta[a].n > Q->meta[b].n; });
     std::vector<wide_block> map;
@@ -1360,155 +1497,182 @@ void tune_record(hg_pool *Q, double seconds_per_step) {
 }
 }  // namespace
 
-hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
-    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
-    Q->last_chunk_ms = Q->last_rebuild_ms = Q->last_wide_ms = 0.0;
-    Q->last_chunks = Q->last_wide_launches = Q->last_chunk_steps = 0;
-    if (nsteps == 0) return HG_OK;
+}  // extern "C"
+
+namespace hg {
+
+hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_every, uint32_t *len_out) {
+    *len_out = 0;
+    if (Q->fl.active) return fail(HG_ERR_INVALID, "pool: an interval is already queued");
+    if (Q->tune_phase == TUNE_START) tune_begin(Q);
+    if (Q->tune_phase == TUNE_RUNNING) tune_select(Q);
+    if (Q->order_dirty) PG_TRY(refresh_order(Q));
+    if (Q->order.empty() || max_len == 0) return HG_OK;
     const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     const bool cadence = nl && rebuild_every;
-    const uint64_t chunk = cadence ? rebuild_every : 2000;
     const pool_view V = make_pool_view(Q);
     const double half = dt / 2.0;
-    std::vector<int32_t> faulted;  // systems that stopped during this call
-    if (Q->tune_phase == TUNE_START) tune_begin(Q);
-    for (uint64_t done = 0; done < nsteps;) {
-        if (Q->tune_phase == TUNE_RUNNING) tune_select(Q);
-        if (Q->order_dirty) PG_TRY(refresh_order(Q));
-        if (Q->order.empty()) break;
-        // one launch runs every system up to the nearest step at which any of
-        // them is due for a rebuild, as hg_step does per system
-        uint64_t len64 = std::min<uint64_t>(chunk, nsteps - done);
-        if (cadence)
-            for (uint32_t s : Q->order) len64 = std::min<uint64_t>(len64, rebuild_every - Q->step_index[s] % rebuild_every);
-        const uint32_t len = uint32_t(len64);
-        const uint32_t count = Q->narrow_count;
-        const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
-        bool any_wide_initial = false;
-        for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
-        const auto t_interval = std::chrono::steady_clock::now();
-        if (nl) {
-            bool any = false;
-            for (uint32_t s : Q->order) {
-                const bool due = Q->meta[s].rebuild || (cadence && Q->step_index[s] % rebuild_every == 0);
-                Q->meta[s].rebuild = due ? 1u : 0u;
-                any |= due;
-            }
-            if (any) {
-                PG_TRY(upload_meta(Q));
-                HG_HIP(hipEventRecord(Q->ev[0], Q->stream));
-                PG_TRY(rebuild_due(Q));
-                HG_HIP(hipEventRecord(Q->ev[1], Q->stream));
-                HG_HIP(hipEventSynchronize(Q->ev[1]));
-                float ms = 0.f;
-                HG_HIP(hipEventElapsedTime(&ms, Q->ev[0], Q->ev[1]));
-                Q->last_rebuild_ms += ms;
-                for (uint32_t s : Q->order) Q->meta[s].rebuild = 0;
-            }
-        }
-        PG_TRY(upload_meta(Q));
-        // wide systems: per-step launches on their own CUs, beside the chunk
-        // kernel on the others (all CUs go to whichever kind is alone)
-        const bool split = Q->wide_blocks && count;
-        hipStream_t sc = Q->stream;
-        if (Q->wide_blocks) {
-            hipStream_t s2 = split ? Q->stream_w : Q->stream;
-            HG_HIP(hipEventRecord(Q->wev[0], s2));
-            auto force = [&](int pass) {
-                return nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2)
-                          : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2);
-            };
-            if (any_wide_initial) {
-                HG_HIP(force(-1));
-                Q->last_wide_launches++;
-            }
-            // the interval's 2*len+1 launches: recorded once into a HIP graph
-            // per (interval length, dt, wide set) and replayed; launched one
-            // by one when graphs are off
-            auto enqueue = [&](hipStream_t st_) -> hipError_t {
-                for (uint32_t st = 0; st < len; st++) {
-                    k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, st > 0 ? 1 : 0,
-                                                                                    half, dt);
-                    hipError_t e = hipGetLastError();
-                    if (e != hipSuccess) return e;
-                    e = nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks,
-                                                                         int(st), st_)
-                           : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, int(st), st_);
-                    if (e != hipSuccess) return e;
-                }
-                k_pool_wide_final_kick<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, half, dt);
-                return hipGetLastError();
-            };
-            if (Q->wide_graph) {
-                if (!Q->wg_exec || Q->wg_len != len || Q->wg_dt != dt || Q->wg_blocks != Q->wide_blocks) {
-                    if (Q->wg_exec) HG_HIP(hipGraphExecDestroy(Q->wg_exec));
-                    Q->wg_exec = nullptr;
-                    hipGraph_t graph = nullptr;
-                    HG_HIP(hipStreamBeginCapture(Q->stream_w, hipStreamCaptureModeThreadLocal));
-                    const hipError_t ec = enqueue(Q->stream_w);
-                    const hipError_t ee = hipStreamEndCapture(Q->stream_w, &graph);
-                    HG_HIP(ec);
-                    HG_HIP(ee);
-                    const hipError_t ei = hipGraphInstantiate(&Q->wg_exec, graph, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(graph);
-                    HG_HIP(ei);
-                    Q->wg_len = len;
-                    Q->wg_dt = dt;
-                    Q->wg_blocks = Q->wide_blocks;
-                }
-                HG_HIP(hipGraphLaunch(Q->wg_exec, s2));
-            } else {
-                HG_HIP(enqueue(s2));
-            }
-            Q->last_wide_launches += 2 * len + 1;
-            HG_HIP(hipEventRecord(Q->wev[1], s2));
-        }
-        if (count) {
-            const unsigned g = split ? std::min(grid, unsigned(2 * (Q->n_cus - Q->wide_cus))) : grid;
-            HG_HIP(launch_zero_u32(Q->counter_d, 1, sc));
-            HG_HIP(hipEventRecord(Q->ev[2], sc));
-            hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len,
-                                                                       dt, g, Q->lds_cap, sc)
-                              : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, g,
-                                                             Q->lds_cap, sc);
-            HG_HIP(e);
-            HG_HIP(hipEventRecord(Q->ev[3], sc));
-        }
-        if (split) HG_HIP(hipStreamWaitEvent(Q->stream, Q->wev[1], 0));
-        HG_HIP(hipStreamSynchronize(Q->stream));
-        // host mirror of what the kernels did to each system, before any
-        // fault is read, so that the systems that go on keep a true mirror
+    // one launch runs every system up to the nearest step at which any of
+    // them is due for a rebuild, as hg_step does per system
+    uint64_t len64 = std::min<uint64_t>(cadence ? rebuild_every : 2000, max_len);
+    if (cadence)
+        for (uint32_t s : Q->order) len64 = std::min<uint64_t>(len64, rebuild_every - Q->step_index[s] % rebuild_every);
+    const uint32_t len = uint32_t(len64);
+    const uint32_t count = Q->narrow_count;
+    const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
+    bool any_wide_initial = false;
+    for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
+    const auto t_interval = std::chrono::steady_clock::now();
+    // The interval's device work is queued without waiting in between: the
+    // meta (with the rebuild flags of the systems due) goes up from pinned
+    // memory, the rebuild follows on the pool's stream, and the wide launches
+    // on the second stream wait for it by an event.
+    HG_HIP(hipEventRecord(Q->evb[0], Q->stream));
+    bool rebuilt = false;
+    if (nl)
         for (uint32_t s : Q->order) {
-            pool_meta &M = Q->meta[s];
-            if (Q->cap_s) {
-                if (!M.acc_valid) M.cur ^= 1u;
-                M.cur ^= (len & 1u);
-            }
-            M.acc_valid = 1;
-            Q->step_index[s] += len;
-            for (uint32_t k = 0; k < len; k++) Q->t[s] = Q->t[s] + dt;
+            const bool due = Q->meta[s].rebuild || (cadence && Q->step_index[s] % rebuild_every == 0);
+            Q->meta[s].rebuild = due ? 1u : 0u;
+            rebuilt |= due;
         }
-        Q->meta_dirty = true;
-        PG_TRY(collect_faults(Q, faulted));
-        float ms = 0.f;
-        if (count) {
-            HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
-            Q->last_chunk_ms += ms;
-            Q->last_chunks++;
-            Q->last_chunk_steps += len;
-        }
-        if (Q->wide_blocks) {
-            HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
-            Q->last_wide_ms += ms;
-        }
-        if (Q->tune_phase == TUNE_RUNNING)
-            tune_record(Q, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_interval).count() / len);
-        done += len;
+    std::memcpy(Q->meta_pinned, Q->meta.data(), Q->S * sizeof(pool_meta));
+    HG_HIP(hipMemcpyAsync(Q->meta_d, Q->meta_pinned, Q->S * sizeof(pool_meta), hipMemcpyHostToDevice, Q->stream));
+    Q->meta_dirty = false;
+    if (rebuilt) {
+        HG_HIP(hipEventRecord(Q->ev[0], Q->stream));
+        PG_TRY(rebuild_due(Q));
+        for (uint32_t s : Q->order) Q->meta[s].rebuild = 0;
     }
-    PG_TRY(upload_meta(Q));
+    HG_HIP(hipEventRecord(Q->ev[1], Q->stream));
+    // wide systems: per-step launches on their own CUs, beside the chunk
+    // kernel on the others (all CUs go to whichever kind is alone)
+    const bool split = Q->wide_blocks && count;
+    hipStream_t sc = Q->stream;
+    if (Q->wide_blocks) {
+        hipStream_t s2 = split ? Q->stream_w : Q->stream;
+        if (split) HG_HIP(hipStreamWaitEvent(Q->stream_w, Q->ev[1], 0));
+        HG_HIP(hipEventRecord(Q->wev[0], s2));
+        auto force = [&](int pass) {
+            return nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2)
+                      : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, pass, s2);
+        };
+        if (any_wide_initial) {
+            HG_HIP(force(-1));
+            Q->last_wide_launches++;
+        }
+        // the interval's 2*len+1 launches: recorded once into a HIP graph
+        // per (interval length, dt, wide set) and replayed; launched one
+        // by one when graphs are off
+        auto enqueue = [&](hipStream_t st_) -> hipError_t {
+            for (uint32_t st = 0; st < len; st++) {
+                k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, st > 0 ? 1 : 0,
+                                                                                half, dt);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+                e = nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->wide_map_d, Q->wide_blocks,
+                                                                     int(st), st_)
+                       : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, V, Q->wide_map_d, Q->wide_blocks, int(st), st_);
+                if (e != hipSuccess) return e;
+            }
+            k_pool_wide_final_kick<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, half, dt);
+            return hipGetLastError();
+        };
+        if (Q->wide_graph) {
+            if (!Q->wg_exec || Q->wg_len != len || Q->wg_dt != dt || Q->wg_blocks != Q->wide_blocks) {
+                if (Q->wg_exec) HG_HIP(hipGraphExecDestroy(Q->wg_exec));
+                Q->wg_exec = nullptr;
+                hipGraph_t graph = nullptr;
+                HG_HIP(hipStreamBeginCapture(Q->stream_w, hipStreamCaptureModeThreadLocal));
+                const hipError_t ec = enqueue(Q->stream_w);
+                const hipError_t ee = hipStreamEndCapture(Q->stream_w, &graph);
+                HG_HIP(ec);
+                HG_HIP(ee);
+                const hipError_t ei = hipGraphInstantiate(&Q->wg_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HG_HIP(ei);
+                Q->wg_len = len;
+                Q->wg_dt = dt;
+                Q->wg_blocks = Q->wide_blocks;
+            }
+            HG_HIP(hipGraphLaunch(Q->wg_exec, s2));
+        } else {
+            HG_HIP(enqueue(s2));
+        }
+        Q->last_wide_launches += 2 * len + 1;
+        HG_HIP(hipEventRecord(Q->wev[1], s2));
+    }
+    if (count) {
+        const unsigned g = split ? std::min(grid, unsigned(2 * (Q->n_cus - Q->wide_cus))) : grid;
+        HG_HIP(launch_zero_u32(Q->counter_d, 1, sc));
+        HG_HIP(hipEventRecord(Q->ev[2], sc));
+        bool facets = false;
+        for (size_t k = Q->order.size() - count; k < Q->order.size(); k++) facets |= Q->meta[Q->order[k]].nf > 0;
+        hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len,
+                                                                   dt, g, Q->lds_cap, facets, sc)
+                          : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, g,
+                                                         Q->lds_cap, facets, sc);
+        HG_HIP(e);
+        HG_HIP(hipEventRecord(Q->ev[3], sc));
+    }
+    if (split) HG_HIP(hipStreamWaitEvent(Q->stream, Q->wev[1], 0));
+    HG_HIP(hipEventRecord(Q->evb[1], Q->stream));
+    Q->fl.active = true;
+    Q->fl.len = len;
+    Q->fl.count = count;
+    Q->fl.rebuilt = rebuilt;
+    Q->fl.wide = Q->wide_blocks != 0;
+    Q->fl.dt = dt;
+    Q->fl.t0 = t_interval;
+    *len_out = len;
+    return HG_OK;
+}
+
+hg_status pool_end(hg_pool *Q, std::vector<int32_t> &faulted) {
+    if (!Q->fl.active) return HG_OK;
+    Q->fl.active = false;
+    const uint32_t len = Q->fl.len;
+    const double dt = Q->fl.dt;
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    // host mirror of what the kernels did to each system, before any fault is
+    // read, so that the systems that go on keep a true mirror
+    for (uint32_t s : Q->order) {
+        pool_meta &M = Q->meta[s];
+        if (Q->cap_s) {
+            if (!M.acc_valid) M.cur ^= 1u;
+            M.cur ^= (len & 1u);
+        }
+        M.acc_valid = 1;
+        Q->step_index[s] += len;
+        for (uint32_t k = 0; k < len; k++) Q->t[s] = Q->t[s] + dt;
+    }
+    Q->meta_dirty = true;
+    PG_TRY(collect_faults(Q, faulted));
+    float ms = 0.f;
+    if (Q->fl.rebuilt) {
+        HG_HIP(hipEventElapsedTime(&ms, Q->ev[0], Q->ev[1]));
+        Q->last_rebuild_ms += ms;
+    }
+    HG_HIP(hipEventElapsedTime(&ms, Q->evb[0], Q->evb[1]));
+    Q->last_busy_ms += ms;
+    if (Q->fl.count) {
+        HG_HIP(hipEventElapsedTime(&ms, Q->ev[2], Q->ev[3]));
+        Q->last_chunk_ms += ms;
+        Q->last_chunks++;
+        Q->last_chunk_steps += len;
+    }
+    if (Q->fl.wide) {
+        HG_HIP(hipEventElapsedTime(&ms, Q->wev[0], Q->wev[1]));
+        Q->last_wide_ms += ms;
+    }
+    if (Q->tune_phase == TUNE_RUNNING)
+        tune_record(Q, std::chrono::duration<double>(std::chrono::steady_clock::now() - Q->fl.t0).count() / len);
+    return HG_OK;
+}
+
+hg_status pool_fault_status(hg_pool *Q, const std::vector<int32_t> &faulted, const char *where) {
     if (faulted.empty()) return HG_OK;
     bool numeric = false;
-    std::string msg = "hg_pool_step: stopped system";
+    std::string msg = std::string(where) + ": stopped system";
     msg += faulted.size() > 1 ? "s" : "";
     for (int32_t s : faulted) {
         const int f = Q->fault[s];
@@ -1525,6 +1689,36 @@ hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_
         msg += why + ")";
     }
     return fail(numeric ? HG_ERR_NUMERIC : HG_ERR_CAPACITY, msg + "; the other systems completed the steps");
+}
+
+void pool_reset_profile(hg_pool *Q) {
+    Q->last_chunk_ms = Q->last_rebuild_ms = Q->last_wide_ms = Q->last_busy_ms = 0.0;
+    Q->last_chunks = Q->last_wide_launches = Q->last_chunk_steps = 0;
+}
+
+void pool_park(hg_pool *Q, int32_t slot) {
+    Q->parked[slot] = 1;
+    Q->order_dirty = true;
+}
+
+}  // namespace hg
+
+extern "C" {
+
+hg_status hg_pool_step(hg_pool *Q, double dt, uint64_t nsteps, uint32_t rebuild_every) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    pool_reset_profile(Q);
+    if (nsteps == 0) return HG_OK;
+    std::vector<int32_t> faulted;  // systems that stopped during this call
+    for (uint64_t done = 0; done < nsteps;) {
+        uint32_t len = 0;
+        PG_TRY(pool_begin(Q, dt, nsteps - done, rebuild_every, &len));
+        if (len == 0) break;
+        PG_TRY(pool_end(Q, faulted));
+        done += len;
+    }
+    PG_TRY(upload_meta(Q));
+    return pool_fault_status(Q, faulted, "hg_pool_step");
 }
 
 hg_status hg_pool_fault(hg_pool *Q, int32_t slot, int *fault) {
@@ -1567,6 +1761,12 @@ hg_status hg_pool_set_wide_cus(hg_pool *Q, int wide_cus) {
     if (wide_cus < 1 || wide_cus >= Q->n_cus)
         return fail(HG_ERR_INVALID, "hg_pool_set_wide_cus: need 0 < wide_cus < " + std::to_string(Q->n_cus));
     Q->wide_cus = wide_cus;
+    return HG_OK;
+}
+
+hg_status hg_pool_last_busy(hg_pool *Q, double *busy_ms) {
+    if (!Q || !busy_ms) return fail(HG_ERR_INVALID, "pool and busy_ms are required");
+    *busy_ms = Q->last_busy_ms;
     return HG_OK;
 }
 
@@ -1742,6 +1942,178 @@ hg_status hg_pool_last_step_profile(hg_pool *Q, double *chunk_ms, double *rebuil
     if (rebuild_ms) *rebuild_ms = Q->last_rebuild_ms;
     if (chunks) *chunks = Q->last_chunks;
     if (chunk_steps) *chunk_steps = Q->last_chunk_steps;
+    return HG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+hg_status upload_rows(hg_pool *Q, T *dev, uint32_t offset, uint32_t n, uint32_t rows, const std::vector<T> &host,
+                      size_t width = 1) {
+    if (rows == 0 || n == 0) return HG_OK;
+    const size_t row = n * width * sizeof(T);
+    HG_HIP(hipMemcpy2DAsync(dev + offset * width, Q->P * width * sizeof(T), host.data(), row, row, rows,
+                            hipMemcpyHostToDevice, Q->stream));
+    HG_HIP(hipStreamSynchronize(Q->stream));
+    return HG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+hg_status hg_pool_save(hg_pool *Q, int32_t slot, void *buf, size_t cap, size_t *size) {
+    PG_TRY(valid_slot(Q, slot));
+    if (!size) return fail(HG_ERR_INVALID, "hg_pool_save: size is required");
+    const pool_meta &M = Q->meta[slot];
+    const uint32_t off = M.offset, n = M.n, cur = M.cur;
+    snapshot z;
+    z.p = Q->p;
+    z.n = n;
+    z.t = Q->t[slot];
+    z.step_index = Q->step_index[slot];
+    z.acc_valid = M.acc_valid;
+    const bool nl = Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
+    z.needs_rebuild = nl && M.rebuild ? 1u : 0u;
+    const double *fields[6] = {Q->x, Q->v, Q->th, Q->w, Q->a, Q->al};
+    std::vector<double> *dst[6] = {&z.x, &z.v, &z.th, &z.w, &z.a, &z.al};
+    for (int f = 0; f < 6; f++) {
+        std::vector<double> q4(4 * size_t(n));
+        PG_TRY(download_rows(Q, fields[f] + 4 * size_t(off), 0, n, 1, q4.data(), 4));
+        *dst[f] = pack3(q4, n);
+    }
+    if (nl && !M.rebuild) {
+        std::vector<uint32_t> nb(size_t(Q->cap_n) * n);
+        z.has_nl = true;
+        z.nl_count.resize(n);
+        PG_TRY(download_rows(Q, Q->nb, off, n, Q->cap_n, nb.data()));
+        PG_TRY(download_rows(Q, Q->ncnt, off, n, 1, z.nl_count.data()));
+        rows_to_csr(nb.data(), z.nl_count.data(), n, 1, 1, z.nl_ids);
+        shift_ids(z.nl_ids, off);
+    }
+    if (Q->cap_s) {
+        std::vector<uint32_t> pj(size_t(Q->cap_s) * n);
+        std::vector<double> ps(size_t(Q->cap_s) * kSpringRec * n);
+        z.has_table = true;
+        z.pt_count.resize(n);
+        PG_TRY(download_rows(Q, Q->pj[cur], off, n, Q->cap_s, pj.data()));
+        PG_TRY(download_rows(Q, Q->ps[cur], off, n, Q->cap_s, ps.data(), kSpringRec));
+        PG_TRY(download_rows(Q, Q->pcnt[cur], off, n, 1, z.pt_count.data()));
+        rows_to_csr(pj.data(), z.pt_count.data(), n, 1, 1, z.pt_tags);
+        rows_to_csr(ps.data(), z.pt_count.data(), n, kSpringRec, 9, z.pt_springs);
+        shift_ids(z.pt_tags, off);
+    }
+    for (auto &b : Q->bonds[slot]) {
+        z.bonds.push_back(b.first);
+        z.bonds.push_back(b.second);
+    }
+    z.skipped = Q->skipped[slot];
+    z.nf = M.nf;
+    if (M.nf > 0) {
+        z.facets.resize(M.nf);
+        z.ss.resize(size_t(M.nf) * 9 * n);
+        HG_HIP(hipMemcpyAsync(z.facets.data(), Q->facets + size_t(slot) * Q->MF, M.nf * sizeof(facet_dev),
+                              hipMemcpyDeviceToHost, Q->stream));
+        HG_HIP(hipStreamSynchronize(Q->stream));
+        PG_TRY(download_rows(Q, Q->ss, off, n, M.nf * 9, z.ss.data()));
+    }
+    if (Q->bstr) {
+        std::vector<uint32_t> bj(size_t(kMaxBondsPerParticle) * n);
+        std::vector<double> bs(size_t(kMaxBondsPerParticle) * n);
+        z.has_neck = true;
+        z.neck_count.resize(n);
+        PG_TRY(download_rows(Q, Q->bj, off, n, kMaxBondsPerParticle, bj.data()));
+        PG_TRY(download_rows(Q, Q->bstr, off, n, kMaxBondsPerParticle, bs.data()));
+        PG_TRY(download_rows(Q, Q->bcnt, off, n, 1, z.neck_count.data()));
+        rows_to_csr(bj.data(), z.neck_count.data(), n, 1, 1, z.neck_j);
+        rows_to_csr(bs.data(), z.neck_count.data(), n, 1, 1, z.neck_str);
+        shift_ids(z.neck_j, off);
+    }
+    const std::vector<uint8_t> bytes = encode_snapshot(z);
+    *size = bytes.size();
+    if (!buf) return HG_OK;
+    if (cap < bytes.size())
+        return fail(HG_ERR_CAPACITY, "hg_pool_save: the snapshot needs " + std::to_string(bytes.size()) + " bytes");
+    std::memcpy(buf, bytes.data(), bytes.size());
+    return HG_OK;
+}
+
+hg_status hg_pool_load(hg_pool *Q, const void *buf, size_t size, int32_t *slot_out) {
+    if (!Q || !slot_out) return fail(HG_ERR_INVALID, "hg_pool_load: pool and slot are required");
+    snapshot z;
+    PG_TRY(decode_snapshot(buf, size, z));
+    if (std::memcmp(&z.p, &Q->p, sizeof(hg_params)) != 0)
+        return fail(HG_ERR_INVALID, "hg_pool_load: the snapshot was taken with other parameters than the pool's");
+    if (z.has_legacy) return fail(HG_ERR_INVALID, "hg_pool_load: pools do not run the legacy sinter_functor");
+    if (z.nf > Q->MF) return fail(HG_ERR_INVALID, "hg_pool_load: more facets than the pool's max_facets");
+    if ((z.has_nl && max_count(z.nl_count) > Q->cap_n) || (z.has_table && max_count(z.pt_count) > Q->cap_s))
+        return fail(HG_ERR_CAPACITY, "hg_pool_load: a row of the snapshot exceeds the pool's capacity");
+    const uint32_t n = uint32_t(z.n);
+    std::vector<std::pair<uint32_t, uint32_t>> bonds(z.bonds.size() / 2);
+    for (size_t b = 0; b < bonds.size(); b++) bonds[b] = {z.bonds[2 * b], z.bonds[2 * b + 1]};
+    int32_t slot = -1;
+    PG_TRY(pool_add_impl(Q, n, z.x.data(), z.v.data(), z.th.data(), z.w.data(), &bonds, z.skipped, &slot));
+    const uint32_t off = Q->meta[slot].offset;
+    auto fill = [&]() -> hg_status {
+        for (auto [dev, host] : {std::pair<double *, const std::vector<double> *>{Q->a, &z.a}, {Q->al, &z.al}}) {
+            std::vector<double> q4(4 * size_t(n), 0.0);
+            for (uint32_t i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) q4[4 * size_t(i) + c] = (*host)[3 * size_t(i) + c];
+            HG_HIP(hipMemcpyAsync(dev + 4 * size_t(off), q4.data(), q4.size() * 8, hipMemcpyHostToDevice, Q->stream));
+            HG_HIP(hipStreamSynchronize(Q->stream));
+        }
+        pool_meta &M = Q->meta[slot];
+        if (z.has_table && Q->cap_s) {
+            std::vector<uint32_t> tags = z.pt_tags, pj;
+            std::vector<double> ps;
+            shift_ids(tags, -int64_t(off));
+            csr_to_rows(z.pt_count, tags, n, Q->cap_s, 1, 1, kNoEntry, pj);
+            csr_to_rows(z.pt_count, z.pt_springs, n, Q->cap_s, kSpringRec, 9, 0.0, ps);
+            M.cur = 0;
+            PG_TRY(upload_rows(Q, Q->pj[0], off, n, Q->cap_s, pj));
+            PG_TRY(upload_rows(Q, Q->ps[0], off, n, Q->cap_s, ps, kSpringRec));
+            PG_TRY(upload_rows(Q, Q->pcnt[0], off, n, 1, z.pt_count));
+        }
+        if (Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST && z.has_nl) {
+            std::vector<uint32_t> ids = z.nl_ids, nb;
+            shift_ids(ids, -int64_t(off));
+            csr_to_rows(z.nl_count, ids, n, Q->cap_n, 1, 1, 0u, nb);
+            PG_TRY(upload_rows(Q, Q->nb, off, n, Q->cap_n, nb));
+            PG_TRY(upload_rows(Q, Q->ncnt, off, n, 1, z.nl_count));
+            M.rebuild = 0;
+        }
+        if (z.nf > 0) {
+            HG_HIP(hipMemcpyAsync(Q->facets + size_t(slot) * Q->MF, z.facets.data(), z.nf * sizeof(facet_dev),
+                                  hipMemcpyHostToDevice, Q->stream));
+            HG_HIP(hipStreamSynchronize(Q->stream));
+            PG_TRY(upload_rows(Q, Q->ss, off, n, z.nf * 9, z.ss));
+            M.nf = z.nf;
+        }
+        if (z.has_neck && Q->bcnt) {
+            if (max_count(z.neck_count) > kMaxBondsPerParticle) return fail(HG_ERR_INVALID, "hg_pool_load: bad neck rows");
+            std::vector<uint32_t> ids = z.neck_j, bj;
+            std::vector<double> bs;
+            shift_ids(ids, -int64_t(off));
+            csr_to_rows(z.neck_count, ids, n, kMaxBondsPerParticle, 1, 1, kNoEntry, bj);
+            csr_to_rows(z.neck_count, z.neck_str, n, kMaxBondsPerParticle, 1, 1, 0.0, bs);
+            PG_TRY(upload_rows(Q, Q->bj, off, n, kMaxBondsPerParticle, bj));
+            PG_TRY(upload_rows(Q, Q->bstr, off, n, kMaxBondsPerParticle, bs));
+            PG_TRY(upload_rows(Q, Q->bcnt, off, n, 1, z.neck_count));
+        }
+        M.acc_valid = z.acc_valid;
+        Q->t[slot] = z.t;
+        Q->step_index[slot] = z.step_index;
+        Q->meta_dirty = true;
+        Q->order_dirty = true;
+        return upload_meta(Q);
+    };
+    const hg_status st = fill();
+    if (st != HG_OK) {
+        const std::string msg = hg_last_error();
+        (void)hg_pool_remove(Q, slot);
+        return fail(st, msg);
+    }
+    *slot_out = slot;
     return HG_OK;
 }
 

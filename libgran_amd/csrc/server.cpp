@@ -1,14 +1,23 @@
 // Continuous-batching server over a pool (include/hipgran.h, "Server").
 //
-// One worker thread owns the pool. Each round it admits queued requests in
-// arrival order while the head fits the arena and a system slot is free,
-// advances every resident request by one quantum (one hg_pool_step: a
-// persistent kernel launch per rebuild interval), and retires the requests
-// that have run their steps: final state copied to host memory, arena range
-// and slot freed for the next request. Submissions from the producer thread
-// overlap with the device work of earlier requests.
+// Producers submit requests; the host work of each (bond search, pair-state
+// rows, state records) runs in hg_server_submit on the producer's thread.
+// One worker thread owns the pool and advances it one rebuild interval per
+// round (at most `quantum` steps, fewer when a request finishes sooner):
+//   1. the interval's launches are queued for the resident requests
+//      (pool_begin) and the worker does not wait for them;
+//   2. while they run, on the pool's copy stream: the requests that finished
+//      in the previous round are read back and their ranges freed, and the
+//      queued requests are uploaded in arrival order as long as the head fits
+//      the arena and a slot is free (they join the launch order next round);
+//   3. the worker waits for the interval (pool_end). A request whose system
+//      faulted (NaN/Inf, or a table overflow) fails alone with HG_ERR_NUMERIC
+//      or HG_ERR_CAPACITY; a request that has run all its steps leaves the
+//      launch order, to be read back during the next round.
+// Spring state stays on the device for the whole life of a request.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -20,6 +29,7 @@
 #include <vector>
 
 #include "hg_internal.h"
+#include "pool_internal.h"
 
 namespace {
 
@@ -28,11 +38,13 @@ using clock_type = std::chrono::steady_clock;
 struct request {
     uint64_t id = 0;
     size_t n = 0;
-    std::vector<double> x, v, th, w;  // inputs, then the final state
+    hg::pool_prep prep;               // host work done at submission
+    std::vector<double> x, v, th, w;  // the final state
     std::vector<double> verts, vfacet;
     size_t nf = 0;
     uint64_t nsteps = 0;
     int state = HG_REQ_QUEUED;
+    hg_status status = HG_OK;  // why it failed
     int32_t slot = -1;
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::string error;
@@ -52,8 +64,7 @@ struct hg_server {
     std::condition_variable cv_work;  // worker: new submissions or stop
     std::condition_variable cv_done;  // waiters: a request finished
     std::deque<uint64_t> queue;       // queued ids, arrival order
-    std::map<uint64_t, request> reqs;
-    std::vector<uint64_t> running;
+    std::map<uint64_t, request> reqs;  // nodes stay put: the worker keeps pointers to live requests
     uint64_t next_id = 1;
     bool stop = false;
     std::string fatal;  // set when the worker stopped on an error
@@ -63,105 +74,202 @@ struct hg_server {
 
     double now() const { return std::chrono::duration<double>(clock_type::now() - t0).count(); }
     void run();
+    void stop_all(const std::string &why, const std::vector<request *> &resident);
 };
+
+// Fails every request not yet finished, after an error the pool cannot go on
+// from (a HIP error); the caller holds no lock.
+void hg_server::stop_all(const std::string &why, const std::vector<request *> &resident) {
+    std::lock_guard<std::mutex> lk(mu);
+    fatal = why;
+    for (request *r : resident) {
+        r->state = HG_REQ_FAILED;
+        r->status = HG_ERR_HIP;
+        r->error = why;
+    }
+    for (uint64_t id : queue) {
+        request &r = reqs[id];
+        r.state = HG_REQ_FAILED;
+        r.status = HG_ERR_HIP;
+        r.error = why;
+    }
+    queue.clear();
+    cv_done.notify_all();
+}
 
 void hg_server::run() {
     if (hipSetDevice(device) != hipSuccess) {
-        std::lock_guard<std::mutex> lk(mu);
-        fatal = "hipSetDevice failed in the server worker";
-        cv_done.notify_all();
+        stop_all("hipSetDevice failed in the server worker", {});
         return;
     }
+    const hipStream_t cs = hg::pool_copy_stream(pool);
+    std::vector<request *> running, retiring, admitted;
+    auto all_resident = [&] {
+        std::vector<request *> v(running);
+        v.insert(v.end(), retiring.begin(), retiring.end());
+        v.insert(v.end(), admitted.begin(), admitted.end());
+        return v;
+    };
     for (;;) {
-        // admission: strict arrival order while the head fits
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+            cv_work.wait(lk, [&] { return stop || !queue.empty() || !running.empty() || !retiring.empty(); });
             if (stop) return;
-            while (!queue.empty()) {
-                request &r = reqs[queue.front()];
-                int32_t slot = -1;
-                hg_status st = hg_pool_add(pool, r.n, r.x.data(), r.v.data(), r.th.data(), r.w.data(), &slot);
-                // no room now: wait for a retirement, unless nothing is resident
-                // (then the request can never be admitted and fails)
-                if (st == HG_ERR_CAPACITY && !running.empty()) break;
-                if (st == HG_OK && r.nf) st = hg_pool_set_facets(pool, slot, r.nf, r.verts.data(), r.vfacet.data());
-                queue.pop_front();
-                if (st != HG_OK) {
-                    r.state = HG_REQ_FAILED;
-                    r.error = hg_last_error();
-                    if (slot >= 0) (void)hg_pool_remove(pool, slot);
-                    stats.failed++;
-                    cv_done.notify_all();
-                    continue;
-                }
-                r.slot = slot;
-                r.state = HG_REQ_RUNNING;
-                r.t_admit = now();
-                running.push_back(r.id);
+        }
+        // 1. the interval of the resident requests, queued
+        uint32_t len = 0;
+        if (!running.empty()) {
+            uint64_t max_len = cfg.quantum;
+            for (request *r : running) {
+                size_t n = 0;
+                double t = 0;
+                uint64_t k = 0;
+                (void)hg_pool_info(pool, r->slot, &n, &t, &k);
+                max_len = std::min<uint64_t>(max_len, r->nsteps - k);
             }
+            hg::pool_reset_profile(pool);
+            if (hg::pool_begin(pool, cfg.dt, max_len, cfg.rebuild_every, &len) != HG_OK) {
+                stop_all(hg_last_error(), all_resident());
+                return;
+            }
+        }
+        // 2a. while it runs: the requests that finished last round
+        const bool any_retired = !retiring.empty();
+        for (request *r : retiring) {
+            hg_status st = hg::pool_read_state(pool, r->slot, r->x.data(), r->v.data(), r->th.data(), r->w.data(), cs);
+            const std::string err = st == HG_OK ? std::string() : hg_last_error();
+            if (hg::pool_release(pool, r->slot, cs) != HG_OK && st == HG_OK) {
+                stop_all(hg_last_error(), all_resident());
+                return;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            r->slot = -1;
+            r->t_done = now();
+            if (st == HG_OK) {
+                r->state = HG_REQ_DONE;
+                stats.done++;
+            } else {
+                r->state = HG_REQ_FAILED;
+                r->status = st;
+                r->error = err;
+                stats.failed++;
+            }
+        }
+        retiring.clear();
+        if (any_retired) cv_done.notify_all();
+        // 2b. admission in arrival order while the head fits
+        for (;;) {
+            request *r = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (queue.empty()) break;
+                r = &reqs[queue.front()];
+            }
+            int32_t slot = -1;
+            hg_status st = hg::pool_commit(pool, r->prep, &slot, cs);
+            std::string err = st == HG_OK ? std::string() : hg_last_error();
+            // no room now: wait for a retirement, unless nothing is resident
+            // (then the request can never be admitted and fails)
+            const bool resident = !running.empty() || !admitted.empty() || len;
+            if (st == HG_ERR_CAPACITY && resident) break;
+            if (st == HG_OK && r->nf) {
+                st = hg::pool_set_facets(pool, slot, r->nf, r->verts.data(), r->vfacet.data(), cs);
+                if (st != HG_OK) {
+                    err = hg_last_error();
+                    (void)hg::pool_release(pool, slot, cs);
+                }
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            queue.pop_front();
+            if (st != HG_OK) {
+                r->state = HG_REQ_FAILED;
+                r->status = st;
+                r->error = err;
+                r->t_done = now();
+                stats.failed++;
+                cv_done.notify_all();
+                continue;
+            }
+            r->prep = hg::pool_prep();  // the host copies are on the device now
+            r->slot = slot;
+            r->state = HG_REQ_RUNNING;
+            r->t_admit = now();
+            admitted.push_back(r);
+        }
+        // 3. the interval's end: faults, finished requests
+        if (len) {
+            std::vector<int32_t> faulted;
+            double busy = 0.0;
+            if (hg::pool_end(pool, faulted) != HG_OK || hg_pool_last_busy(pool, &busy) != HG_OK) {
+                stop_all(hg_last_error(), all_resident());
+                return;
+            }
+            std::vector<request *> still;
+            std::vector<int32_t> release;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                stats.rounds++;
+                stats.device_ms += busy;
+                for (request *r : running) {
+                    stats.particle_steps += uint64_t(r->n) * len;
+                    int fault = 0;
+                    (void)hg_pool_fault(pool, r->slot, &fault);
+                    if (fault) {
+                        r->state = HG_REQ_FAILED;
+                        r->status = (fault & HG_FAULT_NUMERIC) ? HG_ERR_NUMERIC : HG_ERR_CAPACITY;
+                        r->error = (fault & HG_FAULT_NUMERIC)
+                                       ? "the request's state became NaN or infinite"
+                                       : "the request overflowed its neighbour list or pair-state table";
+                        r->t_done = now();
+                        stats.failed++;
+                        release.push_back(r->slot);
+                        r->slot = -1;
+                        continue;
+                    }
+                    size_t n = 0;
+                    double t = 0;
+                    uint64_t k = 0;
+                    (void)hg_pool_info(pool, r->slot, &n, &t, &k);
+                    if (k >= r->nsteps) {
+                        hg::pool_park(pool, r->slot);
+                        retiring.push_back(r);
+                    } else {
+                        still.push_back(r);
+                    }
+                }
+            }
+            running.swap(still);
+            for (int32_t s : release)
+                if (hg::pool_release(pool, s, cs) != HG_OK) {
+                    stop_all(hg_last_error(), all_resident());
+                    return;
+                }
+            if (!release.empty()) cv_done.notify_all();
+        }
+        running.insert(running.end(), admitted.begin(), admitted.end());
+        admitted.clear();
+        {
+            std::lock_guard<std::mutex> lk(mu);
             stats.queued = queue.size();
-            stats.running = running.size();
-            if (running.empty()) continue;
-            // re-tune the pool's launch split when the resident particle count
-            // has moved by half since the last tuning (the split never changes
-            // results, only speed)
+            stats.running = running.size() + retiring.size();
+            // re-tune the pool's launch split when the resident particle
+            // count has moved by half since the last tuning (the split never
+            // changes results, only speed)
             size_t resident = 0;
-            for (uint64_t id : running) resident += reqs[id].n;
-            if (resident > tuned_particles + tuned_particles / 2 || 2 * resident < tuned_particles) {
+            for (request *r : running) resident += r->n;
+            if (resident && (resident > tuned_particles + tuned_particles / 2 || 2 * resident < tuned_particles)) {
                 (void)hg_pool_autotune(pool, 1);
                 tuned_particles = resident;
             }
         }
-        // one quantum for every resident request (the lock is not held, so
-        // submissions proceed meanwhile)
-        hg_status st = hg_pool_step(pool, cfg.dt, cfg.quantum, cfg.rebuild_every);
-        double chunk_ms = 0, rebuild_ms = 0;
-        uint64_t launches = 0;
-        if (st == HG_OK) st = hg_pool_last_step_profile(pool, &chunk_ms, &rebuild_ms, &launches, nullptr);
-        std::lock_guard<std::mutex> lk(mu);
-        if (st != HG_OK) {
-            fatal = hg_last_error();
-            for (uint64_t id : running) {
-                reqs[id].state = HG_REQ_FAILED;
-                reqs[id].error = fatal;
-            }
-            running.clear();
-            cv_done.notify_all();
-            return;
-        }
-        stats.rounds++;
-        stats.device_ms += chunk_ms + rebuild_ms;
-        std::vector<uint64_t> still;
-        for (uint64_t id : running) {
-            request &r = reqs[id];
-            stats.particle_steps += uint64_t(r.n) * cfg.quantum;
-            size_t n = 0;
-            double t = 0;
-            uint64_t k = 0;
-            hg_status s2 = hg_pool_info(pool, r.slot, &n, &t, &k);
-            if (s2 == HG_OK && k < r.nsteps) {
-                still.push_back(id);
-                continue;
-            }
-            if (s2 == HG_OK) s2 = hg_pool_get_state(pool, r.slot, r.x.data(), r.v.data(), r.th.data(), r.w.data());
-            (void)hg_pool_remove(pool, r.slot);
-            r.slot = -1;
-            r.t_done = now();
-            if (s2 == HG_OK) {
-                r.state = HG_REQ_DONE;
-                stats.done++;
-            } else {
-                r.state = HG_REQ_FAILED;
-                r.error = hg_last_error();
-                stats.failed++;
-            }
-        }
-        running.swap(still);
-        stats.running = running.size();
-        cv_done.notify_all();
     }
 }
+
+#define HG_TRY_S(expr)                \
+    do {                              \
+        hg_status st_ = (expr);       \
+        if (st_ != HG_OK) return st_; \
+    } while (0)
 
 extern "C" {
 
@@ -171,8 +279,6 @@ hg_status hg_server_create(const hg_params *params, const hg_server_config *cfg,
     *out = nullptr;
     if (cfg->quantum == 0 || !(cfg->dt > 0.0) || cfg->max_requests == 0)
         return fail(HG_ERR_INVALID, "hg_server_create: quantum, dt and max_requests must be positive");
-    if (params->pair_mode == HG_PAIRS_NEIGHBOR_LIST && cfg->rebuild_every && cfg->quantum % cfg->rebuild_every)
-        return fail(HG_ERR_INVALID, "hg_server_create: quantum must be a multiple of rebuild_every");
     const size_t bpp = hg_pool_bytes_per_particle(params, cfg->max_facets);
     if (bpp == 0) return fail(HG_ERR_INVALID, "hg_server_create: invalid params");
     const size_t max_particles = size_t(cfg->budget_bytes / bpp);
@@ -215,8 +321,6 @@ hg_status hg_server_submit(hg_server *S, size_t n, const double *x0, const doubl
     using namespace hg;
     if (!S || !x0 || !id) return fail(HG_ERR_INVALID, "hg_server_submit: server, x0 and id are required");
     if (n == 0 || nsteps == 0) return fail(HG_ERR_INVALID, "hg_server_submit: empty request");
-    if (nsteps % S->cfg.quantum)
-        return fail(HG_ERR_INVALID, "hg_server_submit: nsteps must be a multiple of the server quantum");
     if (nf > S->cfg.max_facets || (nf && (!verts || !v_facet)))
         return fail(HG_ERR_INVALID, "hg_server_submit: bad facets");
     if (n > S->max_particles)
@@ -225,14 +329,9 @@ hg_status hg_server_submit(hg_server *S, size_t n, const double *x0, const doubl
     request r;
     r.n = n;
     r.nsteps = nsteps;
-    auto take = [n](std::vector<double> &dst, const double *src) {
-        dst.assign(3 * n, 0.0);
-        if (src) std::memcpy(dst.data(), src, 3 * n * sizeof(double));
-    };
-    take(r.x, x0);
-    take(r.v, v0);
-    take(r.th, theta0);
-    take(r.w, omega0);
+    // the host work of the admission, here on the producer's thread
+    HG_TRY_S(pool_prepare(S->params, n, x0, v0, theta0, omega0, nullptr, 0, r.prep));
+    for (auto *f : {&r.x, &r.v, &r.th, &r.w}) f->assign(3 * n, 0.0);
     r.nf = nf;
     if (nf) {
         r.verts.assign(verts, verts + 9 * nf);
@@ -281,9 +380,10 @@ hg_status hg_server_wait(hg_server *S, uint64_t id, double timeout_s, double *x,
     }
     request &r = it->second;
     if (r.state != HG_REQ_DONE) {
-        std::string e = r.error.empty() ? S->fatal : r.error;
+        const std::string e = r.error.empty() ? S->fatal : r.error;
+        const hg_status st = r.status != HG_OK ? r.status : HG_ERR_HIP;
         S->reqs.erase(it);
-        return fail(HG_ERR_HIP, "hg_server_wait: request failed: " + e);
+        return fail(st, "hg_server_wait: request failed: " + e);
     }
     const size_t bytes = 3 * r.n * sizeof(double);
     if (x) std::memcpy(x, r.x.data(), bytes);

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "hg_internal.h"
+#include "snapshot.h"
 
 namespace hg {
 
@@ -242,8 +243,15 @@ extern "C" {
 
 const char *hg_last_error(void) { return g_err.c_str(); }
 
-hg_status hg_create(const hg_params *params, size_t n, const double *x0, const double *v0, const double *theta0,
-                    const double *omega0, hg_system **out) {
+}  // extern "C"
+
+namespace hg {
+namespace {
+// hg_create; `given_bonds` (with `given_skipped`) replaces the bond search on
+// x0 (a system reloaded from a snapshot keeps the bonds it was built with).
+hg_status create_impl(const hg_params *params, size_t n, const double *x0, const double *v0, const double *theta0,
+                      const double *omega0, const std::vector<std::pair<uint32_t, uint32_t>> *given_bonds,
+                      size_t given_skipped, hg_system **out) {
     if (!out) return fail(HG_ERR_INVALID, "out is null");
     *out = nullptr;
     HG_TRY(validate(params, n));
@@ -290,8 +298,13 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
     // Bond topology (alt_sinter ctor) and the initial pair-state table.
     std::vector<std::vector<uint32_t>> rows(n);
     if (S->p.binary_model == HG_BINARY_SINTER) {
-        sinter_topology_host(n, x0, S->p.sinter.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped,
-                             !S->p.sinter_breaking);
+        if (given_bonds) {
+            S->bonds = *given_bonds;
+            S->bonds_skipped = given_skipped;
+        } else {
+            sinter_topology_host(n, x0, S->p.sinter.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped,
+                                 !S->p.sinter_breaking);
+        }
         for (auto &b : S->bonds) {
             rows[b.first].push_back(b.second | kBondBit);
             rows[b.second].push_back(b.first | kBondBit);
@@ -326,8 +339,13 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
         // sinter_functor ctor bond search (sinter_bridge.h:146-161): every
         // pair with |gap| <= critical_separation, no cycle check; the
         // connectors come with hg_set_legacy_connectors
-        sinter_topology_host(n, x0, S->p.contact.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped, false,
-                             true);
+        if (given_bonds) {
+            S->bonds = *given_bonds;
+            S->bonds_skipped = given_skipped;
+        } else {
+            sinter_topology_host(n, x0, S->p.contact.r_part, S->p.sinter.d_crit, S->bonds, S->bonds_skipped, false,
+                                 true);
+        }
         std::vector<std::vector<uint32_t>> part(n);
         for (auto &b : S->bonds) {
             part[b.first].push_back(b.second);
@@ -406,6 +424,15 @@ hg_status hg_create(const hg_params *params, size_t n, const double *x0, const d
     return HG_OK;
 #undef HG_TRYC
 #undef HG_HIPC
+}
+}  // namespace
+}  // namespace hg
+
+extern "C" {
+
+hg_status hg_create(const hg_params *params, size_t n, const double *x0, const double *v0, const double *theta0,
+                    const double *omega0, hg_system **out) {
+    return create_impl(params, n, x0, v0, theta0, omega0, nullptr, 0, out);
 }
 
 hg_status hg_destroy(hg_system *S) {
@@ -519,7 +546,7 @@ hg_status step_impl(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_e
     const uint32_t n = uint32_t(S->n);
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     const int nf_advance = S->p.facet_advance ? 0 : S->nf;  // facets the step moves itself
-    if (n <= kSmallSystemMax && S->p.precision == HG_PRECISION_FP64 && !S->profiling) {
+    if (n <= kSmallSystemMax && S->p.precision == HG_PRECISION_FP64 && !S->profiling && !S->per_step) {
         // small systems: the steps between list rebuilds in one launch of one
         // workgroup instead of three dependent launches per step
         if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
@@ -768,6 +795,12 @@ hg_status hg_profile_enable(hg_system *S, int enable) {
     return HG_OK;
 }
 
+hg_status hg_set_per_step_launches(hg_system *S, int enable) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    S->per_step = enable != 0;
+    return HG_OK;
+}
+
 hg_status hg_profile_read(hg_system *S, uint64_t *launches, double *device_ms) {
     if (!S) return fail(HG_ERR_INVALID, "system is null");
     HG_HIP(hipStreamSynchronize(S->stream));
@@ -867,6 +900,185 @@ hg_status hg_energy(hg_system *S, double mass, double inertia, double *out) {
     HG_HIP(hipMemcpyAsync(h.data(), S->energy, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
     HG_HIP(hipStreamSynchronize(S->stream));
     sum_energy(h.data(), S->n, out);
+    return HG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Device rows of a system (n particles, stride n) copied to the host.
+template <typename T>
+hg_status download_all(hg_system *S, const T *dev, size_t count, std::vector<T> &host) {
+    host.resize(count);
+    if (count == 0) return HG_OK;
+    HG_HIP(hipMemcpyAsync(host.data(), dev, count * sizeof(T), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    return HG_OK;
+}
+template <typename T>
+hg_status upload_all(hg_system *S, T *dev, const std::vector<T> &host) {
+    if (host.empty()) return HG_OK;
+    HG_HIP(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    return HG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+hg_status hg_save(hg_system *S, void *buf, size_t cap, size_t *size) {
+    if (!S || !size) return fail(HG_ERR_INVALID, "hg_save: system and size are required");
+    HG_TRY(settle(S, "hg_save"));
+    const uint32_t n = uint32_t(S->n);
+    snapshot z;
+    z.p = S->p;
+    z.n = n;
+    z.t = S->t;
+    z.step_index = S->step_index;
+    z.acc_valid = S->acc_valid ? 1u : 0u;
+    for (auto [dev, host] : {std::pair<const double *, std::vector<double> *>{S->x, &z.x}, {S->v, &z.v},
+                             {S->th, &z.th}, {S->w, &z.w}, {S->a, &z.a}, {S->al, &z.al}}) {
+        host->resize(3 * size_t(n));
+        HG_TRY(download3(S, dev, host->data()));
+    }
+    if (S->nb) {
+        std::vector<uint32_t> nb;
+        z.has_nl = true;
+        HG_TRY(download_all(S, S->nb, size_t(S->cap_n) * n, nb));
+        HG_TRY(download_all(S, S->ncnt, n, z.nl_count));
+        rows_to_csr(nb.data(), z.nl_count.data(), n, 1, 1, z.nl_ids);
+    }
+    if (S->cap_s) {
+        // the current half of the pair-state table (with the legacy functor:
+        // the contact springs of its unbonded pairs)
+        std::vector<uint32_t> pj;
+        std::vector<double> ps;
+        z.has_table = true;
+        HG_TRY(download_all(S, S->pj[S->cur], size_t(S->cap_s) * n, pj));
+        HG_TRY(download_all(S, S->ps[S->cur], size_t(S->cap_s) * kSpringRec * n, ps));
+        HG_TRY(download_all(S, S->pcnt[S->cur], n, z.pt_count));
+        rows_to_csr(pj.data(), z.pt_count.data(), n, 1, 1, z.pt_tags);
+        rows_to_csr(ps.data(), z.pt_count.data(), n, kSpringRec, 9, z.pt_springs);
+    }
+    for (auto &b : S->bonds) {
+        z.bonds.push_back(b.first);
+        z.bonds.push_back(b.second);
+    }
+    z.skipped = S->bonds_skipped;
+    z.nf = uint32_t(S->nf);
+    if (S->nf > 0) {
+        HG_TRY(download_all(S, S->facets, size_t(S->nf), z.facets));
+        HG_TRY(download_all(S, S->ss, size_t(S->nf) * 9 * n, z.ss));
+    }
+    if (S->bstr) {
+        std::vector<uint32_t> bj;
+        std::vector<double> bs;
+        z.has_neck = true;
+        HG_TRY(download_all(S, S->bj, size_t(S->bcap) * n, bj));
+        HG_TRY(download_all(S, S->bstr, size_t(S->bcap) * n, bs));
+        HG_TRY(download_all(S, S->bcnt, n, z.neck_count));
+        rows_to_csr(bj.data(), z.neck_count.data(), n, 1, 1, z.neck_j);
+        rows_to_csr(bs.data(), z.neck_count.data(), n, 1, 1, z.neck_str);
+    }
+    if (S->lconn) {
+        std::vector<uint32_t> bj, sl;
+        std::vector<double> lc;
+        if (!S->legacy_ready) return fail(HG_ERR_INVALID, "hg_save: the legacy connectors are not set");
+        z.has_legacy = true;
+        HG_TRY(download_all(S, S->bj, size_t(S->bcap) * n, bj));
+        HG_TRY(download_all(S, S->lslot, size_t(S->bcap) * n, sl));
+        HG_TRY(download_all(S, S->lconn, size_t(S->bcap) * kConnRec * n, lc));
+        HG_TRY(download_all(S, S->bcnt, n, z.leg_count));
+        rows_to_csr(bj.data(), z.leg_count.data(), n, 1, 1, z.leg_j);
+        rows_to_csr(sl.data(), z.leg_count.data(), n, 1, 1, z.leg_slot);
+        rows_to_csr(lc.data(), z.leg_count.data(), n, kConnRec, 15, z.leg_conn);
+    }
+    const std::vector<uint8_t> bytes = encode_snapshot(z);
+    *size = bytes.size();
+    if (!buf) return HG_OK;
+    if (cap < bytes.size())
+        return fail(HG_ERR_CAPACITY, "hg_save: the snapshot needs " + std::to_string(bytes.size()) + " bytes");
+    std::memcpy(buf, bytes.data(), bytes.size());
+    return HG_OK;
+}
+
+hg_status hg_load(const void *buf, size_t size, hg_system **out) {
+    if (!out) return fail(HG_ERR_INVALID, "hg_load: out is required");
+    *out = nullptr;
+    snapshot z;
+    HG_TRY(decode_snapshot(buf, size, z));
+    const uint32_t n = uint32_t(z.n);
+    std::vector<std::pair<uint32_t, uint32_t>> bonds(z.bonds.size() / 2);
+    for (size_t b = 0; b < bonds.size(); b++) bonds[b] = {z.bonds[2 * b], z.bonds[2 * b + 1]};
+    hg_system *S = nullptr;
+    HG_TRY(create_impl(&z.p, n, z.x.data(), z.v.data(), z.th.data(), z.w.data(), &bonds, z.skipped, &S));
+    auto fill = [&]() -> hg_status {
+        HG_TRY(upload3(S, S->a, z.a.data()));
+        HG_TRY(upload3(S, S->al, z.al.data()));
+        if (z.has_nl != (S->nb != nullptr) || (z.has_table != (S->cap_s != 0)) || z.has_neck != (S->bstr != nullptr) ||
+            z.has_legacy != (S->lconn != nullptr))
+            return fail(HG_ERR_INVALID, "hg_load: the snapshot's tables do not match its parameters");
+        if (z.has_nl) {
+            if (max_count(z.nl_count) > S->cap_n)
+                return fail(HG_ERR_CAPACITY, "hg_load: a neighbour row exceeds the list capacity");
+            std::vector<uint32_t> nb;
+            csr_to_rows(z.nl_count, z.nl_ids, n, S->cap_n, 1, 1, 0u, nb);
+            HG_TRY(upload_all(S, S->nb, nb));
+            HG_TRY(upload_all(S, S->ncnt, z.nl_count));
+        }
+        if (z.has_table) {
+            if (max_count(z.pt_count) > S->cap_s)
+                return fail(HG_ERR_CAPACITY, "hg_load: a pair-state row exceeds the table capacity");
+            std::vector<uint32_t> pj;
+            std::vector<double> ps;
+            csr_to_rows(z.pt_count, z.pt_tags, n, S->cap_s, 1, 1, kNoEntry, pj);
+            csr_to_rows(z.pt_count, z.pt_springs, n, S->cap_s, kSpringRec, 9, 0.0, ps);
+            S->cur = 0;
+            HG_TRY(upload_all(S, S->pj[0], pj));
+            HG_TRY(upload_all(S, S->ps[0], ps));
+            HG_TRY(upload_all(S, S->pcnt[0], z.pt_count));
+        }
+        if (z.nf > 0) {
+            std::vector<double> verts(9 * size_t(z.nf)), vel(3 * size_t(z.nf));
+            HG_TRY(hg_set_facets(S, z.nf, verts.data(), vel.data()));  // allocates; the records follow
+            HG_TRY(upload_all(S, S->facets, z.facets));
+            HG_TRY(upload_all(S, S->ss, z.ss));
+        }
+        if (z.has_neck) {
+            std::vector<uint32_t> bj;
+            std::vector<double> bs;
+            if (max_count(z.neck_count) > S->bcap) return fail(HG_ERR_INVALID, "hg_load: bad neck rows");
+            csr_to_rows(z.neck_count, z.neck_j, n, S->bcap, 1, 1, kNoEntry, bj);
+            csr_to_rows(z.neck_count, z.neck_str, n, S->bcap, 1, 1, 0.0, bs);
+            HG_TRY(upload_all(S, S->bj, bj));
+            HG_TRY(upload_all(S, S->bstr, bs));
+            HG_TRY(upload_all(S, S->bcnt, z.neck_count));
+        }
+        if (z.has_legacy) {
+            std::vector<uint32_t> bj, sl;
+            std::vector<double> lc;
+            if (max_count(z.leg_count) > S->bcap) return fail(HG_ERR_INVALID, "hg_load: bad legacy rows");
+            csr_to_rows(z.leg_count, z.leg_j, n, S->bcap, 1, 1, kNoEntry, bj);
+            csr_to_rows(z.leg_count, z.leg_slot, n, S->bcap, 1, 1, 0u, sl);
+            csr_to_rows(z.leg_count, z.leg_conn, n, S->bcap, kConnRec, 15, 0.0, lc);
+            HG_TRY(upload_all(S, S->bj, bj));
+            HG_TRY(upload_all(S, S->lslot, sl));
+            HG_TRY(upload_all(S, S->lconn, lc));
+            HG_TRY(upload_all(S, S->bcnt, z.leg_count));
+            S->legacy_ready = true;
+        }
+        S->t = z.t;
+        S->step_index = z.step_index;
+        S->acc_valid = z.acc_valid != 0;
+        return HG_OK;
+    };
+    const hg_status st = fill();
+    if (st != HG_OK) {
+        const std::string msg = hg_last_error();
+        hg_destroy(S);
+        return fail(st, msg);
+    }
+    *out = S;
     return HG_OK;
 }
 

@@ -121,3 +121,15 @@ def test_dropin_legacy_sinter_needs_its_step_handler(tmp_path, handler, ok):
         assert r.returncode == 0, r.stderr
     else:
         assert r.returncode != 0 and "sinter_functor's own step handler" in r.stderr
+
+
+def test_snapshot_loader_refuses_foreign_and_damaged_bytes():
+    """hg_load checks the magic, checksum and sizes before it touches a
+    device, so these run on the host."""
+    lib = _abi.lib()
+    out = C.c_void_p()
+    for junk in (b"", b"HGSNAP01", b"not a snapshot at all, just some bytes" * 10, b"HGSNAP01" + bytes(64)):
+        st = lib.hg_load(junk, len(junk), C.byref(out))
+        assert st == _abi.HG_ERR_INVALID, junk[:16]
+        assert not out.value
+    assert "checksum" in (lib.hg_load(b"HGSNAP01" + bytes(64), 72, C.byref(out)) and lib.hg_last_error().decode())

@@ -150,3 +150,76 @@ def test_energy_mixed_pool_tracks_fp64():
     print(f"[gpu-energy] mixed vs FP64 ledger: max {err.max():.2e} per component {np.array2string(err, precision=2)}",
           flush=True)
     assert err.max() < 1e-6
+
+
+@pytest.mark.parametrize("n,seed", [(777, 11), (20000, 1)])
+def test_diagnostics_equal_the_reference_helpers(n, seed):
+    """hg_diagnostics (one device reduction) against the oracle's sequential
+    restatement of test/compute_energy.cpp:7-58 and
+    test/mass_distribution.cpp:7-27 on a stepped aggregate (n not a multiple
+    of the 1024-thread block). The device sums in another order, so each
+    quantity is held to 1e-12 of its own scale: the kinetic energies and Rg
+    (sums of positive terms) relative to themselves, |P|, |L| and the centre
+    of mass relative to the sums of the magnitudes they are made of."""
+    print(f"[gpu-energy] diagnostics n={n}", flush=True)
+    L = _L()
+    x = L.generate_aggregate(n, R, 1.8, 1.3, seed)
+    v0 = np.random.default_rng(seed).normal(0.0, 1.0, x.shape)
+    c, s, h = S.aggregate_params()
+    g = L.System(L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST,
+                               r_verlet=3.2 * R, contact=c, sinter=s, hamaker=h), x, v0)
+    o = O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R, contact=c,
+                                       sinter=s, hamaker=h), x, v0)
+    g.step(DT, 300, rebuild_every=50)
+    o.step(DT, 300, rebuild_every=50)
+    st = o.state()
+    np.testing.assert_array_equal(g.state()["v"], st["v"])
+    assert np.abs(st["omega"]).max() > 0.0
+    dg, do = g.diagnostics(O.MASS, O.INERTIA), o.diagnostics()
+    for k in ("ke", "ke_trans", "ke_rot", "rg"):
+        assert dg[k] == pytest.approx(do[k], rel=1e-12, abs=0.0), k
+    p_scale = O.MASS * np.abs(st["v"]).sum()
+    l_scale = O.INERTIA * np.abs(st["omega"]).sum() + O.MASS * (np.linalg.norm(st["x"], axis=1)
+                                                               * np.linalg.norm(st["v"], axis=1)).sum()
+    assert abs(dg["P"] - do["P"]) <= 1e-12 * p_scale
+    assert abs(dg["L"] - do["L"]) <= 1e-12 * l_scale
+    np.testing.assert_allclose(dg["cm"], do["cm"], rtol=0, atol=1e-12 * np.abs(st["x"]).max())
+    # the quantities are what their names say, from the state itself
+    assert do["ke_trans"] == pytest.approx(0.5 * O.MASS * (st["v"] ** 2).sum(), rel=1e-12)
+    assert do["rg"] == pytest.approx(np.sqrt(((st["x"] - st["x"].mean(0)) ** 2).sum(1).mean()), rel=1e-12)
+
+
+def test_thermo_series_as_the_reference_driver(tmp_path):
+    """System.run writes energies.dat as main.cpp:142-159 does (header, one
+    line every thermo period before the step, t = n dt, six significant
+    digits) and VTK dumps as writer.cpp:51-54 names them; every line equals
+    the oracle's diagnostics of the same state to the printed precision."""
+    print("[gpu-energy] thermo series", flush=True)
+    L = _L()
+    x = L.generate_aggregate(300, R, 1.8, 1.3, 12)
+    v0 = np.random.default_rng(12).normal(0.0, 1.0, x.shape)
+    c, s, h = S.aggregate_params()
+    g = L.System(L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST,
+                               r_verlet=3.2 * R, contact=c, sinter=s, hamaker=h), x, v0)
+    path = tmp_path / "energies.dat"
+    lines = g.run(DT, 1000, O.MASS, O.INERTIA, thermo_period=150, thermo_path=str(path), rebuild_every=50,
+                  dump_period=400, dump_dir=str(tmp_path), r_part=R)
+    assert lines == 7  # n = 0, 150, ..., 900
+    assert sorted(p.name for p in tmp_path.glob("particles_*.vtk")) == [f"particles_{k}.vtk" for k in range(3)]
+    text = path.read_text().splitlines()
+    assert text[0] == "t\tE_trs\tE_rot\tE_tot\tP\tL"
+    o = O.OracleSystem(O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R, contact=c,
+                                       sinter=s, hamaker=h), x, v0)
+    for k, line in enumerate(text[1:]):
+        n = 150 * k
+        if n:
+            o.step(DT, 150, rebuild_every=50)
+        d = o.diagnostics()
+        cols = line.split("\t")
+        assert cols[0] == f"{float(n) * DT:g}"
+        got = [float(q) for q in cols[1:]]
+        want = [d["ke_trans"], d["ke_rot"], d["ke"], d["P"], d["L"]]
+        for a, b in zip(got, want):
+            assert a == pytest.approx(b, rel=6e-6), f"line {k}: {line}"
+    o.step(DT, 100, rebuild_every=50)
+    np.testing.assert_array_equal(g.state()["x"], o.state()["x"])

@@ -1,9 +1,10 @@
 """Small single systems step through one launch per rebuild interval
 (k_tiny_steps: pair-parallel, up to 27 particles; k_small_steps: one particle
 per thread, up to 256) instead of the per-step kernels, which hg_step uses
-while launch profiling is on. Both must give the same state, bit for bit,
-for every functor mix: positions, velocities, orientations, spins,
-accelerations and the pair-state springs."""
+while launch profiling is on or when hg_set_per_step_launches asks for them.
+All three must give the same state, bit for bit, for every functor mix:
+positions, velocities, orientations, spins, accelerations and the pair-state
+springs."""
 import numpy as np
 import pytest
 
@@ -23,7 +24,7 @@ def _L():
 def run_both(params, x, v, steps, rebuild_every=0, legacy=False, facets=None, necks=None):
     L = _L()
     out = []
-    for profiled in (True, False):  # profiled: per-step launches
+    for path in ("profiled", "per_step", "one_launch"):
         s = L.System(params, x, v)
         if legacy:
             vecs = np.random.default_rng(5).uniform(-1.0, 1.0, (5000, 3))
@@ -33,16 +34,18 @@ def run_both(params, x, v, steps, rebuild_every=0, legacy=False, facets=None, ne
             s.set_facets(*facets)
         if necks is not None:
             s.set_neck_strengths(necks)
-        s.profile(profiled)
+        s.profile(path == "profiled")
+        s.per_step_launches(path == "per_step")
         s.step(1e-13, steps, rebuild_every=rebuild_every)
         st = s.state()
         st["springs0"] = s.springs(0)[1]
         st["springs1"] = s.springs(1)[1]
         out.append(st)
         s.close()
-    for k in out[0]:
-        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
-    return out[1]
+    for other in out[1:]:
+        for k in out[0]:
+            np.testing.assert_array_equal(other[k], out[0][k], err_msg=k)
+    return out[2]
 
 
 @pytest.mark.parametrize("ham", ["FULL", "LIMITED", "SIMPLIFIED"])

@@ -4,10 +4,10 @@ Each reference scenario driver (test/contact.cpp, test/hamaker.cpp,
 test/hamaker_neighbor.cpp, test/alt_sintered.cpp, test/surface.cpp) runs
 t_tot / dt steps and checks kinetic energy, linear momentum, radius of
 gyration or centre-of-mass drift against a printed target within 1 %. The
-same checks run here at full length on the device (FP64 pool path) and on the
-CPU oracle; both take minutes and are opt-in (HG_SLOW=1). By default the
-device runs the first tenth of every scenario and must equal the oracle bit
-for bit there. They pin the system loop and integrator, whose source
+same checks run here at full length on the device (FP64 pool path, part of
+the GPU suite) and on the CPU oracle (minutes of host time: opt-in with
+HG_SLOW=1). The device also runs the first tenth of every scenario beside the
+oracle and must equal it bit for bit there. They pin the system loop and integrator, whose source
 (libtimestep) is not part of the reference tree, against numbers the
 reference's authors produced with it. Measured on MI355X at full length: every
 target met within 1 % except alt_sintered's Rg (below).
@@ -138,10 +138,10 @@ def _nsteps(case):
 CASES = _cases()
 
 
-def _device_run(case, progress, n=None):
-    """Runs one scenario on its own pool (own stream) for n steps (default:
-    the reference's full length) and returns the final state; progress[0]
-    counts the steps done."""
+def _device_run(case, progress, n=None, single=False):
+    """Runs one scenario on its own pool (own stream), or as a single system,
+    for n steps (default: the reference's full length) and returns the final
+    state; progress[0] counts the steps done."""
     import libgran_amd as L
     x, v, facets = _init(case)
     n = _nsteps(case) if n is None else n
@@ -152,16 +152,29 @@ def _device_run(case, progress, n=None):
                       facet_hamaker=case.get("facet_hamaker", 0),
                       surface_contact=case.get("surface_contact", O.contact_params()),
                       surface_hamaker=case.get("surface_hamaker", O.hamaker_params()))
+    every = case.get("rebuild_every", 0)
+    if single:
+        # one launch per slice on the small-system paths
+        s = L.System(p, x, v)
+        if facets:
+            s.set_facets(*facets)
+        done = 0
+        while done < n:
+            k = min(200000, n - done)
+            s.step(case["dt"], k, every)
+            done += k
+            progress[0] = done
+        assert s.time()[1] == n
+        return x, s.state()
     pool = L.Pool(p, max_particles=max(64, x.shape[0]), max_systems=1, max_facets=1 if facets else 0)
     slot = pool.add(x, v)
     if facets:
         pool.set_facets(slot, *facets)
-    every = case.get("rebuild_every", 0)
     slice_len = 100000 - 100000 % every if every else 100000
     done = 0
     while done < n:
         k = min(slice_len, n - done)
-        pool.step(case["dt"], k, every if every and k % every == 0 else (k if every else 0))
+        pool.step(case["dt"], k, every)
         done += k
         progress[0] = done
     assert pool.info(slot)[2] == n
@@ -241,13 +254,13 @@ def _prefix_steps(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.slow
-def test_reference_kats_device():
-    """All five scenarios at full length against the reference's targets
-    (about four minutes: opt-in with HG_SLOW=1)."""
-    runs = _run_all(lambda nm: _nsteps(CASES[nm]))
-    for nm, (x0, st, _) in runs.items():
-        _assert_checks(nm, CASES[nm], _evaluate(CASES[nm], x0, st))
+@pytest.mark.parametrize("name", list(CASES))
+def test_reference_kats_device(name):
+    """One scenario at full length against the reference's targets, as a
+    single system (under a minute and a half of device time each)."""
+    print(f"[kat] {name}: {_nsteps(CASES[name])} steps", flush=True)
+    x0, st = _device_run(CASES[name], [0], single=True)
+    _assert_checks(name, CASES[name], _evaluate(CASES[name], x0, st))
 
 
 @pytest.mark.slow
