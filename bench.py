@@ -50,11 +50,13 @@ T_START = time.perf_counter()
 def log(msg):
     """Progress of the legs, on stderr (the JSON line alone goes to stdout)."""
     print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
-# The pool's autotuner scores each launch split over two rebuild intervals: 13
-# candidates on this batch. With fewer warm-up steps the split it settles on
-# (wide threshold 4096, room for CUs/16) is set directly.
-TUNER_STEPS = 26 * REBUILD
+# The pool's autotuner scores each candidate over two rebuild intervals: two
+# chunk shapes, then 13 launch splits on this batch. With fewer warm-up steps
+# the choice it settles on (512 threads x 1 workgroup per CU; wide threshold
+# 4096, room for CUs/16) is set directly.
+TUNER_STEPS = 30 * REBUILD
 PROFILED_SPLIT = (4096, 16)
+PROFILED_SHAPE = (512, 1)
 
 
 def aggregate_params(k=1.0e4, zeta=0.5, mu=1.0, A=1.0e-20, h0=1.0e-10, cutoff=3.0 * R_PART, d_crit=1e-3 * R_PART):
@@ -91,10 +93,12 @@ def state_bytes_per_particle_step(kbar, cbar):
     return 288 + 4 * kbar + 144 * cbar + (24 + 4 * kbar) / REBUILD
 
 
-def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None):
+def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None, chunk_shape=None):
     import torch
     pool = L.Pool(params(L, precision), max_particles=int(sizes.sum()) + 1024, max_systems=len(xs))
     slots = [pool.add(x, v) for x, v in zip(xs, vs)]
+    if chunk_shape:
+        pool.set_chunk_shape(*chunk_shape)
     if split is None:
         pool.autotune(True)  # tunes the launch split on the warm-up steps; results do not depend on it
     else:
@@ -499,6 +503,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
+    ap.add_argument("--chunk-shape", default=None, help="THREADS:WG_PER_CU chunk-kernel shape (default: tuned)")
     ap.add_argument("--only-single", action="store_true", help="only the N=20,000 FP64 per-step path (profiling)")
     ap.add_argument("--only-small", action="store_true",
                     help="only the reference's 19-particle lattice scenarios (test/hamaker.cpp, hamaker_neighbor.cpp) "
@@ -547,12 +552,14 @@ def main():
     log(f"batch: {total} particles in {t_gen:.1f} s; mixed pool, {args.warmup} warm-up + {args.steps} timed steps")
 
     split = tuple(int(q) for q in args.split.split(":")) if args.split else None
+    chunk_shape = tuple(int(q) for q in args.chunk_shape.split(":")) if args.chunk_shape else None
     if split is None and args.warmup < TUNER_STEPS:
         # too few warm-up steps for the autotuner: the split it picks on this
         # batch, at which the committed counter profiles were taken
         split = PROFILED_SPLIT
+        chunk_shape = chunk_shape or PROFILED_SHAPE
     pool, el_mx, prof_mx, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps,
-                                                                 args.warmup, split)
+                                                                 args.warmup, split, chunk_shape)
     pool.close()
     value = total * args.steps / el_mx
     log(f"mixed pool: {value:.4g} particle-steps/s")
@@ -569,6 +576,7 @@ def main():
                 "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,MIXED> (every aggregate below the wide threshold, "
                           "the steps of one rebuild interval or of the part of one the timed steps cover, per launch)",
                 "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
+                "chunk_shape": f"{tuning['chunk_threads']} threads x {tuning['chunk_wg_per_cu']} per CU",
                 "steps_per_launch": round(steps_per_launch, 2),
                 "particles_per_launch": narrow,
                 "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
@@ -584,7 +592,7 @@ def main():
     if not args.only_batched:
         log("FP64 pool")
         pool, el_64, _, _, _, tuning64, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup,
-                                                     split)
+                                                     split, chunk_shape)
         pool.close()
         log("plain FP64 path on a sample of the batch")
         plain, plain_note = plain_fp64_sample(L, sizes, xs, vs, 200)

@@ -371,12 +371,14 @@ hg_status hg_pool_set_wide_cus(hg_pool *pool, int wide_cus);
  * HIP graph per (interval length, dt, wide set) and replayed; enable = 0
  * launches them one by one instead. Default on. */
 hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
-/* Online autotuner of the launch split: from the next hg_pool_step on, each
- * candidate (wide threshold 0, or 1024..8192 up to the largest system, with
- * room for CUs/16, /8 or /4) runs two rebuild intervals of live steps, scored
- * by the second one's host-measured time per step; the fastest is kept. The
- * split never changes results, so no step is wasted. enable = 0 stops it and
- * keeps the current setting. */
+/* Online autotuner: from the next hg_pool_step on, first the chunk kernel's
+ * shape (256 threads x 2 workgroups per CU, or 512 x 1) at the current launch
+ * split, then the launch split at the better shape (wide threshold 0, or
+ * 1024..8192 up to the largest system, with room for CUs/16, /8 or /4). Each
+ * candidate runs two rebuild intervals of live steps, scored by the second
+ * one's host-measured time per step; the fastest is kept. Neither choice
+ * changes results, so no step is wasted. enable = 0 stops it and keeps the
+ * current setting. */
 hg_status hg_pool_autotune(hg_pool *pool, int enable);
 /* Current split and whether the autotuner has finished (tuned = 1). */
 hg_status hg_pool_get_tuning(hg_pool *pool, size_t *wide_min, int *wide_cus, int *tuned);
@@ -385,6 +387,13 @@ hg_status hg_pool_get_tuning(hg_pool *pool, size_t *wide_min, int *wide_cus, int
  * both streams joined (HIP events); the rest of the call's wall time is host
  * work between intervals. */
 hg_status hg_pool_last_busy(hg_pool *pool, double *busy_ms);
+/* Shape of the chunk kernel: threads per workgroup (256 or 512, default 256)
+ * and workgroups per CU (default 2; at most 2048 / threads). The grid is
+ * wg_per_cu per CU, and each workgroup gets that fraction of the CU's LDS,
+ * which decides which systems keep their state in LDS. Results do not depend
+ * on it; hg_pool_autotune tries the shapes. */
+hg_status hg_pool_set_chunk_shape(hg_pool *pool, int threads, int wg_per_cu);
+hg_status hg_pool_get_chunk_shape(hg_pool *pool, int *threads, int *wg_per_cu);
 /* Device time of the last hg_pool_step's wide launches (second stream, HIP
  * events; milliseconds) and their number. */
 hg_status hg_pool_last_wide_profile(hg_pool *pool, double *wide_ms, uint64_t *launches);

@@ -473,6 +473,11 @@ class Pool:
         """CUs given to the wide launches while both kinds of systems run."""
         _abi.check(_abi.lib().hg_pool_set_wide_cus(self._h, int(wide_cus)))
 
+    def set_chunk_shape(self, threads, wg_per_cu):
+        """Chunk-kernel threads per workgroup (256 or 512) and workgroups per CU
+        (each with that share of the LDS)."""
+        _abi.check(_abi.lib().hg_pool_set_chunk_shape(self._h, int(threads), int(wg_per_cu)))
+
     def set_wide_graph(self, enable):
         """Replay each interval's wide launches as a HIP graph (default) or not."""
         _abi.check(_abi.lib().hg_pool_set_wide_graph(self._h, 1 if enable else 0))
@@ -484,7 +489,10 @@ class Pool:
     def tuning(self):
         wm, wc, done = C.c_size_t(0), C.c_int(0), C.c_int(0)
         _abi.check(_abi.lib().hg_pool_get_tuning(self._h, C.byref(wm), C.byref(wc), C.byref(done)))
-        return dict(wide_min=wm.value, wide_cus=wc.value, tuned=bool(done.value))
+        tpb, wgs = C.c_int(0), C.c_int(0)
+        _abi.check(_abi.lib().hg_pool_get_chunk_shape(self._h, C.byref(tpb), C.byref(wgs)))
+        return dict(wide_min=wm.value, wide_cus=wc.value, tuned=bool(done.value), chunk_threads=tpb.value,
+                    chunk_wg_per_cu=wgs.value)
 
 
 class Server:

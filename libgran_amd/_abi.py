@@ -105,6 +105,8 @@ SIGNATURES = {
     "hg_pool_set_neck_strengths": (C.c_int, [_sysp, C.c_int32, _dp]),
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
     "hg_set_per_step_launches": (C.c_int, [_sysp, C.c_int]),
+    "hg_pool_set_chunk_shape": (C.c_int, [_sysp, C.c_int, C.c_int]),
+    "hg_pool_get_chunk_shape": (C.c_int, [_sysp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),
     "hg_generate_aggregate": (C.c_int, [C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_uint64, _dp]),
     "hg_generate_aggregates": (C.c_int, [C.c_size_t, _u32p, _dp, C.c_double, _u64p, C.c_double, _dp, C.c_int]),

@@ -84,8 +84,11 @@ HGD d3 shear_contact(d3 &xi, d3 n, double stiffness, double damping, double norm
 // ---- contact_force.h:57-96 --------------------------------------------------
 // `active` is false where the reference resets the pair's springs
 // (overlap <= 0, contact_force.h:68-71); the spring is then zero.
-HGD acc6 contact_eval(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp,
-                      bool &active) {
+// vw(vj, wj) gives the partner's velocity and angular velocity; it is called
+// only for an overlapping pair (the others need the positions alone).
+template <class VW>
+HGD acc6 contact_eval_vw(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 wi, VW &&vw, spring9 &sp,
+                         bool &active) {
     d3 d = xi - xj;
     d3 n = normalized(d);
     double overlap = 2.0 * p.r_part - dot(d, n);
@@ -95,6 +98,8 @@ HGD acc6 contact_eval(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3
         return {zero3(), zero3()};
     }
     active = true;
+    d3 vj, wj;
+    vw(vj, wj);
     double rpp = p.r_part - overlap / 2.0;
     double v_n = -dot(vi - vj, n);
     double f_n = p.k * overlap + p.gamma_n * v_n;
@@ -110,6 +115,10 @@ HGD acc6 contact_eval(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3
     d3 tau_r = p.r_part * cross(n, f_r);
     d3 tau_o = p.r_part * f_o;
     return {(f_n * n + f_t) / p.mass, ((-tau_t + tau_r) + tau_o) / p.inertia};
+}
+HGD acc6 contact_eval(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp,
+                      bool &active) {
+    return contact_eval_vw(p, xi, xj, vi, wi, [&](d3 &v, d3 &w) { v = vj; w = wj; }, sp, active);
 }
 
 // ---- alt_sinter_bridge.h:137-156 -------------------------------------------

@@ -35,15 +35,17 @@ HGD f3 zerof() { return {0.f, 0.f, 0.f}; }
 
 // Reciprocal square roots without the correctly rounded division/sqrt
 // expansions (this path is not bit-exact by design): the hardware estimates
-// refined by Newton steps. FP64: v_rsq_f64 then two steps
-// y <- y (3 - a y^2) / 2, each squaring the relative error, to ~1e-16.
-// FP32: v_rsq_f32 / v_rcp_f32 (1 ulp). Arguments must be > 0.
+// refined by Newton steps. FP64: v_rsq_f64 then one step
+// y <- y (3 - a y^2) / 2, which squares the estimate's relative error (below
+// 2^-22) to below 2^-44, 1e-13: a normal or a distance from it is then
+// accurate to far below what the FP32 force arithmetic resolves (the torsion
+// spring's 1e-5 rad residual, the finest thing derived from the normal, to
+// 1e-8 of itself). FP32: v_rsq_f32 / v_rcp_f32 (1 ulp). Arguments must be
+// > 0.
 __device__ __forceinline__ double rsqrt64(double a) {
 #pragma clang fp contract(fast)
-    double y = __builtin_amdgcn_rsq(a);
-    y = y * (1.5 - 0.5 * a * y * y);
-    y = y * (1.5 - 0.5 * a * y * y);
-    return y;
+    const double y = __builtin_amdgcn_rsq(a);
+    return y * (1.5 - 0.5 * a * y * y);
 }
 __device__ __forceinline__ float rsqrt32(float a) { return __builtin_amdgcn_rsqf(a); }
 __device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
@@ -66,6 +68,11 @@ struct mixed_consts {
     double sc_rd, sc_dt;
     float sh_a23, sh_im;
     double sh_r, sh_h0;
+    // squared distances: beyond far2 a pair that is not bonded is neither in
+    // contact nor within 0.25 r of it (hamaker_far_mx); beyond h_cut2 the
+    // limited-range Hamaker force is zero
+    double far2, h_cut2;
+    float h_r2f;
 };
 
 inline mixed_consts make_mixed_consts(const hg_params &p) {
@@ -97,6 +104,10 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
     const hg_hamaker_params &sh = p.surface_hamaker;
     m.sh_a23 = float(2.0 * sh.A / 3.0); m.sh_im = float(1.0 / sh.mass);
     m.sh_r = sh.r_part; m.sh_h0 = sh.h0;
+    const double touch = c.r_part > h.r_part ? 2.0 * c.r_part : 2.0 * h.r_part;
+    m.far2 = (1.125 * touch) * (1.125 * touch);
+    m.h_cut2 = h.r_cutoff * h.r_cutoff;
+    m.h_r2f = float(2.0 * h.r_part);
     return m;
 }
 
@@ -106,13 +117,15 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
 // step (~|v| dt / r, about 1e-5 rad here), which the rescale then restores to
 // full length: the result's direction is that residual's, and an FP32 normal
 // (error ~6e-8 rad) would perturb it by ~1%.
+// The rescale |xi| / |xr| is x2 / sqrt(x2 l2): one reciprocal square root
+// (x2 l2 stays a normal number for springs longer than 1e-77 m).
 __device__ __forceinline__ d3 rotate_spring(d3 xi, d3 n) {
 #pragma clang fp contract(fast)
     d3 xr = xi - dot(xi, n) * n;
     const double l2 = dot(xr, xr);
     if (l2 > 0.0) {
         const double x2 = dot(xi, xi);  // > 0 whenever l2 > 0
-        xr = xr * (x2 * rsqrt64(x2) * rsqrt64(l2));
+        xr = xr * (x2 * rsqrt64(x2 * l2));
     }
     return xr;
 }
@@ -175,9 +188,11 @@ __device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, d
 // These are multiplies and adds only; divisions and square roots (normals of
 // forces, Coulomb test) are FP32. Identities used:
 // n x w_i + n x w_j = n x (w_i + w_j), n x w_i - n x w_j = n x (w_i - w_j).
-template <bool BONDED>
-__device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, double dist, d3 n64, f3 n, d3 vi, d3 vj,
-                                                     d3 wi, d3 wj, spring9 &sp, bool &active) {
+// vw(vj, wj) gives the partner's velocity and angular velocity, for a bonded
+// or overlapping pair only.
+template <bool BONDED, class VW>
+__device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, double dist, d3 n64, f3 n, d3 vi, d3 wi,
+                                                     VW &&vw, spring9 &sp, bool &active) {
 #pragma clang fp contract(fast)
     const double ovd = (BONDED ? m.s_r2 : m.c_r2) - dist;
     if (!BONDED && ovd <= 0.0) {
@@ -186,6 +201,8 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
         return {zerof(), zerof()};
     }
     active = true;
+    d3 vj, wj;
+    vw(vj, wj);
     const float ov = float(ovd);
     const double rd = 0.5 * (BONDED ? m.s_r2 : m.c_r2);
     const double rppd = rd - 0.5 * ovd;
@@ -212,14 +229,12 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
     return {(f_n * n + f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
 }
 
-// Hamaker (full, limited range or simplified) with an FP64 gap.
+// The Hamaker acceleration along n at gap h (h >= h0): the reference's
+// direction is normalized(x_j - x_i) = -n.
 template <int HAM>
-__device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 n) {
+__device__ __forceinline__ f3 hamaker_at_mx(const mixed_consts &m, float h, f3 n) {
 #pragma clang fp contract(fast)
-    if (HAM == HG_HAMAKER_LIMITED && dist > m.h_cut) return zerof();
-    double hd = dist - m.h_r2;
-    if (hd < m.h_h0) hd = m.h_h0;
-    const float h = float(hd), rp = m.h_r;
+    const float rp = m.h_r;
     if constexpr (HAM == HG_HAMAKER_SIMPLIFIED) {
         // simplified_hamaker_force.h:26: 2 A r / h^2 along x_j - x_i = -n
         const float ih = rcp32(h);
@@ -229,8 +244,31 @@ __device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 
     const float ia = rcp32(a), ib = rcp32(b), ih = rcp32(h);
     const float s = m.h_a6 * ((a + h) * ia * ih - 2.0f * ib - 4.0f * rp * rp * ib * ib * ib -
                               2.0f * rp * rp * (a + h) * ia * ia * ih * ih);
-    // the reference's direction is normalized(x_j - x_i) = -n
     return (-s * m.h_im) * n;
+}
+
+// Hamaker (full, limited range or simplified) with an FP64 gap.
+template <int HAM>
+__device__ __forceinline__ f3 hamaker_mx(const mixed_consts &m, double dist, f3 n) {
+#pragma clang fp contract(fast)
+    if (HAM == HG_HAMAKER_LIMITED && dist > m.h_cut) return zerof();
+    double hd = dist - m.h_r2;
+    if (hd < m.h_h0) hd = m.h_h0;
+    return hamaker_at_mx<HAM>(m, float(hd), n);
+}
+
+// Hamaker of a pair beyond far2 (d2 = |d|^2 > far2, so the gap is at least
+// 0.25 r): the distance and normal in FP32 from the FP64 difference d. The
+// gap dist - 2r then carries at most eps_32 * 2.25 r / 0.25 r = 5e-7 of
+// itself, below what the FP32 force arithmetic resolves anyway.
+template <int HAM>
+__device__ __forceinline__ f3 hamaker_far_mx(const mixed_consts &m, d3 d, double d2) {
+#pragma clang fp contract(fast)
+    if (HAM == HG_HAMAKER_LIMITED && d2 > m.h_cut2) return zerof();
+    const float q = float(d2);
+    const float inv = rsqrt32(q);
+    const f3 n = tof(d) * inv;
+    return hamaker_at_mx<HAM>(m, q * inv - m.h_r2f, n);
 }
 
 __device__ __forceinline__ accf surface_contact_mx(const mixed_consts &m, d3 xf, d3 vf_, d3 xi, d3 vi_, d3 w64,

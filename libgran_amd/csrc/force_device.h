@@ -239,45 +239,48 @@ struct lds_id_list {
 // they fetch the last partner again, from cache): a conditional load leaves
 // two values for the compiler to merge, and the merge waits for the load and
 // every memory operation issued before it.
-// IDS is id_list (global memory) or lds_id_list. PREFETCH = false loads each
-// partner's state when its pair comes (fewer registers held across a pair).
-template <bool PREFETCH = true, class IDS, class LOAD, class FOLD>
-__device__ __forceinline__ void nl_partners(const IDS &ids, LOAD &&load, FOLD &&fold) {
+// IDS is id_list (global memory) or lds_id_list. Only positions are
+// gathered here (loadx(j) returns x_j): the fold loads a partner's velocity
+// and angular velocity itself, for the bonded and overlapping pairs, so the
+// partners that only attract (most of a 3.2r list) cost one 24-byte gather.
+// PREFETCH = false loads each position when its pair comes.
+template <bool PREFETCH = true, class IDS, class LOADX, class FOLD>
+__device__ __forceinline__ void nl_partners(const IDS &ids, LOADX &&loadx, FOLD &&fold) {
     if (ids.n == 0) return;
     if constexpr (!PREFETCH) {
         uint32_t jn = ids.at(0);
         for (uint32_t k = 0; k < ids.n; k++) {
             const uint32_t j = jn;
-            d3 xj, vj, wj;
-            load(j, xj, vj, wj);
+            const d3 xj = loadx(j);
             jn = ids.at_clamped(k + 1);
-            fold(j, xj, vj, wj);
+            fold(j, xj);
         }
         return;
     }
     uint32_t jn = ids.at(0), jn2 = ids.at_clamped(1);
-    d3 xn, vn, wn;
-    load(jn, xn, vn, wn);
+    d3 xn = loadx(jn);
     for (uint32_t k = 0; k < ids.n; k++) {
         const uint32_t j = jn;
-        const d3 xj = xn, vj = vn, wj = wn;
+        const d3 xj = xn;
         jn = jn2;
-        load(jn, xn, vn, wn);
+        xn = loadx(jn);
         jn2 = ids.at_clamped(k + 2);
-        fold(j, xj, vj, wj);
+        fold(j, xj);
     }
 }
 
 // The binary functor of one ordered pair: alt_sinter's bonded model for a
 // bonded pair, the contact functor otherwise; keep says whether the pair's
 // springs stay in the table; neck() gives the bond's neck strength (read only
-// when bonds break).
-template <int BIN, class NECK>
-__device__ __forceinline__ acc6 pair_binary(const hg_params &P, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj,
-                                            spring9 &sp, bool bonded, bool &keep, NECK &&neck) {
+// when bonds break); vw(vj, wj) the partner's velocities (called for bonded
+// and overlapping pairs only).
+template <int BIN, class VW, class NECK>
+__device__ __forceinline__ acc6 pair_binary(const hg_params &P, d3 xi, d3 xj, d3 vi, d3 wi, VW &&vw, spring9 &sp,
+                                            bool bonded, bool &keep, NECK &&neck) {
     acc6 b;
     if (BIN == HG_BINARY_SINTER && bonded) {
-        d3 ftot;
+        d3 ftot, vj, wj;
+        vw(vj, wj);
         b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot);
         // alt_breaking_sinter_bridge.h:117-119: (i, j) stops being bonded
         // once its force exceeds the neck strength; dropping the entry
@@ -285,7 +288,7 @@ __device__ __forceinline__ acc6 pair_binary(const hg_params &P, d3 xi, d3 xj, d3
         // reference's untouched contact table holds.
         keep = !(P.sinter_breaking && norm(ftot) > neck());
     } else {
-        b = contact_eval(P.contact, xi, xj, vi, vj, wi, wj, sp, keep);
+        b = contact_eval_vw(P.contact, xi, xj, vi, wi, vw, sp, keep);
     }
     return b;
 }
@@ -316,12 +319,15 @@ __device__ __forceinline__ void fold_pair(acc6 &acc, const acc6 &b, const acc6 &
 
 // One ordered pair through the binary container fold
 // (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
-template <int BIN, int HAM, bool EAGER>
-__device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
-                                          walker_t<EAGER> &W, acc6 &acc) {
+// vw(vj, wj) loads the partner's velocities when the pair needs them.
+template <int BIN, int HAM, bool EAGER, class VW>
+__device__ __forceinline__ void pair_fold_vw(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, VW &&vw,
+                                             walker_t<EAGER> &W, acc6 &acc) {
     acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
     uint32_t lslot_ij = 0;
     if (BIN == HG_BINARY_SINTER_LEGACY && W.legacy_bonded(j, lslot_ij)) {
+        d3 vj, wj;
+        vw(vj, wj);
         // sinter_bridge.h:222-223: the bond's connectors, i's at its slot for
         // j, j's at its slot for i
         const double *ri = W.lconn + (size_t(lslot_ij) * W.stride + W.i) * kConnRec;
@@ -340,11 +346,16 @@ __device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi,
         bool bonded = false;
         W.take(j, sp, bonded);
         bool keep;
-        b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, [&] { return W.neck(j); });
+        b = pair_binary<BIN>(P, xi, xj, vi, wi, vw, sp, bonded, keep, [&] { return W.neck(j); });
         if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
     }
     h = pair_hamaker<HAM>(P, xi, xj);
     fold_pair<BIN, HAM>(acc, b, h);
+}
+template <int BIN, int HAM, bool EAGER>
+__device__ __forceinline__ void pair_fold(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj, d3 wj,
+                                          walker_t<EAGER> &W, acc6 &acc) {
+    pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, [&](d3 &v, d3 &w) { v = vj, w = wj; }, W, acc);
 }
 
 // Unary container over facets (unary_force_container.h:20-32, a right fold),
@@ -371,14 +382,16 @@ __device__ __forceinline__ acc6 facet_fold(const hg_params &P, const facet_dev *
 }
 
 // Mixed-precision fold of one ordered pair (dem_mixed.h); the spring walk is
-// the FP64 one.
-template <int BIN, int HAM, bool EAGER>
-__device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, d3 vj,
-                                             d3 wj, walker_t<EAGER> &W, accf &acc) {
-    double dist;
-    d3 n64;
-    f3 n;
-    pair_kinematics_mx(xi, xj, dist, n64, n);
+// the FP64 one. vw as in pair_fold_vw.
+// A pair that is not bonded and lies beyond far2 is out of contact (its
+// springs, if any, are dropped as the contact functor drops them) and takes
+// the FP32-distance Hamaker form; the others take the FP64 kinematics.
+template <int BIN, int HAM, bool EAGER, class VW>
+__device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, VW &&vw,
+                                             walker_t<EAGER> &W, accf &acc) {
+    const d3 d = xi - xj;
+    const double d2 = dot(d, d);
+    bool near = !(d2 > m.far2);
     accf b{zerof(), zerof()};
     f3 h = zerof();
     if constexpr (BIN != HG_BINARY_NONE) {
@@ -386,15 +399,32 @@ __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, 
         spring9 sp = zero_spring();
         bool bonded = false;
         W.take(j, sp, bonded);
-        bool keep;
-        if (BIN == HG_BINARY_SINTER && bonded) {
-            b = bonded_or_contact_mx<true>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
-            if (m.s_breaking && sqrtf(dotf(b.lin, b.lin)) * m.s_m > float(W.neck(j))) keep = false;
-        } else
-            b = bonded_or_contact_mx<false>(m, dist, n64, n, vi, vj, wi, wj, sp, keep);
-        if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
+        near = near || (BIN == HG_BINARY_SINTER && bonded);
+        if (near) {
+            double dist;
+            d3 n64;
+            f3 n;
+            pair_kinematics_mx(xi, xj, dist, n64, n);
+            bool keep;
+            if (BIN == HG_BINARY_SINTER && bonded) {
+                b = bonded_or_contact_mx<true>(m, dist, n64, n, vi, wi, vw, sp, keep);
+                if (m.s_breaking && sqrtf(dotf(b.lin, b.lin)) * m.s_m > float(W.neck(j))) keep = false;
+            } else
+                b = bonded_or_contact_mx<false>(m, dist, n64, n, vi, wi, vw, sp, keep);
+            if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
+            if constexpr (HAM != HG_HAMAKER_NONE) h = hamaker_mx<HAM>(m, dist, n);
+        }
+    } else if (near) {
+        if constexpr (HAM != HG_HAMAKER_NONE) {
+            double dist;
+            d3 n64;
+            f3 n;
+            pair_kinematics_mx(xi, xj, dist, n64, n);
+            h = hamaker_mx<HAM>(m, dist, n);
+        }
     }
-    if constexpr (HAM != HG_HAMAKER_NONE) h = hamaker_mx<HAM>(m, dist, n);
+    if constexpr (HAM != HG_HAMAKER_NONE)
+        if (!near) h = hamaker_far_mx<HAM>(m, d, d2);
     acc.lin = acc.lin + (b.lin + h);
     acc.ang = acc.ang + b.ang;
 }

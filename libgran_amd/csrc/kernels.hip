@@ -35,9 +35,10 @@ __device__ __forceinline__ void force_nl_one(const hg_params &P, const sys_view 
     if constexpr (BIN == HG_BINARY_SINTER_LEGACY) W.init_legacy(S.pt);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    nl_partners(
-        ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { x = ld3(S.x, j), v = ld3(S.v, j), w = ld3(S.w, j); },
-        [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc); });
+    nl_partners(ids, [&](uint32_t j) { return ld3(S.x, j); }, [&](uint32_t j, d3 xj) {
+        pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, [&](d3 &v, d3 &w) { v = ld3(S.v, j), w = ld3(S.w, j); }, W,
+                               acc);
+    });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
@@ -62,9 +63,10 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    nl_partners(
-        ids, [&](uint32_t j, d3 &x, d3 &v, d3 &w) { x = ld3(S.x, j), v = ld3(S.v, j), w = ld3(S.w, j); },
-        [&](uint32_t j, d3 xj, d3 vj, d3 wj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc); });
+    nl_partners(ids, [&](uint32_t j) { return ld3(S.x, j); }, [&](uint32_t j, d3 xj) {
+        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, [&](d3 &v, d3 &w) { v = ld3(S.v, j), w = ld3(S.w, j); }, W,
+                               acc);
+    });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold_mx(P, MC, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, tod(acc.lin));
@@ -682,7 +684,8 @@ __global__ void __launch_bounds__(kTinyThreads)
                                 if (S.pt.bj[size_t(q) * n + i] == j) r = S.pt.bstr[size_t(q) * n + i];
                         return r;
                     };
-                    b = pair_binary<BIN>(P, xi, xj, vi, vj, wi, wj, sp, bonded, keep, neck);
+                    b = pair_binary<BIN>(P, xi, xj, vi, wi, [&](d3 &v, d3 &w) { v = vj, w = wj; }, sp, bonded,
+                                         keep, neck);
                 }
                 const acc6 e = pair_term<BIN, HAM>(b, pair_hamaker<HAM>(P, xi, xj));
                 double *tm = terms + size_t(t) * 6;

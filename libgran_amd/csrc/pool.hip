@@ -78,21 +78,32 @@ namespace {
 //   STATE_MIRROR x, v, omega copied into LDS (9 doubles per particle) and
 //                refreshed by each owner lane in the kick/drift phase, so
 //                partner gathers read LDS; the state itself stays global;
+//   STATE_POS    x alone copied into LDS (3 doubles per particle) and
+//                refreshed likewise: every partner needs its position, only
+//                the bonded and overlapping ones (a few of a 3.2r list) its
+//                velocities, which then come from global memory;
 //   STATE_GLOBAL everything in global memory.
-enum { STATE_GLOBAL = 0, STATE_MIRROR = 1, STATE_FULL = 2 };
+enum { STATE_GLOBAL = 0, STATE_MIRROR = 1, STATE_FULL = 2, STATE_POS = 3 };
 #ifndef HG_CHUNK_EAGER
 #define HG_CHUNK_EAGER 1
 #endif
 #ifndef HG_CHUNK_PREFETCH
 #define HG_CHUNK_PREFETCH 1
 #endif
+#ifndef HG_CHUNK_MIN_BLOCKS
+#define HG_CHUNK_MIN_BLOCKS 1
+#endif
 constexpr bool kChunkEager = HG_CHUNK_EAGER, kChunkPrefetch = HG_CHUNK_PREFETCH;
 extern __shared__ double pool_lds[];
-constexpr uint32_t kMirrorStride = 9, kFullStride = 18;
+constexpr uint32_t kPosStride = 3, kMirrorStride = 9, kFullStride = 18;
+template <int MODE>
+constexpr uint32_t lds_stride() {
+    return MODE == STATE_FULL ? kFullStride : (MODE == STATE_MIRROR ? kMirrorStride : kPosStride);
+}
 
 template <int MODE>
 __device__ __forceinline__ double *lds_rec(uint32_t offset, uint32_t g) {
-    return pool_lds + size_t(g - offset) * (MODE == STATE_FULL ? kFullStride : kMirrorStride);
+    return pool_lds + size_t(g - offset) * lds_stride<MODE>();
 }
 __device__ __forceinline__ d3 lds3(const double *q) { return d3{q[0], q[1], q[2]}; }
 __device__ __forceinline__ void sds3(double *q, d3 a) {
@@ -102,9 +113,30 @@ __device__ __forceinline__ void sds3(double *q, d3 a) {
 }
 
 template <int MODE>
+__device__ __forceinline__ d3 load_x(const pool_view &V, uint32_t offset, uint32_t g) {
+    if constexpr (MODE == STATE_GLOBAL) return ld3(V.x, g);
+    else return lds3(lds_rec<MODE>(offset, g));
+}
+template <int MODE>
+__device__ __forceinline__ void load_vw(const pool_view &V, uint32_t offset, uint32_t g, d3 &v, d3 &w) {
+    if constexpr (MODE == STATE_GLOBAL || MODE == STATE_POS) {
+        v = ld3(V.v, g);
+        w = ld3(V.w, g);
+    } else {
+        const double *q = lds_rec<MODE>(offset, g);
+        v = lds3(q + 3);
+        w = lds3(q + 6);
+    }
+}
+
+template <int MODE>
 __device__ __forceinline__ void load_state(const pool_view &V, uint32_t offset, uint32_t g, d3 &x, d3 &v, d3 &w) {
     if constexpr (MODE == STATE_GLOBAL) {
         x = ld3(V.x, g);
+        v = ld3(V.v, g);
+        w = ld3(V.w, g);
+    } else if constexpr (MODE == STATE_POS) {
+        x = lds3(lds_rec<MODE>(offset, g));
         v = ld3(V.v, g);
         w = ld3(V.w, g);
     } else {
@@ -133,6 +165,7 @@ __device__ __forceinline__ void lds_fill(const pool_view &V, const pool_meta &M)
     for (uint32_t g = M.offset + threadIdx.x; g < M.offset + M.n; g += blockDim.x) {
         double *q = lds_rec<MODE>(M.offset, g);
         sds3(q, ld3(V.x, g));
+        if constexpr (MODE == STATE_POS) continue;
         sds3(q + 3, ld3(V.v, g));
         sds3(q + 6, ld3(V.w, g));
         if constexpr (MODE == STATE_FULL) {
@@ -198,28 +231,26 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
         id_list ids;
         if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST && !LIDS) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
-        // listed partners of g through fold, in list order
+        // listed partners of g through fold(j, xj), in list order; vw_of(j)
+        // loads a partner's velocities when its pair needs them
         auto partners = [&](auto &&fold) {
-            auto ld = [&](uint32_t j, d3 &x, d3 &v, d3 &w) { load_state<LDS>(V, M.offset, j, x, v, w); };
+            auto ldx = [&](uint32_t j) { return load_x<LDS>(V, M.offset, j); };
             if constexpr (LIDS) {
                 const uint32_t t = g - M.offset;
-                nl_partners<kChunkPrefetch>(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ld, fold);
+                nl_partners<kChunkPrefetch>(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ldx, fold);
             } else {
-                nl_partners<kChunkPrefetch>(ids, ld, fold);
+                nl_partners<kChunkPrefetch>(ids, ldx, fold);
             }
         };
+        auto vw_of = [&](uint32_t j) { return [&V, &M, j](d3 &v, d3 &w) { load_vw<LDS>(V, M.offset, j, v, w); }; };
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj, d3 vj, d3 wj) {
-                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
-                });
+                partners([&](uint32_t j, d3 xj) { pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, vw_of(j), W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
-                    d3 xj, vj, wj;
-                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
-                    pair_fold<BIN, HAM>(P, j, xi, vi, wi, xj, vj, wj, W, acc);
+                    pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, load_x<LDS>(V, M.offset, j), vw_of(j), W, acc);
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
@@ -228,15 +259,11 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         } else {
             accf acc{zerof(), zerof()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj, d3 vj, d3 wj) {
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
-                });
+                partners([&](uint32_t j, d3 xj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vw_of(j), W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
-                    d3 xj, vj, wj;
-                    load_state<LDS>(V, M.offset, j, xj, vj, wj);
-                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vj, wj, W, acc);
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, load_x<LDS>(V, M.offset, j), vw_of(j), W, acc);
                 }
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
@@ -274,8 +301,8 @@ __device__ __forceinline__ void pool_kick_drift(const pool_view &V, const pool_m
             st3(V.v, g, vi);
             st3(V.w, g, wi);
         }
-        if constexpr (LDS != STATE_GLOBAL) {
-            sds3(q, xi);
+        if constexpr (LDS != STATE_GLOBAL) sds3(q, xi);
+        if constexpr (LDS == STATE_MIRROR || LDS == STATE_FULL) {
             sds3(q + 3, vi);
             sds3(q + 6, wi);
         }
@@ -328,7 +355,7 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
     uint16_t *lids = nullptr;
     if constexpr (LIDS) {
         lids = reinterpret_cast<uint16_t *>(
-            pool_lds + (LDS == STATE_GLOBAL ? 0 : size_t(M.n) * (LDS == STATE_FULL ? kFullStride : kMirrorStride)));
+            pool_lds + (LDS == STATE_GLOBAL ? 0 : size_t(M.n) * lds_stride<LDS>()));
         lds_ids_fill(V, M, kmax, lids);
     }
     if constexpr (LDS != STATE_GLOBAL || LIDS) __syncthreads();
@@ -362,16 +389,18 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
 // once the shared counter passes `count`. lds_bytes of dynamic LDS per
 // workgroup decide each system's state placement. FAC = false leaves the
 // facet fold out (every queued system has nf = 0): it holds the kernel's
-// largest register set, and without it more wavefronts fit each SIMD.
-template <int BIN, int HAM, int PAIRS, int PREC, bool FAC>
+// largest register set. TPB threads per workgroup (256 or 512): a 512-thread
+// workgroup alone on a CU has the same waves per SIMD as two of 256 and
+// twice the LDS for one system.
+template <int BIN, int HAM, int PAIRS, int PREC, bool FAC, int TPB>
 // The model constants come by value: the scalar registers they spill to are
 // re-read with v_readlane, which measured faster than reloading them through
 // a pointer inside the pair loop (scalar loads add memory latency there).
-__global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
+__global__ void __launch_bounds__(TPB, HG_CHUNK_MIN_BLOCKS) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
                                                     uint32_t count, uint32_t *counter, uint32_t nsteps, double dt,
                                                     uint32_t lds_bytes) {
     __shared__ uint32_t s_item;
-    __shared__ uint32_t s_red[4];
+    __shared__ uint32_t s_red[TPB / 64];
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -392,6 +421,8 @@ __global__ void __launch_bounds__(256) k_pool_chunk(hg_params P, mixed_consts MC
                 run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else if (need * kMirrorStride + ids <= lds_bytes)
                 run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
+            else if (need * kPosStride + ids <= lds_bytes)
+                run_system<BIN, HAM, PAIRS, PREC, STATE_POS, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else if (ids <= lds_bytes)
                 run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
             else
@@ -622,46 +653,51 @@ __global__ void k_zero_rows_f64(double *p, size_t stride, size_t offset, size_t 
 
 inline unsigned nblk(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
-template <int BIN, int HAM, int PAIRS, int PREC, bool FAC>
+template <int BIN, int HAM, int PAIRS, int PREC, bool FAC, int TPB>
 hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
                           hipStream_t s) {
     const size_t lds_bytes = lds_cap;
     if (lds_bytes > 0) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+        hipError_t e =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC, TPB>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
         if (e != hipSuccess) return e;
     }
-    k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC><<<grid, 256, lds_bytes, s>>>(p, make_mixed_consts(p), V, order, count,
-                                                                           counter, nsteps, dt, lds_cap);
+    k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC, TPB><<<grid, TPB, lds_bytes, s>>>(p, make_mixed_consts(p), V, order,
+                                                                                count, counter, nsteps, dt, lds_cap);
     return hipGetLastError();
 }
 
 template <int BIN, int HAM, int PAIRS>
 hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
-                          bool facets, hipStream_t s) {
-#define HG_KCASE(PREC, FAC)                                                                                  \
-    if (p.precision == PREC && facets == FAC)                                                               \
-        return launch_chunk_k<BIN, HAM, PAIRS, PREC, FAC>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, \
-                                                           s);
-    HG_KCASE(HG_PRECISION_MIXED, true)
-    HG_KCASE(HG_PRECISION_MIXED, false)
-    HG_KCASE(HG_PRECISION_FP64, true)
-    HG_KCASE(HG_PRECISION_FP64, false)
+                          bool facets, int tpb, hipStream_t s) {
+#define HG_KCASE(PREC, FAC, TPB)                                                                                \
+    if (p.precision == PREC && facets == FAC && tpb == TPB)                                                    \
+        return launch_chunk_k<BIN, HAM, PAIRS, PREC, FAC, TPB>(p, V, order, count, counter, nsteps, dt, grid, \
+                                                                lds_cap, s);
+    HG_KCASE(HG_PRECISION_MIXED, true, 256)
+    HG_KCASE(HG_PRECISION_MIXED, false, 256)
+    HG_KCASE(HG_PRECISION_FP64, true, 256)
+    HG_KCASE(HG_PRECISION_FP64, false, 256)
+    HG_KCASE(HG_PRECISION_MIXED, true, 512)
+    HG_KCASE(HG_PRECISION_MIXED, false, 512)
+    HG_KCASE(HG_PRECISION_FP64, true, 512)
+    HG_KCASE(HG_PRECISION_FP64, false, 512)
 #undef HG_KCASE
     return hipErrorInvalidValue;
 }
 
 // facets: whether any queued system has facets (else the kernel without the
-// facet fold runs).
+// facet fold runs); tpb: threads per workgroup (256 or 512).
 template <int PAIRS>
 hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
-                          bool facets, hipStream_t s) {
+                          bool facets, int tpb, hipStream_t s) {
 #define HG_PCASE(B, H) \
     if (p.binary_model == B && p.hamaker_model == H) \
-        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, facets, s);
+        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, facets, tpb, s);
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
@@ -757,7 +793,12 @@ struct hg_pool {
     int n_cus = 0, wide_cus = 0;
     bool wide_graph = true;  // replay the wide launches of an interval as a HIP graph
     int tune_phase = 0;      // TUNE_*
-    std::vector<std::pair<uint32_t, int>> tune_cands;  // (wide_min, wide_cus)
+    struct tune_cand {
+        uint32_t wide_min;
+        int wide_cus, tpb, wgs;  // launch split, chunk shape
+    };
+    std::vector<tune_cand> tune_cands;
+    int tune_stage = 0;  // 0: chunk shapes, 1: launch splits
     std::vector<double> tune_scores;                   // seconds per step
     size_t tune_pos = 0;
     hipGraphExec_t wg_exec = nullptr;
@@ -797,6 +838,9 @@ struct hg_pool {
     int grid = 0;
     uint32_t lds_cap = 0;
     size_t lds_bytes_per_wg = 0;
+    size_t lds_per_cu = 0, lds_per_block = 0;  // device limits
+    int wg_per_cu = 2;                         // chunk-kernel workgroups per CU
+    int chunk_tpb = 256;                       // and their threads
 };
 
 namespace {
@@ -856,6 +900,17 @@ hg_status collect_faults(hg_pool *Q, std::vector<int32_t> &faulted) {
     HG_HIP(launch_zero_u32(reinterpret_cast<uint32_t *>(Q->sflags), Q->S, Q->stream));
     HG_HIP(hipStreamSynchronize(Q->stream));
     return HG_OK;
+}
+
+// k chunk-kernel workgroups per CU (the grid), each with 1/k of the CU's
+// LDS (less the static words) for its system's state and partner ids.
+void set_chunk_wgs(hg_pool *Q, int k) {
+    Q->wg_per_cu = k;
+    Q->grid = Q->n_cus * k;
+    size_t per_wg = std::min<size_t>(Q->lds_per_block, Q->lds_per_cu / size_t(k));
+    per_wg = per_wg > 1024 ? per_wg - 1024 : 0;
+    Q->lds_cap = uint32_t(per_wg);
+    Q->lds_bytes_per_wg = per_wg;
 }
 
 pool_view make_pool_view(hg_pool *Q) {
@@ -1041,20 +1096,14 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     Q->P = max_particles;
     Q->S = max_systems;
     Q->MF = max_facets;
-    // Two workgroups per CU (the chunk kernel's register use allows two waves
-    // per SIMD); each gets half the CU's LDS for its mirror.
-    Q->grid = prop.multiProcessorCount * 2;
-    {
-        size_t per_wg = std::min<size_t>(prop.sharedMemPerBlock, prop.maxSharedMemoryPerMultiProcessor / 2);
-        per_wg = per_wg > 1024 ? per_wg - 1024 : 0;  // room for the static LDS words
-        Q->lds_cap = uint32_t(per_wg);  // bytes of dynamic LDS per workgroup
-        Q->lds_bytes_per_wg = per_wg;
-    }
+    Q->n_cus = prop.multiProcessorCount;
+    Q->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+    Q->lds_per_block = prop.sharedMemPerBlock;
+    set_chunk_wgs(Q, 2);
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream, hipStreamNonBlocking));
     for (auto &e : Q->ev) PG_HIPC(hipEventCreate(&e));
     for (auto &e : Q->wev) PG_HIPC(hipEventCreate(&e));
     for (auto &e : Q->evb) PG_HIPC(hipEventCreate(&e));
-    Q->n_cus = prop.multiProcessorCount;
     Q->wide_min = kDefaultWideMin;
     Q->wide_cus = int(std::lround(kDefaultWideCuFraction * Q->n_cus));
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream_w, hipStreamNonBlocking));
@@ -1452,20 +1501,25 @@ hg_status refresh_order(hg_pool *Q) {
     return HG_OK;
 }
 
-// Online tuner of the wide threshold and the room left for wide launches.
+// Online tuner, in two stages: the chunk kernel's shape (256 threads, two
+// workgroups per CU; 512 threads, one: the same waves per SIMD with twice the
+// LDS per system) at the current launch split, then the launch split (wide
+// threshold and the CUs left for the wide launches) at the better shape.
 // Every candidate runs two intervals (the first may record a graph); the
-// second one's host-measured time per step is its score. The split does not
-// change any result, so tuning happens on live steps.
-void tune_begin(hg_pool *Q) {
+// second one's host-measured time per step is its score. Neither choice
+// changes any result, so tuning happens on live steps.
+void tune_splits(hg_pool *Q) {
     Q->tune_cands.clear();
     size_t largest = 0;
     for (size_t s = 0; s < Q->S; s++)
         if (Q->active[s]) largest = std::max<size_t>(largest, Q->meta[s].n);
-    Q->tune_cands.push_back({0u, Q->wide_cus});
+    Q->tune_cands.push_back({0u, Q->wide_cus, Q->chunk_tpb, Q->wg_per_cu});
     for (uint32_t wm : {1024u, 2048u, 4096u, 8192u}) {
         if (wm > largest) break;  // no system would go wide
-        for (int frac : {16, 8, 4}) Q->tune_cands.push_back({wm, std::max(1, Q->n_cus / frac)});
+        for (int frac : {16, 8, 4})
+            Q->tune_cands.push_back({wm, std::max(1, Q->n_cus / frac), Q->chunk_tpb, Q->wg_per_cu});
     }
+    Q->tune_stage = 1;
     Q->tune_scores.assign(Q->tune_cands.size(), 0.0);
     Q->tune_pos = 0;
     Q->tune_phase = Q->tune_cands.size() > 1 ? TUNE_RUNNING : TUNE_DONE;
@@ -1475,14 +1529,26 @@ void tune_begin(hg_pool *Q) {
     }
 }
 
-void tune_select(hg_pool *Q) {
-    const auto &c = Q->tune_cands[Q->tune_pos / 2];
-    if (Q->wide_min != c.first) {
-        Q->wide_min = c.first;
+void tune_begin(hg_pool *Q) {
+    const uint32_t wm = uint32_t(Q->wide_min);
+    Q->tune_cands = {{wm, Q->wide_cus, 256, 2}, {wm, Q->wide_cus, 512, 1}};
+    Q->tune_stage = 0;
+    Q->tune_scores.assign(Q->tune_cands.size(), 0.0);
+    Q->tune_pos = 0;
+    Q->tune_phase = TUNE_RUNNING;
+}
+
+void tune_apply(hg_pool *Q, const hg_pool::tune_cand &c) {
+    if (Q->wide_min != c.wide_min) {
+        Q->wide_min = c.wide_min;
         Q->order_dirty = true;
     }
-    Q->wide_cus = c.second;
+    Q->wide_cus = c.wide_cus;
+    Q->chunk_tpb = c.tpb;
+    set_chunk_wgs(Q, c.wgs);
 }
+
+void tune_select(hg_pool *Q) { tune_apply(Q, Q->tune_cands[Q->tune_pos / 2]); }
 
 void tune_record(hg_pool *Q, double seconds_per_step) {
     if (Q->tune_pos % 2 == 1) Q->tune_scores[Q->tune_pos / 2] = seconds_per_step;
@@ -1490,10 +1556,12 @@ void tune_record(hg_pool *Q, double seconds_per_step) {
     size_t best = 0;
     for (size_t k = 1; k < Q->tune_cands.size(); k++)
         if (Q->tune_scores[k] < Q->tune_scores[best]) best = k;
-    Q->wide_min = Q->tune_cands[best].first;
-    Q->wide_cus = Q->tune_cands[best].second;
+    tune_apply(Q, Q->tune_cands[best]);
     Q->order_dirty = true;
-    Q->tune_phase = TUNE_DONE;
+    if (Q->tune_stage == 0)
+        tune_splits(Q);
+    else
+        Q->tune_phase = TUNE_DONE;
 }
 }  // namespace
 
@@ -1602,15 +1670,15 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
         HG_HIP(hipEventRecord(Q->wev[1], s2));
     }
     if (count) {
-        const unsigned g = split ? std::min(grid, unsigned(2 * (Q->n_cus - Q->wide_cus))) : grid;
+        const unsigned g = split ? std::min(grid, unsigned(Q->wg_per_cu * (Q->n_cus - Q->wide_cus))) : grid;
         HG_HIP(launch_zero_u32(Q->counter_d, 1, sc));
         HG_HIP(hipEventRecord(Q->ev[2], sc));
         bool facets = false;
         for (size_t k = Q->order.size() - count; k < Q->order.size(); k++) facets |= Q->meta[Q->order[k]].nf > 0;
         hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len,
-                                                                   dt, g, Q->lds_cap, facets, sc)
+                                                                   dt, g, Q->lds_cap, facets, Q->chunk_tpb, sc)
                           : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, g,
-                                                         Q->lds_cap, facets, sc);
+                                                         Q->lds_cap, facets, Q->chunk_tpb, sc);
         HG_HIP(e);
         HG_HIP(hipEventRecord(Q->ev[3], sc));
     }
@@ -1750,9 +1818,26 @@ hg_status hg_pool_get_tuning(hg_pool *Q, size_t *wide_min, int *wide_cus, int *t
     return HG_OK;
 }
 
+hg_status hg_pool_get_chunk_shape(hg_pool *Q, int *threads, int *wg_per_cu) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (threads) *threads = Q->chunk_tpb;
+    if (wg_per_cu) *wg_per_cu = Q->wg_per_cu;
+    return HG_OK;
+}
+
 hg_status hg_pool_set_wide_graph(hg_pool *Q, int enable) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
     Q->wide_graph = enable != 0;
+    return HG_OK;
+}
+
+hg_status hg_pool_set_chunk_shape(hg_pool *Q, int threads, int wg_per_cu) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (threads != 256 && threads != 512) return fail(HG_ERR_INVALID, "hg_pool_set_chunk_shape: threads 256 or 512");
+    if (wg_per_cu < 1 || wg_per_cu > 2048 / threads)
+        return fail(HG_ERR_INVALID, "hg_pool_set_chunk_shape: wg_per_cu 1.." + std::to_string(2048 / threads));
+    Q->chunk_tpb = threads;
+    set_chunk_wgs(Q, wg_per_cu);
     return HG_OK;
 }
 
