@@ -51,11 +51,13 @@ def log(msg):
     """Progress of the legs, on stderr (the JSON line alone goes to stdout)."""
     print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 # The pool's autotuner scores each candidate over two rebuild intervals: two
-# chunk shapes, then 13 launch splits on this batch. With fewer warm-up steps
-# the choice it settles on (512 threads x 1 workgroup per CU; wide threshold
-# 4096, room for CUs/16) is set directly.
-TUNER_STEPS = 30 * REBUILD
-PROFILED_SPLIT = (4096, 16)
+# chunk shapes, then 17 launch splits on this batch. With fewer warm-up steps
+# a fixed choice is set: 512 threads x 1 workgroup per CU (what the tuner
+# picks), wide threshold 8192 with room for CUs/32 (within 1 % of the tuner's
+# best throughput and the best chunk-kernel fraction of a sweep of 10 splits),
+# at which the committed counter profiles were taken.
+TUNER_STEPS = 38 * REBUILD
+PROFILED_SPLIT = (8192, 8)
 PROFILED_SHAPE = (512, 1)
 
 
@@ -405,6 +407,12 @@ COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_hbm_counte
 VALU_COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_valu_counters.txt")
 
 
+def same_shape(txt, tuning):
+    """Whether a counter profile was taken at this run's chunk-kernel shape."""
+    m = re.search(r"chunk shape\s*(\d+)\s*x\s*(\d+)", txt)
+    return bool(m) and (int(m.group(1)), int(m.group(2))) == (tuning["chunk_threads"], tuning["chunk_wg_per_cu"])
+
+
 def measured_traffic(tuning, count, steps_per_launch):
     """HBM bytes per chunk launch from the committed FETCH_SIZE/WRITE_SIZE
     profile (copy-calibrated), taken at 50 steps per launch and scaled to this
@@ -419,8 +427,8 @@ def measured_traffic(tuning, count, steps_per_launch):
     split = re.search(r"launch split fixed at\s*(\d+):(\d+)", txt)
     if not m or not split:
         return None, "counter profile unreadable"
-    if count != 4096 or tuning["wide_min"] != int(split.group(1)):
-        return None, f"counter profile was measured at wide threshold {split.group(1)}, this run differs"
+    if count != 4096 or tuning["wide_min"] != int(split.group(1)) or not same_shape(txt, tuning):
+        return None, f"counter profile was measured at wide threshold {split.group(1)} and another chunk shape"
     scale = steps_per_launch / REBUILD
     return (int(int(m.group(1).replace(",", "")) * scale),
             f"profiles/pool_chunk_mixed_batch4096_hbm_counters.txt (measured at split {split.group(1)}:"
@@ -440,6 +448,8 @@ def measured_valu(tuning, count, launch_us, steps_per_launch):
     f32 = re.search(r"FP32\s+([\d,]+) flop per launch", txt)
     f64 = re.search(r"FP64\s+([\d,]+) flop per launch", txt)
     if not (split and f32 and f64) or count != 4096 or tuning["wide_min"] != int(split.group(1)):
+        return None
+    if not same_shape(txt, tuning):
         return None
     out = {"source": "profiles/pool_chunk_mixed_batch4096_valu_counters.txt"}
     for name, m, peak in (("fp32", f32, 157.3), ("fp64", f64, 78.6)):

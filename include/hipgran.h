@@ -374,7 +374,7 @@ hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
 /* Online autotuner: from the next hg_pool_step on, first the chunk kernel's
  * shape (256 threads x 2 workgroups per CU, or 512 x 1) at the current launch
  * split, then the launch split at the better shape (wide threshold 0, or
- * 1024..8192 up to the largest system, with room for CUs/16, /8 or /4). Each
+ * 1024..8192 up to the largest system, with room for CUs/32, /16, /8 or /4). Each
  * candidate runs two rebuild intervals of live steps, scored by the second
  * one's host-measured time per step; the fastest is kept. Neither choice
  * changes results, so no step is wasted. enable = 0 stops it and keeps the
@@ -476,6 +476,22 @@ hg_status hg_profile_read(hg_system *sys, uint64_t *launches, double *device_ms)
  * same either way. Not in the reference (one code path per functor there):
  * the per-step baseline against which the one-launch path is timed. */
 hg_status hg_set_per_step_launches(hg_system *sys, int enable);
+/* Cumulative dissipated work, the ledger's counterpart of the energy the
+ * damping and friction take out (not in the reference, which reports kinetic
+ * energy only). enable = 1 zeroes the accumulator and from the next step adds
+ * each ordered pair's half of its pair's loss: gamma_n v_n^2 dt, and for the
+ * tangential, rolling and torsion springs the work their forces take from the
+ * relative motion (the rolling term weighted r/rpp, as the reference's rolling
+ * torque and velocity differ by that factor) minus the change of the spring's
+ * energy k/2 |xi|^2; a separating contact loses its springs whole. With it,
+ * hg_energy's total plus this work stays constant to the integrator's error.
+ * FP64 systems with the contact or alt_sinter functor (bonds that do not
+ * break) only; facet terms are not counted, as in hg_energy; steps run as
+ * per-step launches while it is on. enable = 0 stops adding. */
+hg_status hg_set_dissipation_tracking(hg_system *sys, int enable);
+/* out[5]: dissipated work since tracking was enabled (joules): normal,
+ * tangential, rolling, torsion and their total, summed in particle order. */
+hg_status hg_get_dissipated_work(hg_system *sys, double *out);
 
 /* Seeded tunable-dimension cluster-cluster aggregate of n touching spheres of
  * radius r_part (fractal dimension df, prefactor kf). x_out = 3*n doubles.

@@ -325,6 +325,17 @@ class System:
     def profile(self, enable=True):
         _abi.check(_abi.lib().hg_profile_enable(self._h, 1 if enable else 0))
 
+    def track_dissipation(self, enable=True):
+        """Start (zeroing it) or stop the cumulative dissipated-work ledger."""
+        _abi.check(_abi.lib().hg_set_dissipation_tracking(self._h, 1 if enable else 0))
+
+    def dissipated_work(self):
+        """Dissipated work since tracking started (J): normal, tangential,
+        rolling, torsion and total."""
+        out = np.zeros(5)
+        _abi.check(_abi.lib().hg_get_dissipated_work(self._h, _abi.dptr(out)))
+        return dict(zip(("normal", "tangential", "rolling", "torsion", "total"), out))
+
     def per_step_launches(self, enable=True):
         """Per-step kernels for a small FP64 system instead of its one
         launch per rebuild interval (same results; a timing baseline)."""

@@ -105,6 +105,8 @@ SIGNATURES = {
     "hg_pool_set_neck_strengths": (C.c_int, [_sysp, C.c_int32, _dp]),
     "hg_profile_enable": (C.c_int, [_sysp, C.c_int]),
     "hg_set_per_step_launches": (C.c_int, [_sysp, C.c_int]),
+    "hg_set_dissipation_tracking": (C.c_int, [_sysp, C.c_int]),
+    "hg_get_dissipated_work": (C.c_int, [_sysp, _dp]),
     "hg_pool_set_chunk_shape": (C.c_int, [_sysp, C.c_int, C.c_int]),
     "hg_pool_get_chunk_shape": (C.c_int, [_sysp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hg_profile_read": (C.c_int, [_sysp, _u64p, _dp]),

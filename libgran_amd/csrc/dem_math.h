@@ -81,18 +81,43 @@ HGD d3 shear_contact(d3 &xi, d3 n, double stiffness, double damping, double norm
     return min_ref(f_selected, f0n) * t;
 }
 
+// Dissipated work of one ordered pair over one step of length dt (the
+// builder's ledger term; the reference has none). The pair's forces on both
+// particles take work -f_n v_n + f_t.v_t + (r/rpp) f_r.v_r + f_o.v_o per unit
+// time from their motion (the rolling torque uses r, its velocity rpp); the
+// elastic part of f_n is the gradient of U_normal, the rest of the normal
+// work, gamma_n v_n^2, is lost, and of each shear term the work minus the
+// change of its spring's energy k/2 |xi|^2 is lost (damping, Coulomb slip,
+// the rescaled rotation). Each ordered pair holds half of the pair: d4 +=
+// {gamma_n v_n^2 dt / 2, then per shear term c: -w_c (f_c.v_c) dt / 2 -
+// k_c/4 (|xi_c|^2 - |xi_c,old|^2)}, w = 1, r/rpp, 1.
+HGD void pair_dissipation(double *d4, double gamma_n, double v_n, double dt, double w_r, const double k[3],
+                          const d3 f[3], const d3 v[3], const double u_old[3], const spring9 &sp) {
+    d4[0] = d4[0] + 0.5 * gamma_n * v_n * v_n * dt;
+    for (int c = 0; c < 3; c++) {
+        const double w = c == 1 ? w_r : 1.0;
+        d4[1 + c] = d4[1 + c] + (-0.5 * w * dot(f[c], v[c]) * dt - 0.25 * k[c] * (dot(sp.s[c], sp.s[c]) - u_old[c]));
+    }
+}
+
 // ---- contact_force.h:57-96 --------------------------------------------------
 // `active` is false where the reference resets the pair's springs
 // (overlap <= 0, contact_force.h:68-71); the spring is then zero.
 // vw(vj, wj) gives the partner's velocity and angular velocity; it is called
-// only for an overlapping pair (the others need the positions alone).
-template <class VW>
+// only for an overlapping pair (the others need the positions alone). TRACK
+// adds the pair's dissipated work to d4 (pair_dissipation); the springs a
+// separating pair drops are lost whole.
+template <bool TRACK = false, class VW>
 HGD acc6 contact_eval_vw(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 wi, VW &&vw, spring9 &sp,
-                         bool &active) {
+                         bool &active, double *d4 = nullptr) {
     d3 d = xi - xj;
     d3 n = normalized(d);
     double overlap = 2.0 * p.r_part - dot(d, n);
     if (overlap <= 0) {
+        if constexpr (TRACK) {
+            const double k[3] = {p.k_t, p.k_r, p.k_o};
+            for (int c = 0; c < 3; c++) d4[1 + c] = d4[1 + c] + 0.25 * k[c] * dot(sp.s[c], sp.s[c]);
+        }
         active = false;
         sp = zero_spring();
         return {zero3(), zero3()};
@@ -108,9 +133,17 @@ HGD acc6 contact_eval_vw(const hg_contact_params &p, d3 xi, d3 xj, d3 vi, d3 wi,
     d3 v_t = v_ij - dot(v_ij, n) * n;
     d3 v_r = (-rpp) * (cwi - cwj);
     d3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    double u_old[3] = {0.0, 0.0, 0.0};
+    if constexpr (TRACK)
+        for (int c = 0; c < 3; c++) u_old[c] = dot(sp.s[c], sp.s[c]);
     d3 f_t = shear_contact(sp.s[0], n, p.k_t, p.gamma_t, f_n, p.mu_s, p.phi_d, v_t, p.dt);
     d3 f_r = shear_contact(sp.s[1], n, p.k_r, p.gamma_r, f_n, p.mu_r, p.phi_r, v_r, p.dt);
     d3 f_o = shear_contact(sp.s[2], n, p.k_o, p.gamma_o, f_n, p.mu_o, p.phi_o, v_o, p.dt);
+    if constexpr (TRACK) {
+        const double k[3] = {p.k_t, p.k_r, p.k_o};
+        const d3 f[3] = {f_t, f_r, f_o}, v[3] = {v_t, v_r, v_o};
+        pair_dissipation(d4, p.gamma_n, v_n, p.dt, p.r_part / rpp, k, f, v, u_old, sp);
+    }
     d3 tau_t = rpp * cross(n, f_t);
     d3 tau_r = p.r_part * cross(n, f_r);
     d3 tau_o = p.r_part * f_o;
@@ -136,8 +169,10 @@ HGD d3 shear_sinter(d3 &xi, d3 n, double stiffness, double damping, d3 rel_v, do
 
 // ---- alt_sinter_bridge.h:92-129 (bonded branch; no overlap gate) ----------
 // ftot receives f_n n + f_t, the force alt_breaking_sinter_bridge.h:115-118
-// compares with the neck strength.
-HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp, d3 &ftot) {
+// compares with the neck strength. TRACK: as contact_eval_vw.
+template <bool TRACK = false>
+HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 wi, d3 wj, spring9 &sp, d3 &ftot,
+                     double *d4 = nullptr) {
     d3 d = xi - xj;
     d3 n = normalized(d);
     double overlap = 2.0 * p.r_part - dot(d, n);
@@ -149,9 +184,17 @@ HGD acc6 sinter_eval(const hg_sinter_params &p, d3 xi, d3 xj, d3 vi, d3 vj, d3 w
     d3 v_t = v_ij - dot(v_ij, n) * n;
     d3 v_r = (-rpp) * (cwi - cwj);
     d3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    double u_old[3] = {0.0, 0.0, 0.0};
+    if constexpr (TRACK)
+        for (int c = 0; c < 3; c++) u_old[c] = dot(sp.s[c], sp.s[c]);
     d3 f_t = shear_sinter(sp.s[0], n, p.k_t, p.gamma_t, v_t, p.dt);
     d3 f_r = shear_sinter(sp.s[1], n, p.k_r, p.gamma_r, v_r, p.dt);
     d3 f_o = shear_sinter(sp.s[2], n, p.k_o, p.gamma_o, v_o, p.dt);
+    if constexpr (TRACK) {
+        const double k[3] = {p.k_t, p.k_r, p.k_o};
+        const d3 f[3] = {f_t, f_r, f_o}, v[3] = {v_t, v_r, v_o};
+        pair_dissipation(d4, p.gamma_n, v_n, p.dt, p.r_part / rpp, k, f, v, u_old, sp);
+    }
     d3 tau_t = rpp * cross(n, f_t);
     d3 tau_r = p.r_part * cross(n, f_r);
     d3 tau_o = p.r_part * f_o;

@@ -239,33 +239,37 @@ struct lds_id_list {
 // they fetch the last partner again, from cache): a conditional load leaves
 // two values for the compiler to merge, and the merge waits for the load and
 // every memory operation issued before it.
-// IDS is id_list (global memory) or lds_id_list. Only positions are
-// gathered here (loadx(j) returns x_j): the fold loads a partner's velocity
-// and angular velocity itself, for the bonded and overlapping pairs, so the
-// partners that only attract (most of a 3.2r list) cost one 24-byte gather.
-// PREFETCH = false loads each position when its pair comes.
-template <bool PREFETCH = true, class IDS, class LOADX, class FOLD>
-__device__ __forceinline__ void nl_partners(const IDS &ids, LOADX &&loadx, FOLD &&fold) {
+// IDS is id_list (global memory) or lds_id_list. load(j) returns what is
+// gathered ahead for partner j, the fold gets it: positions alone where the
+// state sits in LDS (the fold then loads a partner's velocities itself, for
+// the bonded and overlapping pairs only, a few of a 3.2r list), or the whole
+// pstate from global memory, whose latency only the prefetch hides. PREFETCH
+// = false loads each partner when its pair comes.
+struct pstate {
+    d3 x, v, w;
+};
+template <bool PREFETCH = true, class IDS, class LOAD, class FOLD>
+__device__ __forceinline__ void nl_partners(const IDS &ids, LOAD &&load, FOLD &&fold) {
     if (ids.n == 0) return;
     if constexpr (!PREFETCH) {
         uint32_t jn = ids.at(0);
         for (uint32_t k = 0; k < ids.n; k++) {
             const uint32_t j = jn;
-            const d3 xj = loadx(j);
+            const auto sj = load(j);
             jn = ids.at_clamped(k + 1);
-            fold(j, xj);
+            fold(j, sj);
         }
         return;
     }
     uint32_t jn = ids.at(0), jn2 = ids.at_clamped(1);
-    d3 xn = loadx(jn);
+    auto sn = load(jn);
     for (uint32_t k = 0; k < ids.n; k++) {
         const uint32_t j = jn;
-        const d3 xj = xn;
+        const auto sj = sn;
         jn = jn2;
-        xn = loadx(jn);
+        sn = load(jn);
         jn2 = ids.at_clamped(k + 2);
-        fold(j, xj);
+        fold(j, sj);
     }
 }
 
@@ -274,21 +278,22 @@ __device__ __forceinline__ void nl_partners(const IDS &ids, LOADX &&loadx, FOLD 
 // springs stay in the table; neck() gives the bond's neck strength (read only
 // when bonds break); vw(vj, wj) the partner's velocities (called for bonded
 // and overlapping pairs only).
-template <int BIN, class VW, class NECK>
+// TRACK adds the pair's dissipated work to d4 (pair_dissipation).
+template <int BIN, bool TRACK = false, class VW, class NECK>
 __device__ __forceinline__ acc6 pair_binary(const hg_params &P, d3 xi, d3 xj, d3 vi, d3 wi, VW &&vw, spring9 &sp,
-                                            bool bonded, bool &keep, NECK &&neck) {
+                                            bool bonded, bool &keep, NECK &&neck, double *d4 = nullptr) {
     acc6 b;
     if (BIN == HG_BINARY_SINTER && bonded) {
         d3 ftot, vj, wj;
         vw(vj, wj);
-        b = sinter_eval(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot);
+        b = sinter_eval<TRACK>(P.sinter, xi, xj, vi, vj, wi, wj, sp, ftot, d4);
         // alt_breaking_sinter_bridge.h:117-119: (i, j) stops being bonded
         // once its force exceeds the neck strength; dropping the entry
         // leaves the contact functor a pair with zero springs, as the
         // reference's untouched contact table holds.
         keep = !(P.sinter_breaking && norm(ftot) > neck());
     } else {
-        b = contact_eval_vw(P.contact, xi, xj, vi, wi, vw, sp, keep);
+        b = contact_eval_vw<TRACK>(P.contact, xi, xj, vi, wi, vw, sp, keep, d4);
     }
     return b;
 }
@@ -320,9 +325,9 @@ __device__ __forceinline__ void fold_pair(acc6 &acc, const acc6 &b, const acc6 &
 // One ordered pair through the binary container fold
 // (binary_force_container.h:20-32): {contact | alt_sinter} + {hamaker}.
 // vw(vj, wj) loads the partner's velocities when the pair needs them.
-template <int BIN, int HAM, bool EAGER, class VW>
+template <int BIN, int HAM, bool TRACK = false, bool EAGER, class VW>
 __device__ __forceinline__ void pair_fold_vw(const hg_params &P, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, VW &&vw,
-                                             walker_t<EAGER> &W, acc6 &acc) {
+                                             walker_t<EAGER> &W, acc6 &acc, double *d4 = nullptr) {
     acc6 b{zero3(), zero3()}, h{zero3(), zero3()};
     uint32_t lslot_ij = 0;
     if (BIN == HG_BINARY_SINTER_LEGACY && W.legacy_bonded(j, lslot_ij)) {
@@ -346,7 +351,7 @@ __device__ __forceinline__ void pair_fold_vw(const hg_params &P, uint32_t j, d3 
         bool bonded = false;
         W.take(j, sp, bonded);
         bool keep;
-        b = pair_binary<BIN>(P, xi, xj, vi, wi, vw, sp, bonded, keep, [&] { return W.neck(j); });
+        b = pair_binary<BIN, TRACK>(P, xi, xj, vi, wi, vw, sp, bonded, keep, [&] { return W.neck(j); }, d4);
         if (keep) W.store(j | (bonded ? kBondBit : 0u), sp);
     }
     h = pair_hamaker<HAM>(P, xi, xj);

@@ -95,6 +95,7 @@ struct sys_view {
     double *ss;  // surface springs: (f*9 + c)*n + i
     int nf;
     int *flags;
+    double *diss = nullptr;  // dissipated work, 4 per particle (when tracked)
 };
 
 struct hg_system_impl {
@@ -148,6 +149,9 @@ struct hg_system_impl {
     // launch profiling (hg_profile_enable)
     bool profiling = false;
     bool per_step = false;  // hg_set_per_step_launches: no one-launch small path
+    // dissipated work (hg_set_dissipation_tracking): 4 doubles per particle
+    double *diss = nullptr;
+    bool track_diss = false;
     struct timed_range {
         int phase;
         hipEvent_t begin, end;

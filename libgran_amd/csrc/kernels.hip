@@ -26,7 +26,10 @@ namespace {
 
 // Neighbour-list acceleration pass: one lane per particle i, partners in the
 // list's ascending order, so the per-particle sum has the reference's order.
-template <int BIN, int HAM>
+// TRACK: the particle's share of its pairs' dissipated work this step is
+// added to S.diss (4 doubles per particle: normal, tangential, rolling,
+// torsion).
+template <int BIN, int HAM, bool TRACK = false>
 __device__ __forceinline__ void force_nl_one(const hg_params &P, const sys_view &S, uint32_t i) {
     const d3 xi = ld3(S.x, i), vi = ld3(S.v, i), wi = ld3(S.w, i);
     acc6 acc{zero3(), zero3()};
@@ -35,21 +38,25 @@ __device__ __forceinline__ void force_nl_one(const hg_params &P, const sys_view 
     if constexpr (BIN == HG_BINARY_SINTER_LEGACY) W.init_legacy(S.pt);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    nl_partners(ids, [&](uint32_t j) { return ld3(S.x, j); }, [&](uint32_t j, d3 xj) {
-        pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, [&](d3 &v, d3 &w) { v = ld3(S.v, j), w = ld3(S.w, j); }, W,
-                               acc);
-    });
+    double d4[4] = {0.0, 0.0, 0.0, 0.0};
+    nl_partners(ids, [&](uint32_t j) { return pstate{ld3(S.x, j), ld3(S.v, j), ld3(S.w, j)}; },
+                [&](uint32_t j, const pstate &sj) {
+                    pair_fold_vw<BIN, HAM, TRACK>(P, j, xi, vi, wi, sj.x, [&](d3 &v, d3 &w) { v = sj.v, w = sj.w; },
+                                                  W, acc, d4);
+                });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
     st3(S.al, i, acc.ang);
+    if constexpr (TRACK)
+        for (int c = 0; c < 4; c++) S.diss[4 * size_t(i) + c] = S.diss[4 * size_t(i) + c] + d4[c];
 }
 
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool TRACK>
 __global__ void __launch_bounds__(256) k_force_nl(hg_params P, sys_view S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S.n) return;
-    force_nl_one<BIN, HAM>(P, S, i);
+    force_nl_one<BIN, HAM, TRACK>(P, S, i);
 }
 
 // Mixed-precision neighbour-list pass (HG_PRECISION_MIXED, dem_mixed.h).
@@ -63,10 +70,11 @@ __global__ void __launch_bounds__(256) k_force_nl_mx(hg_params P, mixed_consts M
     if constexpr (BIN != HG_BINARY_NONE) W.init(S.pt, i);
     id_list ids;
     ids.init(S.nb, S.ncnt, S.n, i, S.cap_n);
-    nl_partners(ids, [&](uint32_t j) { return ld3(S.x, j); }, [&](uint32_t j, d3 xj) {
-        pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, [&](d3 &v, d3 &w) { v = ld3(S.v, j), w = ld3(S.w, j); }, W,
-                               acc);
-    });
+    nl_partners(ids, [&](uint32_t j) { return pstate{ld3(S.x, j), ld3(S.v, j), ld3(S.w, j)}; },
+                [&](uint32_t j, const pstate &sj) {
+                    pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, sj.x, [&](d3 &v, d3 &w) { v = sj.v, w = sj.w; }, W,
+                                           acc);
+                });
     close_table<BIN>(W, S.pt.pcnt_new, i, S.flags);
     if (S.nf > 0) acc = acc + facet_fold_mx(P, MC, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, tod(acc.lin));
@@ -82,7 +90,7 @@ struct tile_lds {
 };
 // The pass for particle i = blockIdx.x * blockDim.x + threadIdx.x; every
 // thread of the block takes part (it stages tiles), live or not.
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool TRACK = false>
 __device__ __forceinline__ void force_all_block(const hg_params &P, const sys_view &S, tile_lds &T) {
     auto &sx = T.sx;
     auto &sv = T.sv;
@@ -96,6 +104,7 @@ __device__ __forceinline__ void force_all_block(const hg_params &P, const sys_vi
         wi = ld3(S.w, i);
     }
     acc6 acc{zero3(), zero3()};
+    double d4[4] = {0.0, 0.0, 0.0, 0.0};
     walker_t<false> W;
     if constexpr (BIN != HG_BINARY_NONE) {
         if (live) W.init(S.pt, i);
@@ -117,9 +126,10 @@ __device__ __forceinline__ void force_all_block(const hg_params &P, const sys_vi
             for (uint32_t jj = 0; jj < m; jj++) {
                 const uint32_t j = base + jj;
                 if (j == i) continue;
-                pair_fold<BIN, HAM>(P, j, xi, vi, wi, d3{sx[0][jj], sx[1][jj], sx[2][jj]},
-                                    d3{sv[0][jj], sv[1][jj], sv[2][jj]}, d3{sw[0][jj], sw[1][jj], sw[2][jj]}, W,
-                                    acc);
+                pair_fold_vw<BIN, HAM, TRACK>(
+                    P, j, xi, vi, wi, d3{sx[0][jj], sx[1][jj], sx[2][jj]},
+                    [&](d3 &v, d3 &w) { v = d3{sv[0][jj], sv[1][jj], sv[2][jj]}, w = d3{sw[0][jj], sw[1][jj], sw[2][jj]}; },
+                    W, acc, d4);
             }
         }
     }
@@ -128,12 +138,14 @@ __device__ __forceinline__ void force_all_block(const hg_params &P, const sys_vi
     if (S.nf > 0) acc = acc + facet_fold(P, S.facets, S.nf, S.ss, S.n, i, xi, vi, wi);
     st3(S.a, i, acc.lin);
     st3(S.al, i, acc.ang);
+    if constexpr (TRACK)
+        for (int c = 0; c < 4; c++) S.diss[4 * size_t(i) + c] = S.diss[4 * size_t(i) + c] + d4[c];
 }
 
-template <int BIN, int HAM>
+template <int BIN, int HAM, bool TRACK>
 __global__ void __launch_bounds__(kTile) k_force_all(hg_params P, sys_view S) {
     __shared__ tile_lds T;
-    force_all_block<BIN, HAM>(P, S, T);
+    force_all_block<BIN, HAM, TRACK>(P, S, T);
 }
 
 // Kick (completing the previous step when `pending`), kick, drift:
@@ -566,11 +578,21 @@ hipError_t launch_force_t(const hg_params &p, const sys_view &sv, hipStream_t s)
     if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST && p.precision == HG_PRECISION_MIXED) {
         if constexpr (BIN == HG_BINARY_SINTER_LEGACY) return hipErrorInvalidValue;  // FP64 only
         else k_force_nl_mx<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, make_mixed_consts(p), sv);
+    } else if (sv.diss) {
+        // dissipation tracking (FP64 contact and alt_sinter bonds only)
+        if constexpr (BIN == HG_BINARY_CONTACT || BIN == HG_BINARY_SINTER) {
+            if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST)
+                k_force_nl<BIN, HAM, true><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
+            else
+                k_force_all<BIN, HAM, true><<<blocks_for(sv.n, kTile), kTile, 0, s>>>(p, sv);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else if (p.pair_mode == HG_PAIRS_NEIGHBOR_LIST) {
         // 64-lane blocks spread a modest particle count over more CUs.
-        k_force_nl<BIN, HAM><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
+        k_force_nl<BIN, HAM, false><<<blocks_for(sv.n, 64), 64, 0, s>>>(p, sv);
     } else {
-        k_force_all<BIN, HAM><<<blocks_for(sv.n, kTile), kTile, 0, s>>>(p, sv);
+        k_force_all<BIN, HAM, false><<<blocks_for(sv.n, kTile), kTile, 0, s>>>(p, sv);
     }
     return hipGetLastError();
 }

@@ -231,22 +231,37 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         if constexpr (BIN != HG_BINARY_NONE) W.init(T, g);
         id_list ids;
         if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST && !LIDS) ids.init(V.nb, V.ncnt, V.stride, g, V.cap_n);
-        // listed partners of g through fold(j, xj), in list order; vw_of(j)
-        // loads a partner's velocities when its pair needs them
+        // listed partners of g through fold(j, xj, vw), in list order: vw
+        // gives a partner's velocities when its pair needs them, from LDS or
+        // from a global gather made a pair ahead with its position
         auto partners = [&](auto &&fold) {
-            auto ldx = [&](uint32_t j) { return load_x<LDS>(V, M.offset, j); };
+            auto run = [&](const auto &ids_) {
+                if constexpr (LDS == STATE_GLOBAL) {
+                    nl_partners<kChunkPrefetch>(
+                        ids_, [&](uint32_t j) { return pstate{ld3(V.x, j), ld3(V.v, j), ld3(V.w, j)}; },
+                        [&](uint32_t j, const pstate &sj) {
+                            fold(j, sj.x, [&](d3 &v, d3 &w) { v = sj.v, w = sj.w; });
+                        });
+                } else {
+                    nl_partners<kChunkPrefetch>(
+                        ids_, [&](uint32_t j) { return load_x<LDS>(V, M.offset, j); },
+                        [&](uint32_t j, d3 xj) {
+                            fold(j, xj, [&](d3 &v, d3 &w) { load_vw<LDS>(V, M.offset, j, v, w); });
+                        });
+                }
+            };
             if constexpr (LIDS) {
                 const uint32_t t = g - M.offset;
-                nl_partners<kChunkPrefetch>(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]}, ldx, fold);
+                run(lds_id_list{lids, M.n, t, M.offset, lids[kmax * M.n + t]});
             } else {
-                nl_partners<kChunkPrefetch>(ids, ldx, fold);
+                run(ids);
             }
         };
         auto vw_of = [&](uint32_t j) { return [&V, &M, j](d3 &v, d3 &w) { load_vw<LDS>(V, M.offset, j, v, w); }; };
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj) { pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, vw_of(j), W, acc); });
+                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, vw, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
@@ -259,7 +274,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         } else {
             accf acc{zerof(), zerof()};
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vw_of(j), W, acc); });
+                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vw, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
@@ -1516,7 +1531,7 @@ void tune_splits(hg_pool *Q) {
     Q->tune_cands.push_back({0u, Q->wide_cus, Q->chunk_tpb, Q->wg_per_cu});
     for (uint32_t wm : {1024u, 2048u, 4096u, 8192u}) {
         if (wm > largest) break;  // no system would go wide
-        for (int frac : {16, 8, 4})
+        for (int frac : {32, 16, 8, 4})
             Q->tune_cands.push_back({wm, std::max(1, Q->n_cus / frac), Q->chunk_tpb, Q->wg_per_cu});
     }
     Q->tune_stage = 1;

@@ -196,6 +196,7 @@ sys_view make_view(hg_system *S) {
     v.pt.lslot = S->lslot;
     v.facets = S->facets;
     v.ss = S->ss;
+    v.diss = S->track_diss ? S->diss : nullptr;
     v.nf = S->nf;
     v.flags = S->flags;
     return v;
@@ -546,7 +547,8 @@ hg_status step_impl(hg_system *S, double dt, uint64_t nsteps, uint32_t rebuild_e
     const uint32_t n = uint32_t(S->n);
     const bool nl = S->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     const int nf_advance = S->p.facet_advance ? 0 : S->nf;  // facets the step moves itself
-    if (n <= kSmallSystemMax && S->p.precision == HG_PRECISION_FP64 && !S->profiling && !S->per_step) {
+    if (n <= kSmallSystemMax && S->p.precision == HG_PRECISION_FP64 && !S->profiling && !S->per_step &&
+        !S->track_diss) {
         // small systems: the steps between list rebuilds in one launch of one
         // workgroup instead of three dependent launches per step
         if (S->p.binary_model == HG_BINARY_SINTER_LEGACY && !S->legacy_ready)
@@ -792,6 +794,37 @@ hg_status hg_profile_enable(hg_system *S, int enable) {
         S->prof_ms[k] = 0.0;
     }
     S->profiling = enable != 0;
+    return HG_OK;
+}
+
+hg_status hg_set_dissipation_tracking(hg_system *S, int enable) {
+    if (!S) return fail(HG_ERR_INVALID, "system is null");
+    if (enable) {
+        if (S->p.precision != HG_PRECISION_FP64 ||
+            (S->p.binary_model != HG_BINARY_CONTACT && S->p.binary_model != HG_BINARY_SINTER) ||
+            S->p.sinter_breaking)
+            return fail(HG_ERR_INVALID, "hg_set_dissipation_tracking: FP64 systems with the contact or alt_sinter "
+                                        "functor (bonds that do not break) only");
+        if (!S->diss) HG_TRY(dalloc(S, &S->diss, 4 * S->n));
+        HG_HIP(launch_zero_f64(S->diss, 4 * S->n, S->stream));
+    }
+    S->track_diss = enable != 0;
+    return HG_OK;
+}
+
+hg_status hg_get_dissipated_work(hg_system *S, double *out) {
+    if (!S || !out) return fail(HG_ERR_INVALID, "system and out are required");
+    if (!S->diss) return fail(HG_ERR_INVALID, "hg_get_dissipated_work: tracking was never enabled");
+    std::vector<double> h(4 * S->n);
+    HG_HIP(hipMemcpyAsync(h.data(), S->diss, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HG_HIP(hipStreamSynchronize(S->stream));
+    if (S->flags_unchecked) HG_TRY(check_flags(S, "hg_get_dissipated_work"));
+    for (int c = 0; c < 4; c++) {
+        double sum = 0.0;
+        for (size_t i = 0; i < S->n; i++) sum = sum + h[4 * i + c];
+        out[c] = sum;
+    }
+    out[4] = ((out[0] + out[1]) + out[2]) + out[3];
     return HG_OK;
 }
 

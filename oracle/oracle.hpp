@@ -160,13 +160,34 @@ inline v3 shear_contact(v3 &xi, v3 n, double stiffness, double damping, double n
     return std::min(f_selected, norm(f0)) * t;
 }
 
+// Dissipated work of one ordered pair over one evaluation (the builder's
+// ledger term, not in the reference): the pair's forces take work
+// -f_n v_n + f_t.v_t + (r/rpp) f_r.v_r + f_o.v_o per unit time from the two
+// particles' motion (derived from the torques the functors apply: the rolling
+// torque uses r, the rolling velocity rpp). gamma_n v_n^2 of it is lost, and
+// of each shear term the work minus the change of its spring energy
+// k/2 |xi|^2. An ordered pair holds half of its pair.
+inline void pair_dissipation(double *d4, double gamma_n, double v_n, double dt, double w_r, const double k[3],
+                             const v3 f[3], const v3 v[3], const double u_old[3], const spring3 &sp) {
+    d4[0] = d4[0] + 0.5 * gamma_n * v_n * v_n * dt;
+    for (int c = 0; c < 3; c++) {
+        const double w = c == 1 ? w_r : 1.0;
+        d4[1 + c] = d4[1 + c] + (-0.5 * w * dot(f[c], v[c]) * dt - 0.25 * k[c] * (dot(sp.s[c], sp.s[c]) - u_old[c]));
+    }
+}
+
 // contact_force.h:57-96. `sp` is the (i,j) spring triple; `reset` is set when the
-// reference would reset it (overlap <= 0).
+// reference would reset it (overlap <= 0). With d4, the pair's dissipated work
+// (pair_dissipation; a reset spring's energy is lost whole) is added to it.
 inline acc2 contact_eval(const contact_params &p, v3 xi_, v3 xj, v3 vi, v3 vj, v3 wi, v3 wj,
-                         spring3 &sp, bool &reset) {
+                         spring3 &sp, bool &reset, double *d4 = nullptr) {
     v3 n = normalized(xi_ - xj);
     double overlap = 2.0 * p.r_part - dot(xi_ - xj, n);
     if (overlap <= 0) {
+        if (d4) {
+            const double k[3] = {p.k_t, p.k_r, p.k_o};
+            for (int c = 0; c < 3; c++) d4[1 + c] = d4[1 + c] + 0.25 * k[c] * dot(sp.s[c], sp.s[c]);
+        }
         reset = true;
         sp = spring3{{vzero, vzero, vzero}};
         return {vzero, vzero};
@@ -179,9 +200,15 @@ inline acc2 contact_eval(const contact_params &p, v3 xi_, v3 xj, v3 vi, v3 vj, v
     v3 v_t = v_ij - dot(v_ij, n) * n;
     v3 v_r = (-rpp) * (cross(n, wi) - cross(n, wj));
     v3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    const double u_old[3] = {dot(sp.s[0], sp.s[0]), dot(sp.s[1], sp.s[1]), dot(sp.s[2], sp.s[2])};
     v3 f_t = shear_contact(sp.s[0], n, p.k_t, p.gamma_t, f_n, p.mu_s, p.phi_d, v_t, p.dt);
     v3 f_r = shear_contact(sp.s[1], n, p.k_r, p.gamma_r, f_n, p.mu_r, p.phi_r, v_r, p.dt);
     v3 f_o = shear_contact(sp.s[2], n, p.k_o, p.gamma_o, f_n, p.mu_o, p.phi_o, v_o, p.dt);
+    if (d4) {
+        const double k[3] = {p.k_t, p.k_r, p.k_o};
+        const v3 f[3] = {f_t, f_r, f_o}, vv[3] = {v_t, v_r, v_o};
+        pair_dissipation(d4, p.gamma_n, v_n, p.dt, p.r_part / rpp, k, f, vv, u_old, sp);
+    }
     v3 tau_t = rpp * cross(n, f_t);
     v3 tau_r = p.r_part * cross(n, f_r);
     v3 tau_o = p.r_part * f_o;
@@ -206,7 +233,7 @@ inline v3 shear_sinter(v3 &xi, v3 n, double stiffness, double damping, v3 rel_v,
 // alt_breaking_sinter_bridge.h:85-125 evaluates the same expressions and also
 // needs the force before the division by the mass: *ftot receives f_n n + f_t.
 inline acc2 sinter_bonded_eval(const sinter_params &p, v3 xi_, v3 xj, v3 vi, v3 vj, v3 wi, v3 wj,
-                               spring3 &sp, v3 *ftot = nullptr) {
+                               spring3 &sp, v3 *ftot = nullptr, double *d4 = nullptr) {
     v3 n = normalized(xi_ - xj);
     double overlap = 2.0 * p.r_part - dot(xi_ - xj, n);
     double rpp = p.r_part - overlap / 2.0;
@@ -216,9 +243,15 @@ inline acc2 sinter_bonded_eval(const sinter_params &p, v3 xi_, v3 xj, v3 vi, v3 
     v3 v_t = v_ij - dot(v_ij, n) * n;
     v3 v_r = (-rpp) * (cross(n, wi) - cross(n, wj));
     v3 v_o = (p.r_part * (dot(n, wi) - dot(n, wj))) * n;
+    const double u_old[3] = {dot(sp.s[0], sp.s[0]), dot(sp.s[1], sp.s[1]), dot(sp.s[2], sp.s[2])};
     v3 f_t = shear_sinter(sp.s[0], n, p.k_t, p.gamma_t, v_t, p.dt);
     v3 f_r = shear_sinter(sp.s[1], n, p.k_r, p.gamma_r, v_r, p.dt);
     v3 f_o = shear_sinter(sp.s[2], n, p.k_o, p.gamma_o, v_o, p.dt);
+    if (d4) {
+        const double k[3] = {p.k_t, p.k_r, p.k_o};
+        const v3 f[3] = {f_t, f_r, f_o}, vv[3] = {v_t, v_r, v_o};
+        pair_dissipation(d4, p.gamma_n, v_n, p.dt, p.r_part / rpp, k, f, vv, u_old, sp);
+    }
     v3 tau_t = rpp * cross(n, f_t);
     v3 tau_r = p.r_part * cross(n, f_r);
     v3 tau_o = p.r_part * f_o;
@@ -529,6 +562,10 @@ public:
     std::vector<std::vector<uint32_t>> nbr;
     std::vector<facet> facets;
     std::vector<spring_rows> surface_springs;  // one row set per facet, key 0
+    // cumulative dissipated work per particle (normal, tangential, rolling,
+    // torsion), accumulated while track_diss is set
+    bool track_diss = false;
+    std::vector<double> diss;
 
     dem_system(const system_params &prm, size_t n_, const double *x0, const double *v0,
                const double *th0, const double *w0)
@@ -614,7 +651,7 @@ public:
     }
 
     // One ordered pair through the binary container fold.
-    acc2 pair(size_t i, uint32_t j) {
+    acc2 pair(size_t i, uint32_t j, double *d4 = nullptr) {
         bool have_b = p.binary_model != BIN_NONE, have_h = p.hamaker_model != HAM_NONE;
         acc2 b{vzero, vzero}, h{vzero, vzero};
         int64_t lb = p.binary_model == BIN_SINTER_LEGACY ? legacy_bond_of(i, j) : -1;
@@ -626,7 +663,8 @@ public:
         } else if (have_b) {
             if (p.binary_model == BIN_SINTER && is_bonded(i, j)) {
                 v3 ftot;
-                b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], sinter_springs.at(i, j), &ftot);
+                b = sinter_bonded_eval(p.sinter, x[i], x[j], v[i], v[j], w[i], w[j], sinter_springs.at(i, j), &ftot,
+                                       d4);
                 // alt_breaking_sinter_bridge.h:117-119: the ordered pair (i, j)
                 // stops being bonded once its force exceeds the neck strength;
                 // from then on the embedded contact functor takes it, with
@@ -643,7 +681,7 @@ public:
                 bool present = it != row.end() && it->first == j;
                 if (present) tmp = it->second;
                 bool reset = false;
-                b = contact_eval(p.contact, x[i], x[j], v[i], v[j], w[i], w[j], tmp, reset);
+                b = contact_eval(p.contact, x[i], x[j], v[i], v[j], w[i], w[j], tmp, reset, d4);
                 if (reset) {
                     if (present) row.erase(it);
                 } else if (present) {
@@ -775,20 +813,24 @@ public:
         for (long ii = 0; ii < nn; ii++) {
             size_t i = size_t(ii);
             acc2 acc{vzero, vzero};
+            double d4[4] = {0.0, 0.0, 0.0, 0.0};
+            double *dp = track_diss ? d4 : nullptr;
             if (p.neighbor_mode == 1) {
                 for (uint32_t j : nbr[i]) {
-                    acc2 e = pair(i, j);
+                    acc2 e = pair(i, j, dp);
                     acc.lin = acc.lin + e.lin;
                     acc.ang = acc.ang + e.ang;
                 }
             } else {
                 for (uint32_t j = 0; j < uint32_t(n); j++) {
                     if (j == i) continue;
-                    acc2 e = pair(i, j);
+                    acc2 e = pair(i, j, dp);
                     acc.lin = acc.lin + e.lin;
                     acc.ang = acc.ang + e.ang;
                 }
             }
+            if (track_diss)
+                for (int c = 0; c < 4; c++) diss[4 * i + c] = diss[4 * i + c] + d4[c];
             if (!facets.empty()) acc = acc + unary(i);
             a[i] = acc.lin;
             al[i] = acc.ang;

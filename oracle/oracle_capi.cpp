@@ -462,6 +462,23 @@ void or_sys_energy_terms(void *s, double mass, double inertia, double *out) {
     static_cast<dem_system *>(s)->energy_terms(mass, inertia, out);
 }
 
+// Dissipated-work ledger: enable zeroes and starts it; out[5] = normal,
+// tangential, rolling, torsion and total, summed in particle order.
+void or_sys_set_dissipation_tracking(void *s, int enable) {
+    auto *S = static_cast<dem_system *>(s);
+    if (enable) S->diss.assign(4 * S->n, 0.0);
+    S->track_diss = enable != 0;
+}
+void or_sys_dissipated_work(void *s, double *out) {
+    auto *S = static_cast<dem_system *>(s);
+    for (int c = 0; c < 4; c++) {
+        double sum = 0.0;
+        for (size_t i = 0; i < S->diss.size() / 4; i++) sum = sum + S->diss[4 * i + c];
+        out[c] = sum;
+    }
+    out[4] = ((out[0] + out[1]) + out[2]) + out[3];
+}
+
 // Per particle: sum over evaluated partners and over the functors of each
 // pair of |functor acceleration| (linear, angular), without changing any
 // spring: the scale of the terms a rounded fold adds, including terms that

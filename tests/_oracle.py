@@ -141,6 +141,8 @@ def lib():
         L.or_closest_point.argtypes = [_dp, C.c_size_t, _dp, _dp]
         L.or_breaking_seq.argtypes = [_dp, _dp, C.c_double, C.c_size_t, _dp, _dp, C.POINTER(C.c_int)]
         L.or_sys_set_neck_strengths.argtypes = [C.c_void_p, _dp]
+        L.or_sys_set_dissipation_tracking.argtypes = [C.c_void_p, C.c_int]
+        L.or_sys_dissipated_work.argtypes = [C.c_void_p, _dp]
         _szp = C.POINTER(C.c_size_t)
         L.or_legacy_frames.restype = C.c_size_t
         L.or_legacy_frames.argtypes = [C.c_size_t, _dp, C.c_double, C.c_double, _dp, C.c_size_t, _u32p, _dp,
@@ -294,6 +296,15 @@ class OracleSystem:
         if getattr(self, "_h", None):
             lib().or_sys_destroy(self._h)
             self._h = None
+
+    def track_dissipation(self, enable=True):
+        """Start (zeroing it) or stop the cumulative dissipated-work ledger."""
+        lib().or_sys_set_dissipation_tracking(self._h, 1 if enable else 0)
+
+    def dissipated_work(self):
+        out = np.zeros(5)
+        lib().or_sys_dissipated_work(self._h, dptr(out))
+        return dict(zip(("normal", "tangential", "rolling", "torsion", "total"), out))
 
     def set_neck_strengths(self, strengths):
         """alt_breaking_sinter neck strengths, one per bond in bonds() order."""

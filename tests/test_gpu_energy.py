@@ -223,3 +223,49 @@ def test_thermo_series_as_the_reference_driver(tmp_path):
             assert a == pytest.approx(b, rel=6e-6), f"line {k}: {line}"
     o.step(DT, 100, rebuild_every=50)
     np.testing.assert_array_equal(g.state()["x"], o.state()["x"])
+
+
+def test_dissipated_work_equals_the_oracle():
+    """hg_get_dissipated_work against the oracle's ledger: a damped glancing
+    contact with spin (all pairs), and a damped aggregate through neighbour
+    lists (bonds and contacts). The per-pair terms are the same FP64
+    arithmetic and both sum per particle in pair order, then over particles in
+    particle order: equal to the last bit, and the run itself unchanged."""
+    print("[gpu-energy] dissipated work", flush=True)
+    L = _L()
+    dt = 1e-14
+    g = 0.4e-9
+    con = np.array([1.0e4, 2.0e-9, 1.0e4, g, 0.3, 0.8, 1.0e4, g, 0.3, 0.8, 1.0e4, g, 0.3, 0.8, R, O.MASS, O.INERTIA,
+                    dt])
+    x = np.array([[0.0, 0.0, 0.0], [np.sqrt((2.0 * R + 1e-11) ** 2 - (0.4 * R) ** 2), 0.4 * R, 0.0]])
+    v = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+    w = np.array([[0.0, 0.0, 3e7], [2e7, 0.0, -1e7]])
+    cases = [("glancing contact", L.make_params(L.BINARY_CONTACT, L.HAMAKER_NONE, contact=con),
+              O.system_params(O.BIN_CONTACT, O.HAM_NONE, contact=con), x, v, w, 3000, 0)]
+    xa = L.generate_aggregate(500, R, 1.8, 1.3, 31)
+    va = np.random.default_rng(31).normal(0.0, 2.0, xa.shape)
+    c, s, h = S.aggregate_params()
+    cases.append(("aggregate, lists", L.make_params(L.BINARY_SINTER, L.HAMAKER_LIMITED, pair_mode=L.PAIRS_NEIGHBOR_LIST,
+                                                     r_verlet=3.2 * R, contact=c, sinter=s, hamaker=h),
+                  O.system_params(O.BIN_SINTER, O.HAM_LIMITED, neighbor_mode=1, r_verlet=3.2 * R, contact=c,
+                                  sinter=s, hamaker=h), xa, va, None, 300, 50))
+    for name, pg, po, x0, v0, w0, steps, every in cases:
+        gs = L.System(pg, x0, v0, omega0=w0)
+        o = O.OracleSystem(po, x0, v0, w0=w0)
+        gs.track_dissipation()
+        o.track_dissipation()
+        ddt = 1e-14 if every == 0 else 1e-13
+        gs.step(ddt, steps, rebuild_every=every)
+        o.step(ddt, steps, rebuild_every=every)
+        wg, wo = gs.dissipated_work(), o.dissipated_work()
+        print(f"[gpu-energy] {name}: {wg}", flush=True)
+        assert wo["total"] > 0.0
+        for k in wo:
+            assert wg[k] == wo[k], (name, k, wg[k], wo[k])
+        st = o.state()
+        for f in ("x", "v", "omega"):
+            np.testing.assert_array_equal(gs.state()[f], st[f], err_msg=f"{name}: {f}")
+    bad = L.System(L.make_params(L.BINARY_CONTACT, L.HAMAKER_NONE, contact=con, pair_mode=L.PAIRS_NEIGHBOR_LIST,
+                                 r_verlet=3.2 * R, precision=L.PRECISION_MIXED), xa, va)
+    with pytest.raises(L.HipgranError):
+        bad.track_dissipation()
