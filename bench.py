@@ -51,13 +51,14 @@ def log(msg):
     """Progress of the legs, on stderr (the JSON line alone goes to stdout)."""
     print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 # The pool's autotuner scores each candidate over two rebuild intervals: two
-# chunk shapes, then 17 launch splits on this batch. With fewer warm-up steps
+# chunk shapes, then 21 launch splits on this batch. With fewer warm-up steps
 # a fixed choice is set: 512 threads x 1 workgroup per CU (what the tuner
-# picks), wide threshold 8192 with room for CUs/32 (within 1 % of the tuner's
-# best throughput and the best chunk-kernel fraction of a sweep of 10 splits),
-# at which the committed counter profiles were taken.
-TUNER_STEPS = 38 * REBUILD
-PROFILED_SPLIT = (8192, 8)
+# picks) and wide threshold 4096 with the wide steps run before the chunk
+# kernel rather than beside it (the fastest of a sweep of 14 splits, and the
+# best chunk-kernel fraction among the fast ones), at which the committed
+# counter profiles were taken.
+TUNER_STEPS = 46 * REBUILD
+PROFILED_SPLIT = (4096, 0)
 PROFILED_SHAPE = (512, 1)
 
 
@@ -601,9 +602,17 @@ def main():
     cpu = None
     if not args.only_batched:
         log("FP64 pool")
-        pool, el_64, _, _, _, tuning64, _ = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps, args.warmup,
-                                                     split, chunk_shape)
+        pool, el_64, prof64, kbar64, cbar64, tuning64, narrow64 = run_pool(L, L.PRECISION_FP64, sizes, xs, vs,
+                                                                           args.steps, args.warmup, split, chunk_shape)
         pool.close()
+        c64 = max(prof64["chunks"], 1)
+        us64 = prof64["chunk_ms"] * 1e3 / c64
+        b64 = state_bytes_per_particle_step(kbar64, cbar64) * narrow64 * prof64["chunk_steps"] / c64
+        extra["roofline_fp64_chunk"] = {
+            "kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,FP64>", "avg_launch_us": round(us64, 1),
+            "steps_per_launch": round(prof64["chunk_steps"] / c64, 2), "particles_per_launch": narrow64,
+            "alg_bytes_per_launch": int(b64), "achieved": round(b64 / (us64 * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(b64 / (us64 * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
         log("plain FP64 path on a sample of the batch")
         plain, plain_note = plain_fp64_sample(L, sizes, xs, vs, 200)
         log("single N=20,000 aggregate, FP64 and mixed")

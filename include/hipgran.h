@@ -363,9 +363,11 @@ hg_status hg_pool_last_step_profile(hg_pool *pool, double *chunk_ms, double *reb
  * hg_pool_step. */
 hg_status hg_pool_set_wide_threshold(hg_pool *pool, size_t wide_min);
 /* While both kinds of systems are present, the chunk kernel's persistent grid
- * shrinks from two workgroups per CU to 2 * (CUs - wide_cus), leaving room
- * worth wide_cus CUs (default a quarter) for the wide launches that run
- * beside it. 0 < wide_cus < CU count. */
+ * shrinks to wg_per_cu * (CUs - wide_cus) workgroups, leaving room worth
+ * wide_cus CUs (default a quarter) for the wide launches that run beside it
+ * on a second stream. wide_cus = 0 runs them one after the other instead:
+ * the interval's wide steps, then the chunk kernel, each on the whole GPU.
+ * 0 <= wide_cus < CU count. */
 hg_status hg_pool_set_wide_cus(hg_pool *pool, int wide_cus);
 /* The wide launches of one interval (2 per step + 1) are recorded once into a
  * HIP graph per (interval length, dt, wide set) and replayed; enable = 0
@@ -374,7 +376,8 @@ hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
 /* Online autotuner: from the next hg_pool_step on, first the chunk kernel's
  * shape (256 threads x 2 workgroups per CU, or 512 x 1) at the current launch
  * split, then the launch split at the better shape (wide threshold 0, or
- * 1024..8192 up to the largest system, with room for CUs/32, /16, /8 or /4). Each
+ * 1024..8192 up to the largest system, run one after the other or beside the
+ * chunk kernel with room for CUs/32, /16, /8 or /4). Each
  * candidate runs two rebuild intervals of live steps, scored by the second
  * one's host-measured time per step; the fastest is kept. Neither choice
  * changes results, so no step is wasted. enable = 0 stops it and keeps the

@@ -1531,6 +1531,7 @@ void tune_splits(hg_pool *Q) {
     Q->tune_cands.push_back({0u, Q->wide_cus, Q->chunk_tpb, Q->wg_per_cu});
     for (uint32_t wm : {1024u, 2048u, 4096u, 8192u}) {
         if (wm > largest) break;  // no system would go wide
+        Q->tune_cands.push_back({wm, 0, Q->chunk_tpb, Q->wg_per_cu});  // one after the other
         for (int frac : {32, 16, 8, 4})
             Q->tune_cands.push_back({wm, std::max(1, Q->n_cus / frac), Q->chunk_tpb, Q->wg_per_cu});
     }
@@ -1629,7 +1630,9 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
     HG_HIP(hipEventRecord(Q->ev[1], Q->stream));
     // wide systems: per-step launches on their own CUs, beside the chunk
     // kernel on the others (all CUs go to whichever kind is alone)
-    const bool split = Q->wide_blocks && count;
+    // wide_cus = 0: no split; the wide steps are queued on the pool's stream
+    // ahead of the chunk kernel and each kind has the whole GPU in turn
+    const bool split = Q->wide_blocks && count && Q->wide_cus > 0;
     hipStream_t sc = Q->stream;
     if (Q->wide_blocks) {
         hipStream_t s2 = split ? Q->stream_w : Q->stream;
@@ -1858,8 +1861,8 @@ hg_status hg_pool_set_chunk_shape(hg_pool *Q, int threads, int wg_per_cu) {
 
 hg_status hg_pool_set_wide_cus(hg_pool *Q, int wide_cus) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
-    if (wide_cus < 1 || wide_cus >= Q->n_cus)
-        return fail(HG_ERR_INVALID, "hg_pool_set_wide_cus: need 0 < wide_cus < " + std::to_string(Q->n_cus));
+    if (wide_cus < 0 || wide_cus >= Q->n_cus)
+        return fail(HG_ERR_INVALID, "hg_pool_set_wide_cus: need 0 <= wide_cus < " + std::to_string(Q->n_cus));
     Q->wide_cus = wide_cus;
     return HG_OK;
 }
