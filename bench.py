@@ -96,10 +96,11 @@ def state_bytes_per_particle_step(kbar, cbar):
     return 288 + 4 * kbar + 144 * cbar + (24 + 4 * kbar) / REBUILD
 
 
-def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None, chunk_shape=None):
+def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None, chunk_shape=None, packing=True):
     import torch
     pool = L.Pool(params(L, precision), max_particles=int(sizes.sum()) + 1024, max_systems=len(xs))
     slots = [pool.add(x, v) for x, v in zip(xs, vs)]
+    pool.set_packing(packing)
     if chunk_shape:
         pool.set_chunk_shape(*chunk_shape)
     if split is None:
@@ -116,6 +117,7 @@ def run_pool(L, precision, sizes, xs, vs, steps, warmup, split=None, chunk_shape
     el = time.perf_counter() - t0
     prof = pool.last_step_profile()
     tuning = pool.tuning()
+    tuning["packing"] = packing
     # measured neighbour and spring counts on a sample of the chunk kernel's systems
     wm = tuning["wide_min"]
     narrow = [s for s, n in zip(slots, sizes) if not wm or n < wm]
@@ -409,9 +411,13 @@ VALU_COUNTERS = os.path.join(ROOT, "profiles", "pool_chunk_mixed_batch4096_valu_
 
 
 def same_shape(txt, tuning):
-    """Whether a counter profile was taken at this run's chunk-kernel shape."""
+    """Whether a counter profile was taken at this run's chunk-kernel shape and
+    packing (a profile that names no packing was taken without)."""
     m = re.search(r"chunk shape\s*(\d+)\s*x\s*(\d+)", txt)
-    return bool(m) and (int(m.group(1)), int(m.group(2))) == (tuning["chunk_threads"], tuning["chunk_wg_per_cu"])
+    pk = re.search(r"packing (on|off)", txt)
+    packed = bool(pk) and pk.group(1) == "on"
+    return (bool(m) and (int(m.group(1)), int(m.group(2))) == (tuning["chunk_threads"], tuning["chunk_wg_per_cu"])
+            and packed == tuning.get("packing", True))
 
 
 def measured_traffic(tuning, count, steps_per_launch):
@@ -512,6 +518,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3000)
     ap.add_argument("--serving-requests", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pack", action="store_true", help="one chunk workgroup per system (no packs of small ones)")
     ap.add_argument("--split", default=None, help="WIDE_MIN:WIDE_CUS fixed launch split instead of the autotuner")
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
     ap.add_argument("--chunk-shape", default=None, help="THREADS:WG_PER_CU chunk-kernel shape (default: tuned)")
@@ -570,7 +577,8 @@ def main():
         split = PROFILED_SPLIT
         chunk_shape = chunk_shape or PROFILED_SHAPE
     pool, el_mx, prof_mx, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps,
-                                                                 args.warmup, split, chunk_shape)
+                                                                 args.warmup, split, chunk_shape,
+                                                                 not args.no_pack)
     pool.close()
     value = total * args.steps / el_mx
     log(f"mixed pool: {value:.4g} particle-steps/s")
@@ -588,6 +596,7 @@ def main():
                           "the steps of one rebuild interval or of the part of one the timed steps cover, per launch)",
                 "avg_launch_us": round(chunk_us, 1), "launches": int(prof_mx["chunks"]),
                 "chunk_shape": f"{tuning['chunk_threads']} threads x {tuning['chunk_wg_per_cu']} per CU",
+                "packing": "on" if tuning["packing"] else "off",
                 "steps_per_launch": round(steps_per_launch, 2),
                 "particles_per_launch": narrow,
                 "alg_bytes_per_particle_step": round(bpps, 1), "alg_bytes_per_launch": int(bytes_launch),
@@ -603,7 +612,8 @@ def main():
     if not args.only_batched:
         log("FP64 pool")
         pool, el_64, prof64, kbar64, cbar64, tuning64, narrow64 = run_pool(L, L.PRECISION_FP64, sizes, xs, vs,
-                                                                           args.steps, args.warmup, split, chunk_shape)
+                                                                           args.steps, args.warmup, split, chunk_shape,
+                                                                           not args.no_pack)
         pool.close()
         c64 = max(prof64["chunks"], 1)
         us64 = prof64["chunk_ms"] * 1e3 / c64

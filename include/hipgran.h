@@ -269,15 +269,18 @@ hg_status hg_breaking_sinter_topology(size_t n, const double *x0, double r_part,
  * pair mode, precision, dt) in one device arena of max_particles particles.
  * hg_pool_step advances every active system by the same number of steps with
  * one persistent kernel launch per rebuild interval: each workgroup takes one
- * whole system at a time (largest first) and runs all of its steps with
- * workgroup barriers between the kick/drift and force phases. A system's
+ * whole system at a time (largest first), or in neighbour-list mode a pack of
+ * small systems that together have at most one particle per thread, and runs
+ * all of their steps with workgroup barriers between the kick/drift and force
+ * phases. A system's
  * results do not depend on which other systems share the pool. Systems can be
  * added and removed between steps (continuous batching); a system's
  * per-step behaviour is that of an hg_system with the same params.
  * ------------------------------------------------------------------------ */
 typedef struct hg_pool hg_pool;
 
-/* max_facets: facets per system (0..16). */
+/* max_particles < 2^31, max_systems < 2^24, max_facets: facets per system
+ * (0..16). */
 hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t max_systems, size_t max_facets,
                          hg_pool **out);
 hg_status hg_pool_destroy(hg_pool *pool);
@@ -373,6 +376,11 @@ hg_status hg_pool_set_wide_cus(hg_pool *pool, int wide_cus);
  * HIP graph per (interval length, dt, wide set) and replayed; enable = 0
  * launches them one by one instead. Default on. */
 hg_status hg_pool_set_wide_graph(hg_pool *pool, int enable);
+/* Packing (neighbour-list pools, default on): the chunk kernel steps small
+ * systems side by side in one workgroup, one particle per thread, instead of
+ * one system per workgroup. Results are the same either way; enable = 0
+ * gives every system a workgroup of its own. */
+hg_status hg_pool_set_packing(hg_pool *pool, int enable);
 /* Online autotuner: from the next hg_pool_step on, first the chunk kernel's
  * shape (256 threads x 2 workgroups per CU, or 512 x 1) at the current launch
  * split, then the launch split at the better shape (wide threshold 0, or

@@ -493,6 +493,11 @@ class Pool:
         """Replay each interval's wide launches as a HIP graph (default) or not."""
         _abi.check(_abi.lib().hg_pool_set_wide_graph(self._h, 1 if enable else 0))
 
+    def set_packing(self, enable):
+        """Step small systems side by side in one chunk workgroup (default) or
+        give each its own workgroup; results are the same."""
+        _abi.check(_abi.lib().hg_pool_set_packing(self._h, 1 if enable else 0))
+
     def autotune(self, enable=True):
         """Tune the wide threshold and room on the next live intervals."""
         _abi.check(_abi.lib().hg_pool_autotune(self._h, 1 if enable else 0))

@@ -144,6 +144,7 @@ SIGNATURES = {
     "hg_pool_last_busy": (C.c_int, [_sysp, _dp]),
     "hg_pool_set_wide_cus": (C.c_int, [_sysp, C.c_int]),
     "hg_pool_set_wide_graph": (C.c_int, [_sysp, C.c_int]),
+    "hg_pool_set_packing": (C.c_int, [_sysp, C.c_int]),
     "hg_pool_autotune": (C.c_int, [_sysp, C.c_int]),
     "hg_pool_get_tuning": (C.c_int, [_sysp, _szp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hg_server_create": (C.c_int, [C.POINTER(Params), C.POINTER(ServerConfig), C.POINTER(_sysp)]),

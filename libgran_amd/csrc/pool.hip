@@ -400,56 +400,167 @@ __device__ __forceinline__ void run_system(const hg_params &P, const mixed_const
     __syncthreads();
 }
 
-// One rebuild interval of every queued system. Exit: every workgroup leaves
-// once the shared counter passes `count`. lds_bytes of dynamic LDS per
-// workgroup decide each system's state placement. FAC = false leaves the
-// facet fold out (every queued system has nf = 0): it holds the kernel's
-// largest register set. TPB threads per workgroup (256 or 512): a 512-thread
-// workgroup alone on a CU has the same waves per SIMD as two of 256 and
-// twice the LDS for one system.
+// One system's interval by the whole workgroup, in the state tier its size
+// allows (s_red: block_max_u32's scratch words).
+template <int BIN, int HAM, int PAIRS, int PREC, bool FAC>
+__device__ __forceinline__ void run_one(const hg_params &P, const mixed_consts &MC, const pool_view &V,
+                                        uint32_t slot, uint32_t nsteps, double dt, uint32_t lds_bytes,
+                                        uint32_t *s_red) {
+    const pool_meta M = V.meta[slot];
+    facet_dev *F = V.facets + size_t(slot) * V.max_facets;
+    const size_t need = size_t(M.n) * sizeof(double);
+    if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
+        // the partner ids go to LDS too wherever they fit beside the state
+        uint32_t kmax = 0;
+        for (uint32_t t = threadIdx.x; t < M.n; t += blockDim.x) kmax = max(kmax, V.ncnt[M.offset + t]);
+        kmax = block_max_u32(kmax, s_red);
+        const size_t ids = M.n <= 65535u ? lds_ids_bytes(M.n, kmax) : ~size_t(0) / 2;
+        if (need * kFullStride + ids <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
+        else if (need * kMirrorStride + ids <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
+        else if (need * kPosStride + ids <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_POS, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
+        else if (ids <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
+        else
+            run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
+    } else {
+        if (need * kFullStride <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
+        else if (need * kMirrorStride <= lds_bytes)
+            run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
+        else
+            run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
+    }
+}
+
+// Several small systems (a pack: count members, all of them at most one
+// particle per thread together) stepped side by side by one workgroup, so
+// that a system of 100 particles does not leave 400 lanes idle through every
+// step. Neighbour-list mode, all state in LDS: member m's particles take pack
+// positions base_m .. base_m + n_m - 1 (thread = position). Each thread steps
+// its own particle with the single-system device code, given its member's
+// meta with offset_m - base_m in place of the offset: LDS records and partner
+// ids are addressed by (arena index - that offset) = pack position (unsigned
+// arithmetic), and the ids are stored as pack positions. Every operation of a
+// particle is that of run_system, so a system's result does not depend on
+// packing. Returns false, having done nothing, when the pack needs more
+// threads or LDS than the workgroup has; the caller then steps the members
+// one at a time.
+constexpr uint32_t kMaxPack = 8;
+template <int BIN, int HAM, int PREC, bool FAC>
+__device__ __forceinline__ bool run_pack(const hg_params &P, const mixed_consts &MC, const pool_view &V,
+                                         const uint32_t *slots, uint32_t cnt, uint32_t nsteps, double dt,
+                                         uint32_t lds_bytes, uint32_t *s_red, pool_meta *s_pm, uint32_t *s_base) {
+    if (threadIdx.x == 0) {
+        uint32_t base = 0;
+        for (uint32_t m = 0; m < cnt; m++) {
+            s_pm[m] = V.meta[slots[m]];
+            s_base[m] = base;
+            base += s_pm[m].n;
+        }
+        s_base[cnt] = base;
+    }
+    __syncthreads();
+    const uint32_t N = s_base[cnt], t = threadIdx.x;
+    uint32_t m = 0;
+    bool all_valid = true;
+    for (uint32_t k = 0; k < cnt; k++) {
+        if (k > 0 && t >= s_base[k]) m = k;
+        all_valid = all_valid && s_pm[k].acc_valid;
+    }
+    const bool own = t < N;
+    const pool_meta Pm = s_pm[m];
+    const uint32_t g = Pm.offset + (t - s_base[m]);
+    pool_meta Mv = Pm;
+    Mv.offset = Pm.offset - s_base[m];
+    Mv.n = N;
+    uint32_t kmax = own ? V.ncnt[g] : 0u;
+    kmax = block_max_u32(kmax, s_red);
+    if (N > blockDim.x || size_t(N) * sizeof(double) * kFullStride + lds_ids_bytes(N, kmax) > lds_bytes) return false;
+    facet_dev *F = V.facets + size_t(Pm.slot) * V.max_facets;
+    uint16_t *lids = reinterpret_cast<uint16_t *>(pool_lds + size_t(N) * kFullStride);
+    if (own) {
+        double *q = pool_lds + size_t(t) * kFullStride;
+        sds3(q, ld3(V.x, g));
+        sds3(q + 3, ld3(V.v, g));
+        sds3(q + 6, ld3(V.w, g));
+        sds3(q + 9, ld3(V.th, g));
+        sds3(q + 12, ld3(V.a, g));
+        sds3(q + 15, ld3(V.al, g));
+        const uint32_t c = V.ncnt[g];
+        lids[kmax * N + t] = uint16_t(c);
+        for (uint32_t k = 0; k < c; k++) lids[k * N + t] = uint16_t(V.nb[size_t(k) * V.stride + g] - Mv.offset);
+    }
+    __syncthreads();
+    const double half = dt / 2.0;
+    uint32_t cur = Pm.cur;
+    auto advance_all = [&]() {
+        for (uint32_t k = 0; k < cnt; k++)
+            advance_facets(V.facets + size_t(s_pm[k].slot) * V.max_facets, int(s_pm[k].nf), dt);
+    };
+    for (int s = all_valid ? 0 : -1; s < int(nsteps); s++) {
+        if (s >= 0) {
+            if (own) pool_kick_drift<STATE_FULL>(V, Mv, s > 0, half, dt, g, g + 1, 1);
+            if (s > 0 && t == 0) advance_all();
+            __syncthreads();
+        }
+        if (own && (s >= 0 || !Pm.acc_valid)) {
+            pool_force_phase<BIN, HAM, HG_PAIRS_NEIGHBOR_LIST, PREC, STATE_FULL, true, FAC>(P, MC, V, Mv, cur, F, g,
+                                                                                           g + 1, 1, lids, kmax);
+            cur ^= 1u;
+        }
+        __syncthreads();
+    }
+    if (nsteps > 0) {
+        if (own) pool_final_kick<STATE_FULL>(V, Mv, half, g, g + 1, 1);
+        if (t == 0) advance_all();
+    } else if (own) {
+        const double *q = pool_lds + size_t(t) * kFullStride;
+        st3(V.a, g, lds3(q + 12));
+        st3(V.al, g, lds3(q + 15));
+    }
+    __syncthreads();
+    return true;
+}
+
+// One rebuild interval of every queued system. The queue holds packs
+// (packs[i] = first | count << 24 over the slots in `order`): a pack of one
+// is a system stepped by the whole workgroup, a larger one small systems
+// stepped side by side (run_pack). Exit: every workgroup leaves once the
+// shared counter passes `count`. lds_bytes of dynamic LDS per workgroup
+// decide each system's state placement. FAC = false leaves the facet fold
+// out (every queued system has nf = 0): it holds the kernel's largest
+// register set. TPB threads per workgroup (256 or 512): a 512-thread
+// workgroup alone on a CU has the same waves per SIMD as two of 256 and twice
+// the LDS for one system.
 template <int BIN, int HAM, int PAIRS, int PREC, bool FAC, int TPB>
 // The model constants come by value: the scalar registers they spill to are
 // re-read with v_readlane, which measured faster than reloading them through
 // a pointer inside the pair loop (scalar loads add memory latency there).
-__global__ void __launch_bounds__(TPB, HG_CHUNK_MIN_BLOCKS) k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order,
-                                                    uint32_t count, uint32_t *counter, uint32_t nsteps, double dt,
-                                                    uint32_t lds_bytes) {
+__global__ void __launch_bounds__(TPB, HG_CHUNK_MIN_BLOCKS)
+    k_pool_chunk(hg_params P, mixed_consts MC, pool_view V, const uint32_t *order, const uint32_t *packs,
+                 uint32_t count, uint32_t *counter, uint32_t nsteps, double dt, uint32_t lds_bytes) {
     __shared__ uint32_t s_item;
     __shared__ uint32_t s_red[TPB / 64];
+    __shared__ pool_meta s_pm[kMaxPack];
+    __shared__ uint32_t s_base[kMaxPack + 1];
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t item = s_item;
         __syncthreads();
         if (item >= count) break;
-        const uint32_t slot = order[item];
-        const pool_meta M = V.meta[slot];
-        facet_dev *F = V.facets + size_t(slot) * V.max_facets;
-        const size_t need = size_t(M.n) * sizeof(double);
+        const uint32_t pk = packs[item];
+        const uint32_t first = pk & 0xFFFFFFu, cnt = pk >> 24;
         if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-            // the partner ids go to LDS too wherever they fit beside the state
-            uint32_t kmax = 0;
-            for (uint32_t t = threadIdx.x; t < M.n; t += blockDim.x) kmax = max(kmax, V.ncnt[M.offset + t]);
-            kmax = block_max_u32(kmax, s_red);
-            const size_t ids = M.n <= 65535u ? lds_ids_bytes(M.n, kmax) : ~size_t(0) / 2;
-            if (need * kFullStride + ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
-            else if (need * kMirrorStride + ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
-            else if (need * kPosStride + ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_POS, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
-            else if (ids <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, true, FAC>(P, MC, V, M, F, nsteps, dt, kmax);
-            else
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
-        } else {
-            if (need * kFullStride <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_FULL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
-            else if (need * kMirrorStride <= lds_bytes)
-                run_system<BIN, HAM, PAIRS, PREC, STATE_MIRROR, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
-            else
-                run_system<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, FAC>(P, MC, V, M, F, nsteps, dt, 0);
+            if (cnt > 1 &&
+                run_pack<BIN, HAM, PREC, FAC>(P, MC, V, order + first, cnt, nsteps, dt, lds_bytes, s_red, s_pm, s_base))
+                continue;
         }
+        for (uint32_t k = 0; k < cnt; k++)
+            run_one<BIN, HAM, PAIRS, PREC, FAC>(P, MC, V, order[first + k], nsteps, dt, lds_bytes, s_red);
     }
 }
 
@@ -669,7 +780,8 @@ __global__ void k_zero_rows_f64(double *p, size_t stride, size_t offset, size_t 
 inline unsigned nblk(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
 
 template <int BIN, int HAM, int PAIRS, int PREC, bool FAC, int TPB>
-hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t *order, const uint32_t *packs,
+                          uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
                           hipStream_t s) {
     const size_t lds_bytes = lds_cap;
@@ -680,18 +792,20 @@ hipError_t launch_chunk_k(const hg_params &p, const pool_view &V, const uint32_t
         if (e != hipSuccess) return e;
     }
     k_pool_chunk<BIN, HAM, PAIRS, PREC, FAC, TPB><<<grid, TPB, lds_bytes, s>>>(p, make_mixed_consts(p), V, order,
-                                                                                count, counter, nsteps, dt, lds_cap);
+                                                                                packs, count, counter, nsteps, dt,
+                                                                                lds_cap);
     return hipGetLastError();
 }
 
 template <int BIN, int HAM, int PAIRS>
-hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t *order, const uint32_t *packs,
+                          uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
                           bool facets, int tpb, hipStream_t s) {
 #define HG_KCASE(PREC, FAC, TPB)                                                                                \
     if (p.precision == PREC && facets == FAC && tpb == TPB)                                                    \
-        return launch_chunk_k<BIN, HAM, PAIRS, PREC, FAC, TPB>(p, V, order, count, counter, nsteps, dt, grid, \
-                                                                lds_cap, s);
+        return launch_chunk_k<BIN, HAM, PAIRS, PREC, FAC, TPB>(p, V, order, packs, count, counter, nsteps, dt, \
+                                                                grid, lds_cap, s);
     HG_KCASE(HG_PRECISION_MIXED, true, 256)
     HG_KCASE(HG_PRECISION_MIXED, false, 256)
     HG_KCASE(HG_PRECISION_FP64, true, 256)
@@ -707,12 +821,14 @@ hipError_t launch_chunk_t(const hg_params &p, const pool_view &V, const uint32_t
 // facets: whether any queued system has facets (else the kernel without the
 // facet fold runs); tpb: threads per workgroup (256 or 512).
 template <int PAIRS>
-hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, uint32_t count,
+hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t *order, const uint32_t *packs,
+                          uint32_t count,
                           uint32_t *counter, uint32_t nsteps, double dt, unsigned grid, uint32_t lds_cap,
                           bool facets, int tpb, hipStream_t s) {
 #define HG_PCASE(B, H) \
     if (p.binary_model == B && p.hamaker_model == H) \
-        return launch_chunk_t<B, H, PAIRS>(p, V, order, count, counter, nsteps, dt, grid, lds_cap, facets, tpb, s);
+        return launch_chunk_t<B, H, PAIRS>(p, V, order, packs, count, counter, nsteps, dt, grid, lds_cap, facets, \
+                                           tpb, s);
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_PCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
@@ -796,6 +912,9 @@ struct hg_pool {
     uint32_t *order_d = nullptr, *counter_d = nullptr;
     std::vector<uint32_t> order;   // every active system, largest first
     uint32_t narrow_count = 0;     // systems below wide_min, uploaded to order_d
+    uint32_t *packs_d = nullptr;   // the chunk kernel's queue: first | count << 24 over order_d
+    uint32_t pack_count = 0;
+    bool packing = true;           // small systems share a workgroup (neighbour-list mode)
     uint32_t wide_min = 0;         // systems with n >= wide_min use the wide launches (0: none)
     wide_block *wide_map_d = nullptr;
     size_t wide_map_cap = 0;
@@ -1065,8 +1184,9 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
                          hg_pool **out) {
     if (!out || !params) return fail(HG_ERR_INVALID, "hg_pool_create: params and out are required");
     *out = nullptr;
-    if (max_particles == 0 || max_particles >= (size_t(1) << 31) || max_systems == 0)
-        return fail(HG_ERR_INVALID, "hg_pool_create: need 0 < max_particles < 2^31 and max_systems > 0");
+    if (max_particles == 0 || max_particles >= (size_t(1) << 31) || max_systems == 0 ||
+        max_systems >= (size_t(1) << 24))
+        return fail(HG_ERR_INVALID, "hg_pool_create: need 0 < max_particles < 2^31 and 0 < max_systems < 2^24");
     if (max_facets > size_t(kMaxFacets)) return fail(HG_ERR_INVALID, "hg_pool_create: at most 16 facets per system");
     if (params->precision != HG_PRECISION_FP64 && params->precision != HG_PRECISION_MIXED)
         return fail(HG_ERR_INVALID, "hg_pool_create: unknown precision");
@@ -1164,6 +1284,7 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     if (Q->MF) PG_TRYC(palloc(Q, &Q->ss, Q->MF * 9 * P));
     PG_TRYC(palloc(Q, &Q->meta_d, Q->S));
     PG_TRYC(palloc(Q, &Q->order_d, Q->S));
+    PG_TRYC(palloc(Q, &Q->packs_d, Q->S));
     Q->wide_map_cap = Q->P / kWideBlock + Q->S + 1;
     PG_TRYC(palloc(Q, &Q->wide_map_d, Q->wide_map_cap));
     PG_TRYC(palloc(Q, &Q->counter_d, 1));
@@ -1505,9 +1626,30 @@ hg_status refresh_order(hg_pool *Q) {
     if (map.size() > Q->wide_map_cap) return fail(HG_ERR_INVALID, "hg_pool_step: wide map overflow");
     Q->narrow_count = uint32_t(Q->order.size() - first_narrow);
     Q->wide_blocks = unsigned(map.size());
-    if (Q->narrow_count)
-        HG_HIP(hipMemcpyAsync(Q->order_d, Q->order.data() + first_narrow, size_t(Q->narrow_count) * 4,
-                              hipMemcpyHostToDevice, Q->stream));
+    // packs, largest first: a system of at least a workgroup's threads goes
+    // alone; a smaller one takes the smallest remaining systems beside it
+    // while their particles fit one thread each (neighbour-list mode)
+    const bool pack = Q->packing && Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
+    const uint32_t tpb = uint32_t(Q->chunk_tpb);
+    std::vector<uint32_t> slots, packs;
+    slots.reserve(Q->narrow_count);
+    for (size_t lo = first_narrow, hi = Q->order.size(); lo < hi;) {
+        const uint32_t first = uint32_t(slots.size());
+        uint32_t s0 = Q->order[lo++], tot = Q->meta[s0].n, cnt = 1;
+        slots.push_back(s0);
+        while (pack && lo < hi && cnt < kMaxPack && tot + Q->meta[Q->order[hi - 1]].n <= tpb) {
+            const uint32_t s1 = Q->order[--hi];
+            slots.push_back(s1);
+            tot += Q->meta[s1].n;
+            cnt++;
+        }
+        packs.push_back(first | (cnt << 24));
+    }
+    Q->pack_count = uint32_t(packs.size());
+    if (Q->narrow_count) {
+        HG_HIP(hipMemcpyAsync(Q->order_d, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, Q->stream));
+        HG_HIP(hipMemcpyAsync(Q->packs_d, packs.data(), packs.size() * 4, hipMemcpyHostToDevice, Q->stream));
+    }
     if (!map.empty())
         HG_HIP(hipMemcpyAsync(Q->wide_map_d, map.data(), map.size() * sizeof(wide_block), hipMemcpyHostToDevice,
                               Q->stream));
@@ -1560,6 +1702,7 @@ void tune_apply(hg_pool *Q, const hg_pool::tune_cand &c) {
         Q->order_dirty = true;
     }
     Q->wide_cus = c.wide_cus;
+    if (Q->chunk_tpb != c.tpb) Q->order_dirty = true;  // the packs follow the threads per workgroup
     Q->chunk_tpb = c.tpb;
     set_chunk_wgs(Q, c.wgs);
 }
@@ -1603,7 +1746,7 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
         for (uint32_t s : Q->order) len64 = std::min<uint64_t>(len64, rebuild_every - Q->step_index[s] % rebuild_every);
     const uint32_t len = uint32_t(len64);
     const uint32_t count = Q->narrow_count;
-    const unsigned grid = unsigned(std::min<uint64_t>(count, uint64_t(Q->grid)));
+    const unsigned grid = unsigned(std::min<uint64_t>(Q->pack_count, uint64_t(Q->grid)));
     bool any_wide_initial = false;
     for (size_t k = 0; k < Q->order.size() - count; k++) any_wide_initial |= !Q->meta[Q->order[k]].acc_valid;
     const auto t_interval = std::chrono::steady_clock::now();
@@ -1693,10 +1836,11 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
         HG_HIP(hipEventRecord(Q->ev[2], sc));
         bool facets = false;
         for (size_t k = Q->order.size() - count; k < Q->order.size(); k++) facets |= Q->meta[Q->order[k]].nf > 0;
-        hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, count, Q->counter_d, len,
-                                                                   dt, g, Q->lds_cap, facets, Q->chunk_tpb, sc)
-                          : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, count, Q->counter_d, len, dt, g,
-                                                         Q->lds_cap, facets, Q->chunk_tpb, sc);
+        hipError_t e = nl ? launch_chunk_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, V, Q->order_d, Q->packs_d, Q->pack_count,
+                                                                   Q->counter_d, len, dt, g, Q->lds_cap, facets,
+                                                                   Q->chunk_tpb, sc)
+                          : launch_chunk_p<HG_PAIRS_ALL>(Q->p, V, Q->order_d, Q->packs_d, Q->pack_count, Q->counter_d,
+                                                         len, dt, g, Q->lds_cap, facets, Q->chunk_tpb, sc);
         HG_HIP(e);
         HG_HIP(hipEventRecord(Q->ev[3], sc));
     }
@@ -1849,11 +1993,19 @@ hg_status hg_pool_set_wide_graph(hg_pool *Q, int enable) {
     return HG_OK;
 }
 
+hg_status hg_pool_set_packing(hg_pool *Q, int enable) {
+    if (!Q) return fail(HG_ERR_INVALID, "pool is null");
+    if (Q->packing != (enable != 0)) Q->order_dirty = true;
+    Q->packing = enable != 0;
+    return HG_OK;
+}
+
 hg_status hg_pool_set_chunk_shape(hg_pool *Q, int threads, int wg_per_cu) {
     if (!Q) return fail(HG_ERR_INVALID, "pool is null");
     if (threads != 256 && threads != 512) return fail(HG_ERR_INVALID, "hg_pool_set_chunk_shape: threads 256 or 512");
     if (wg_per_cu < 1 || wg_per_cu > 2048 / threads)
         return fail(HG_ERR_INVALID, "hg_pool_set_chunk_shape: wg_per_cu 1.." + std::to_string(2048 / threads));
+    if (Q->chunk_tpb != threads) Q->order_dirty = true;
     Q->chunk_tpb = threads;
     set_chunk_wgs(Q, wg_per_cu);
     return HG_OK;

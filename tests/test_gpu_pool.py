@@ -145,6 +145,38 @@ def test_pool_result_alone_equals_in_mix(precision):
     same(alone.state(sa), mix.state(sm), "alone vs mix")
 
 
+@pytest.mark.parametrize("shape", [(512, 1), (256, 2), (256, 8)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pool_packed_small_systems_equal_alone(precision, shape):
+    """Small systems stepped side by side in one chunk workgroup (a pack) give
+    the results each gives alone, with packing on and off, over whole and
+    partial rebuild intervals. At 256 x 8 a workgroup's LDS share is too small
+    for a pack's state, so its members are stepped one at a time."""
+    announce(f"packed small systems, precision {precision}, shape {shape}")
+    L = _L()
+    sizes = (50, 64, 90, 120, 150, 200, 260, 333, 75, 410)
+    cases = [agg_case(n, 70 + k) for k, n in enumerate(sizes)]
+    alone = []
+    for x, v0 in cases:
+        P = L.Pool(nl_params(precision), max_particles=1000, max_systems=1)
+        s = P.add(x, v0)
+        P.step(DT, 120, rebuild_every=50)
+        alone.append(P.state(s))
+    if precision == 0:
+        for k in (0, 7):
+            o = oracle_nl(*cases[k])
+            o.step(DT, 120, rebuild_every=50)
+            same(o.state(), alone[k], f"oracle vs alone, n {sizes[k]}")
+    for packing in (True, False):
+        mix = L.Pool(nl_params(precision), max_particles=5000, max_systems=16)
+        mix.set_chunk_shape(*shape)
+        mix.set_packing(packing)
+        slots = [mix.add(x, v0) for x, v0 in cases]
+        mix.step(DT, 120, rebuild_every=50)
+        for k, s in enumerate(slots):
+            same(alone[k], mix.state(s), f"packing {packing}, n {sizes[k]}")
+
+
 def test_pool_continuous_admission_and_retirement():
     """Systems join and leave between steps; each result equals its own
     alone-run with the same number of steps."""
