@@ -675,21 +675,30 @@ __global__ void __launch_bounds__(256) k_pool_cell_count(const double *x, const 
     atomicAdd(&bucket_count[h], 1u);
 }
 
-__global__ void __launch_bounds__(256) k_pool_cell_fill(uint32_t hw, const uint32_t *cell_of,
-                                                        const uint32_t *bucket_start, uint32_t *bucket_fill,
-                                                        uint32_t *cell_items) {
+// A binned particle as the list fill reads it: its position, index and
+// system in one 32-byte record, so that a bucket's candidates are one
+// contiguous read each rather than an index followed by two gathers.
+struct cell_rec {
+    double x, y, z;
+    uint32_t j, slot;
+};
+
+__global__ void __launch_bounds__(256) k_pool_cell_fill(const double *x, const uint32_t *owner, uint32_t hw,
+                                                        const uint32_t *cell_of, const uint32_t *bucket_start,
+                                                        uint32_t *bucket_fill, cell_rec *recs) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= hw) return;
     const uint32_t h = cell_of[g];
     if (h == kNoEntry) return;
-    cell_items[bucket_start[h] + atomicAdd(&bucket_fill[h], 1u)] = g;
+    const d3 p = ld3(x, g);
+    recs[bucket_start[h] + atomicAdd(&bucket_fill[h], 1u)] = cell_rec{p.x, p.y, p.z, g, owner[g]};
 }
 
 constexpr uint32_t kListRegs = 16;
 __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uint32_t *owner, const uint32_t *cell_of,
                                                       uint32_t hw, double cs, uint32_t mask, double r_verlet,
                                                       const uint32_t *bucket_count, const uint32_t *bucket_start,
-                                                      const uint32_t *cell_items, uint32_t *nb, uint32_t *ncnt,
+                                                      const cell_rec *recs, uint32_t *nb, uint32_t *ncnt,
                                                       size_t stride, uint32_t cap, int *flags, int *sflags) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= hw || cell_of[g] == kNoEntry) return;
@@ -737,12 +746,21 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
         }
         cnt++;
     };
-    // the 27 cells unrolled, so the hashes already visited (a bucket shared
-    // by two cells is scanned once) sit in registers too
-    uint32_t seen[27];
+    // Cells are twice the list radius wide, so a partner lies in the
+    // particle's own cell or the neighbour on the side of the half the
+    // particle sits in, along each axis: 2 x 2 x 2 cells. (In cell units a
+    // partner is less than 0.5 (1 - 1e-9) away; a particle in the lower half,
+    // u - floor(u) < 0.5, has its partners in floor(u) - 1 .. floor(u), one in
+    // the upper half in floor(u) .. floor(u) + 1.) The cells are unrolled, so
+    // the hashes already visited (a bucket shared by two cells is scanned
+    // once) sit in registers too.
+    const long long bx = cx - (xi.x / cs - double(cx) < 0.5 ? 1 : 0);
+    const long long by = cy - (xi.y / cs - double(cy) < 0.5 ? 1 : 0);
+    const long long bz = cz - (xi.z / cs - double(cz) < 0.5 ? 1 : 0);
+    uint32_t seen[8];
 #pragma unroll
-    for (int c = 0; c < 27; c++) {
-        const uint32_t h = pool_hash(slot, cx + (c % 3 - 1), cy + (c / 3 % 3 - 1), cz + (c / 9 - 1), mask);
+    for (int c = 0; c < 8; c++) {
+        const uint32_t h = pool_hash(slot, bx + (c & 1), by + (c >> 1 & 1), bz + (c >> 2), mask);
         seen[c] = h;
         bool dup = false;
 #pragma unroll
@@ -750,9 +768,10 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
         if (dup) continue;
         const uint32_t b = bucket_start[h], e = b + bucket_count[h];
         for (uint32_t q = b; q < e; q++) {
-            const uint32_t j = cell_items[q];
-            if (j == g || owner[j] != slot) continue;
-            const d3 d = xi - ld3(x, j);
+            const cell_rec c = recs[q];
+            const uint32_t j = c.j;
+            if (j == g || c.slot != slot) continue;
+            const d3 d = xi - d3{c.x, c.y, c.z};
             if (!(dot(d, d) < d2_max)) continue;
             if (norm(d) < r_verlet) insert(j);
         }
@@ -945,7 +964,8 @@ struct hg_pool {
     bool order_dirty = true;
     bool meta_dirty = true;
     uint32_t hash_size = 0, *cell_of = nullptr, *bucket_count = nullptr, *bucket_start = nullptr,
-             *bucket_fill = nullptr, *cell_items = nullptr, *scan_tmp = nullptr;
+             *bucket_fill = nullptr, *scan_tmp = nullptr;
+    cell_rec *cell_recs = nullptr;
     int *flags = nullptr, *flags_host = nullptr;
     // per-system fault words set by the kernels (device, cleared after every
     // read) and the faults recorded so far (host, per slot)
@@ -1100,7 +1120,7 @@ hg_status download_rows(hg_pool *Q, const T *dev, uint32_t offset, uint32_t n, u
 hg_status rebuild_due(hg_pool *Q) {
     const uint32_t hw = Q->high_water;
     if (hw == 0) return HG_OK;
-    const double cs = Q->p.r_verlet * (1.0 + 1e-9);
+    const double cs = 2.0 * Q->p.r_verlet * (1.0 + 1e-9);  // two list radii: 2 x 2 x 2 cells per particle
     const uint32_t mask = Q->hash_size - 1;
     HG_HIP(launch_zero_u32(Q->bucket_count, Q->hash_size, Q->stream));
     HG_HIP(launch_zero_u32(Q->bucket_fill, Q->hash_size, Q->stream));
@@ -1108,11 +1128,11 @@ hg_status rebuild_due(hg_pool *Q) {
                                                            Q->bucket_count);
     HG_HIP(hipGetLastError());
     HG_HIP(launch_exclusive_scan(Q->bucket_count, Q->bucket_start, Q->hash_size, Q->scan_tmp, Q->stream));
-    k_pool_cell_fill<<<nblk(hw, 256), 256, 0, Q->stream>>>(hw, Q->cell_of, Q->bucket_start, Q->bucket_fill,
-                                                          Q->cell_items);
+    k_pool_cell_fill<<<nblk(hw, 256), 256, 0, Q->stream>>>(Q->x, Q->owner, hw, Q->cell_of, Q->bucket_start,
+                                                          Q->bucket_fill, Q->cell_recs);
     HG_HIP(hipGetLastError());
     k_pool_nl_fill<<<nblk(hw, 64), 64, 0, Q->stream>>>(Q->x, Q->owner, Q->cell_of, hw, cs, mask, Q->p.r_verlet,
-                                                      Q->bucket_count, Q->bucket_start, Q->cell_items, Q->nb,
+                                                      Q->bucket_count, Q->bucket_start, Q->cell_recs, Q->nb,
                                                       Q->ncnt, Q->P, Q->cap_n, Q->flags, Q->sflags);
     HG_HIP(hipGetLastError());
     return HG_OK;
@@ -1176,7 +1196,7 @@ size_t hg_pool_bytes_per_particle(const hg_params *p, size_t max_facets) {
     b += 4;                              // owner
     if (p->binary_model == HG_BINARY_SINTER && p->sinter_breaking)
         b += kMaxBondsPerParticle * 12 + 4;  // neck-strength lists
-    if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST) b += 4 + 4 + 3 * 2 * 4 + 4;  // cell list scratch
+    if (p->pair_mode == HG_PAIRS_NEIGHBOR_LIST) b += 4 + 32 + 3 * 2 * 4 + 4;  // cell list scratch
     return b;
 }
 
@@ -1256,7 +1276,7 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
         while (hs < 2 * P && hs < (1u << 30)) hs <<= 1;
         Q->hash_size = hs;
         PG_TRYC(palloc(Q, &Q->cell_of, P));
-        PG_TRYC(palloc(Q, &Q->cell_items, P));
+        PG_TRYC(palloc(Q, &Q->cell_recs, P));
         PG_TRYC(palloc(Q, &Q->bucket_count, hs));
         PG_TRYC(palloc(Q, &Q->bucket_start, hs));
         PG_TRYC(palloc(Q, &Q->bucket_fill, hs));
