@@ -5,7 +5,9 @@
 // test), the facet closest point, and the spring vectors: their rotation into
 // the contact plane, rescaling and the accumulation xi += v dt. In FP32 with
 // a*b+c contraction allowed: the normal, relative velocities, normal force,
-// Coulomb test and cap, torques and the Hamaker scalar. The gap h must stay
+// Coulomb test and cap, torques and the Hamaker scalar. Dot and cross
+// products and a*x+y vector updates, FP64 and FP32, are explicit fused
+// multiply-adds (one rounding where the FP64 path has two). The gap h must stay
 // FP64: h = |d| - 2r near h0 = 0.1 nm loses eps_32 * 2r/h0 ~ 1.7e-5 of
 // relative precision in FP32. Accelerations are summed in FP32 per particle.
 //
@@ -32,6 +34,29 @@ HGD f3 crossf(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z
 HGD f3 tof(d3 a) { return {float(a.x), float(a.y), float(a.z)}; }
 HGD d3 tod(f3 a) { return {double(a.x), double(a.y), double(a.z)}; }
 HGD f3 zerof() { return {0.f, 0.f, 0.f}; }
+
+// Fused forms for this path: each product folded into the following sum is
+// one explicit fused multiply-add, so the rounding is fixed by the source and
+// is the same in every kernel that inlines it (alone or packed, chunk or wide,
+// pool or single system), and the vector code issues one instruction where a
+// multiply and an add were two.
+HGD double dotm(d3 a, d3 b) { return __builtin_fma(a.z, b.z, __builtin_fma(a.y, b.y, a.x * b.x)); }
+HGD d3 crossm(d3 a, d3 b) {
+    return {__builtin_fma(a.y, b.z, -(a.z * b.y)), __builtin_fma(a.z, b.x, -(a.x * b.z)),
+            __builtin_fma(a.x, b.y, -(a.y * b.x))};
+}
+// s a + b
+HGD d3 fmam(double s, d3 a, d3 b) {
+    return {__builtin_fma(s, a.x, b.x), __builtin_fma(s, a.y, b.y), __builtin_fma(s, a.z, b.z)};
+}
+HGD float dotmf(f3 a, f3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
+HGD f3 crossmf(f3 a, f3 b) {
+    return {__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+            __builtin_fmaf(a.x, b.y, -(a.y * b.x))};
+}
+HGD f3 fmamf(float s, f3 a, f3 b) {
+    return {__builtin_fmaf(s, a.x, b.x), __builtin_fmaf(s, a.y, b.y), __builtin_fmaf(s, a.z, b.z)};
+}
 
 // Reciprocal square roots without the correctly rounded division/sqrt
 // expansions (this path is not bit-exact by design): the hardware estimates
@@ -121,10 +146,10 @@ inline mixed_consts make_mixed_consts(const hg_params &p) {
 // (x2 l2 stays a normal number for springs longer than 1e-77 m).
 __device__ __forceinline__ d3 rotate_spring(d3 xi, d3 n) {
 #pragma clang fp contract(fast)
-    d3 xr = xi - dot(xi, n) * n;
-    const double l2 = dot(xr, xr);
+    d3 xr = fmam(-dotm(xi, n), n, xi);
+    const double l2 = dotm(xr, xr);
     if (l2 > 0.0) {
-        const double x2 = dot(xi, xi);  // > 0 whenever l2 > 0
+        const double x2 = dotm(xi, xi);  // > 0 whenever l2 > 0
         xr = xr * (x2 * rsqrt64(x2 * l2));
     }
     return xr;
@@ -138,22 +163,22 @@ __device__ __forceinline__ f3 shear_contact_mx(d3 &xi, d3 n64, float k, float g,
                                                f3 rv, d3 rv64, double dt) {
 #pragma clang fp contract(fast)
     xi = rotate_spring(xi, n64);
-    const f3 f0 = (-k) * tof(xi) - g * rv;
+    const f3 f0 = fmamf(-k, tof(xi), (-g) * rv);
     const float sf = mus * fn, df = mud * fn;
-    const float q = dotf(f0, f0);
+    const float q = dotmf(f0, f0);
     const float iq = q > 0.f ? rsqrt32(q) : 0.f;
     const float f0n = q * iq;
     const f3 t = f0 * iq;
     float fsel;
     if (f0n <= sf) {
-        xi = xi + rv64 * dt;
+        xi = fmam(dt, rv64, xi);
         fsel = sf;
     } else {
         const double kd = double(k), gd = double(g);
-        const d3 f064 = (-kd) * xi - gd * rv64;
-        const double l2 = dot(f064, f064);
+        const d3 f064 = fmam(-kd, xi, (-gd) * rv64);
+        const double l2 = dotm(f064, f064);
         const d3 t64 = l2 > 0.0 ? f064 * rsqrt64(l2) : zero3();
-        xi = (-1.0 / kd) * (double(df) * t64 + gd * rv64);
+        xi = (-1.0 / kd) * fmam(double(df), t64, gd * rv64);
         fsel = df;
     }
     return ((f0n < fsel) ? f0n : fsel) * t;
@@ -163,8 +188,8 @@ __device__ __forceinline__ f3 shear_contact_mx(d3 &xi, d3 n64, float k, float g,
 __device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, d3 n64, float k, float g, f3 rv, d3 rv64, double dt) {
 #pragma clang fp contract(fast)
     xi = rotate_spring(xi, n64);
-    const f3 f0 = (-k) * tof(xi) - g * rv;
-    xi = xi + rv64 * dt;
+    const f3 f0 = fmamf(-k, tof(xi), (-g) * rv);
+    xi = fmam(dt, rv64, xi);
     return f0;
 }
 
@@ -172,7 +197,7 @@ __device__ __forceinline__ f3 shear_sinter_mx(d3 &xi, d3 n64, float k, float g, 
 __device__ __forceinline__ void pair_kinematics_mx(d3 xi, d3 xj, double &dist, d3 &n64, f3 &n) {
 #pragma clang fp contract(fast)
     const d3 d = xi - xj;
-    const double d2 = dot(d, d);
+    const double d2 = dotm(d, d);
     // coincident particles (d2 == 0) get a zero normal, as normalized() of a
     // zero vector gives in the reference (Eigen) and in the FP64 path
     const double inv = d2 > 0.0 ? rsqrt64(d2) : 0.0;
@@ -208,12 +233,12 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
     const double rppd = rd - 0.5 * ovd;
     const float r = float(rd), rpp = float(rppd);
     const d3 dv = vi - vj, dw = wi - wj, sw = wi + wj;
-    const float v_n = float(-dot(dv, n64));
+    const float v_n = float(-dotm(dv, n64));
     const float f_n = (BONDED ? m.s_k : m.c_k) * ov + (BONDED ? m.s_gn : m.c_gn) * v_n;
-    const d3 v_ij = dv + rppd * cross(n64, sw);
-    const d3 vt64 = v_ij - dot(v_ij, n64) * n64;
-    const d3 vr64 = (-rppd) * cross(n64, dw);
-    const d3 vo64 = (rd * dot(n64, dw)) * n64;
+    const d3 v_ij = fmam(rppd, crossm(n64, sw), dv);
+    const d3 vt64 = fmam(-dotm(v_ij, n64), n64, v_ij);
+    const d3 vr64 = (-rppd) * crossm(n64, dw);
+    const d3 vo64 = (rd * dotm(n64, dw)) * n64;
     const f3 v_t = tof(vt64), v_r = tof(vr64), v_o = tof(vo64);
     f3 f_t, f_r, f_o;
     if (BONDED) {
@@ -225,8 +250,8 @@ __device__ __forceinline__ accf bonded_or_contact_mx(const mixed_consts &m, doub
         f_r = shear_contact_mx(sp.s[1], n64, m.c_kr, m.c_gr, f_n, m.c_mur, m.c_mudr, v_r, vr64, m.c_dt);
         f_o = shear_contact_mx(sp.s[2], n64, m.c_ko, m.c_go, f_n, m.c_muo, m.c_mudo, v_o, vo64, m.c_dt);
     }
-    const f3 tau = (-rpp) * crossf(n, f_t) + r * crossf(n, f_r) + r * f_o;
-    return {(f_n * n + f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
+    const f3 tau = fmamf(-rpp, crossmf(n, f_t), fmamf(r, crossmf(n, f_r), r * f_o));
+    return {fmamf(f_n, n, f_t) * (BONDED ? m.s_im : m.c_im), tau * (BONDED ? m.s_ii : m.c_ii)};
 }
 
 // The Hamaker acceleration along n at gap h (h >= h0): the reference's

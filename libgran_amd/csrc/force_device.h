@@ -395,7 +395,7 @@ template <int BIN, int HAM, bool EAGER, class VW>
 __device__ __forceinline__ void pair_fold_mx(const mixed_consts &m, uint32_t j, d3 xi, d3 vi, d3 wi, d3 xj, VW &&vw,
                                              walker_t<EAGER> &W, accf &acc) {
     const d3 d = xi - xj;
-    const double d2 = dot(d, d);
+    const double d2 = dotm(d, d);
     bool near = !(d2 > m.far2);
     accf b{zerof(), zerof()};
     f3 h = zerof();
