@@ -100,6 +100,21 @@ struct mixed_consts {
     float h_r2f;
 };
 
+// Holds the FP64 constants of the pair path in vector registers for a pass
+// (an empty asm with a vector-register operand): kept as scalars, they are
+// spilled and re-read lane by lane at every use inside the pair loop.
+__device__ __forceinline__ void pin_vgpr(double &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ mixed_consts pinned(const mixed_consts &MC) {
+    mixed_consts m = MC;
+    pin_vgpr(m.far2);
+    pin_vgpr(m.h_cut2);
+    pin_vgpr(m.c_r2);
+    pin_vgpr(m.s_r2);
+    pin_vgpr(m.c_dt);
+    pin_vgpr(m.s_dt);
+    return m;
+}
+
 inline mixed_consts make_mixed_consts(const hg_params &p) {
     mixed_consts m;
     const hg_contact_params &c = p.contact;

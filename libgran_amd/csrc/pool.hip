@@ -273,8 +273,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
             store_acc<LDS>(V, M.offset, g, acc.lin, acc.ang);
         } else {
             accf acc{zerof(), zerof()};
+            const mixed_consts m = pinned(MC);
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_mx<BIN, HAM>(MC, j, xi, vi, wi, xj, vw, W, acc); });
+                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_mx<BIN, HAM>(m, j, xi, vi, wi, xj, vw, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
