@@ -696,6 +696,7 @@ __global__ void __launch_bounds__(256) k_pool_cell_fill(const double *x, const u
 }
 
 constexpr uint32_t kListRegs = 16;
+constexpr uint32_t kRecBatch = 4;
 __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uint32_t *owner, const uint32_t *cell_of,
                                                       uint32_t hw, double cs, uint32_t mask, double r_verlet,
                                                       const uint32_t *bucket_count, const uint32_t *bucket_start,
@@ -768,13 +769,21 @@ __global__ void __launch_bounds__(256) k_pool_nl_fill(const double *x, const uin
         for (int s = 0; s < c; s++) dup |= seen[s] == h;
         if (dup) continue;
         const uint32_t b = bucket_start[h], e = b + bucket_count[h];
-        for (uint32_t q = b; q < e; q++) {
-            const cell_rec c = recs[q];
-            const uint32_t j = c.j;
-            if (j == g || c.slot != slot) continue;
-            const d3 d = xi - d3{c.x, c.y, c.z};
-            if (!(dot(d, d) < d2_max)) continue;
-            if (norm(d) < r_verlet) insert(j);
+        // kRecBatch records loaded together (the last one again past the
+        // end), then tested in bucket order: the loads overlap one another
+        // instead of each waiting for the previous test
+        for (uint32_t q = b; q < e; q += kRecBatch) {
+            cell_rec c[kRecBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kRecBatch; k++) c[k] = recs[q + k < e ? q + k : e - 1];
+#pragma unroll
+            for (uint32_t k = 0; k < kRecBatch; k++) {
+                const uint32_t j = c[k].j;
+                if (q + k >= e || j == g || c[k].slot != slot) continue;
+                const d3 d = xi - d3{c[k].x, c[k].y, c[k].z};
+                if (!(dot(d, d) < d2_max)) continue;
+                if (norm(d) < r_verlet) insert(j);
+            }
         }
     }
     if (!spilled) {
