@@ -205,12 +205,18 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *s_red) {
 // Acceleration pass for particles g = g_first, g_first + g_step, ... < g_end
 // of system M (a workgroup strides over the whole system; a wide launch gives
 // each thread one particle). With LIDS the partner ids come from the system's
-// LDS copy `lids` (lds_ids_fill).
-template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false, bool FAC = true>
+// LDS copy `lids` (lds_ids_fill). after(g, x, v, w, a, alpha) runs once a
+// particle's accelerations are stored, with its state and accelerations
+// still in registers.
+struct no_after {
+    __device__ void operator()(uint32_t, d3, d3, d3, d3, d3) const {}
+};
+template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false, bool FAC = true, class AFTER = no_after>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                                  const pool_meta &M, uint32_t cur, const facet_dev *F,
                                                  uint32_t g_first, uint32_t g_end, uint32_t g_step,
-                                                 const uint16_t *lids = nullptr, uint32_t kmax = 0) {
+                                                 const uint16_t *lids = nullptr, uint32_t kmax = 0,
+                                                 AFTER after = AFTER{}) {
     pair_table T;
     T.pj_old = cur ? V.pj1 : V.pj0;
     T.ps_old = cur ? V.ps1 : V.ps0;
@@ -271,6 +277,7 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
             if (FAC && M.nf > 0) acc = acc + facet_fold(P, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
             store_acc<LDS>(V, M.offset, g, acc.lin, acc.ang);
+            after(g, xi, vi, wi, acc.lin, acc.ang);
         } else {
             accf acc{zerof(), zerof()};
             const mixed_consts m = pinned(MC);
@@ -284,7 +291,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
             }
             close_table<BIN>(W, T.pcnt_new, g, V.flags, V.sflags + M.slot);
             if (FAC && M.nf > 0) acc = acc + facet_fold_mx(P, MC, F, int(M.nf), V.ss, V.stride, g, xi, vi, wi);
-            store_acc<LDS>(V, M.offset, g, tod(acc.lin), tod(acc.ang));
+            const d3 a64 = tod(acc.lin), al64 = tod(acc.ang);
+            store_acc<LDS>(V, M.offset, g, a64, al64);
+            after(g, xi, vi, wi, a64, al64);
         }
     }
 }
@@ -623,6 +632,66 @@ __global__ void __launch_bounds__(kWideBlock) k_pool_wide_final_kick(pool_view V
     pool_final_kick<STATE_GLOBAL>(V, M, half, g, g + 1, 1);
 }
 
+// Fused wide steps (no wide system has facets, whose one-per-step advance
+// other blocks of the same launch would race): positions, velocities and
+// angular velocities alternate between two buffers, so that one launch can
+// read every particle's step-s state (Vi) while it writes step s+1's (Vo).
+// theta, the accelerations and the pair tables stay in place (Vi and Vo share
+// them; only a particle's own thread touches its theta and accelerations).
+// Kick/drift of the next step for particle g: pool_kick_drift's operations
+// with pending = true, on the state and accelerations the pass leaves in
+// registers, written into Vo (theta in place).
+__device__ __forceinline__ void kick_drift_io(const pool_view &Vi, const pool_view &Vo, double half, double dt,
+                                              uint32_t g, d3 x0, d3 vi, d3 wi, d3 ai, d3 ali, d3 th0) {
+    vi = vi + ai * half;
+    wi = wi + ali * half;
+    vi = vi + ai * half;
+    wi = wi + ali * half;
+    const d3 xi = x0 + vi * dt;
+    const d3 thi = th0 + wi * dt;
+    st3(Vo.x, g, xi);
+    st3(Vi.th, g, thi);
+    st3(Vo.v, g, vi);
+    st3(Vo.w, g, wi);
+}
+
+// The pass of step `pass` from Vi, then the kick/drift of step pass+1 into Vo.
+template <int BIN, int HAM, int PAIRS, int PREC>
+__global__ void __launch_bounds__(kWideBlock) k_pool_wide_step(hg_params P, mixed_consts MC, pool_view Vi,
+                                                               pool_view Vo, const wide_block *map, int pass,
+                                                               double half, double dt) {
+    const wide_block b = map[blockIdx.x];
+    const pool_meta M = Vi.meta[b.slot];
+    const uint32_t cur = M.cur ^ ((uint32_t(pass) + (M.acc_valid ? 0u : 1u)) & 1u);
+    const uint32_t g = M.offset + b.start + threadIdx.x;
+    if (g >= M.offset + M.n) return;
+    const facet_dev *F = Vi.facets + size_t(b.slot) * Vi.max_facets;
+    const d3 th0 = ld3(Vi.th, g);
+    pool_force_phase<BIN, HAM, PAIRS, PREC, STATE_GLOBAL, false, true>(
+        P, MC, Vi, M, cur, F, g, g + 1, 1, nullptr, 0, [&](uint32_t gg, d3 x0, d3 v0, d3 w0, d3 a, d3 al) {
+            kick_drift_io(Vi, Vo, half, dt, gg, x0, v0, w0, a, al, th0);
+        });
+}
+
+// Final kick of the fused steps: pool_final_kick's operations from Vi into
+// Vo (the pool's own buffers), positions copied across when they differ.
+__global__ void __launch_bounds__(kWideBlock) k_pool_wide_final_kick_io(pool_view Vi, pool_view Vo,
+                                                                        const wide_block *map, double half) {
+    const wide_block b = map[blockIdx.x];
+    const pool_meta M = Vi.meta[b.slot];
+    const uint32_t g = M.offset + b.start + threadIdx.x;
+    if (g >= M.offset + M.n) return;
+    const d3 vi = ld3(Vi.v, g) + ld3(Vi.a, g) * half;
+    const d3 wi = ld3(Vi.w, g) + ld3(Vi.al, g) * half;
+    if (Vi.x != Vo.x) st3(Vo.x, g, ld3(Vi.x, g));
+    st3(Vo.v, g, vi);
+    st3(Vo.w, g, wi);
+    if (nonfinite(vi, wi)) {
+        atomicOr(&Vi.flags[FLAG_NUMERIC], 1);
+        atomicOr(&Vi.sflags[M.slot], HG_FAULT_NUMERIC);
+    }
+}
+
 // Sums the per-particle ledger terms in particle order, the order sum_energy
 // uses on the host: lane c < 7 adds term c of every particle, then lane 0 adds
 // the seven sums; out[0..7].
@@ -874,23 +943,34 @@ hipError_t launch_chunk_p(const hg_params &p, const pool_view &V, const uint32_t
     return hipErrorInvalidValue;
 }
 
+// Vo null: the pass alone (k_pool_wide_force); else the fused step into Vo.
 template <int BIN, int HAM, int PAIRS>
 hipError_t launch_wide_force_t(const hg_params &p, const pool_view &V, const wide_block *map, unsigned blocks,
-                               int pass, hipStream_t s) {
-    if (p.precision == HG_PRECISION_MIXED)
-        k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_MIXED>
-            <<<blocks, kWideBlock, 0, s>>>(p, make_mixed_consts(p), V, map, pass);
-    else
-        k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_FP64>
-            <<<blocks, kWideBlock, 0, s>>>(p, make_mixed_consts(p), V, map, pass);
+                               int pass, hipStream_t s, const pool_view *Vo, double half, double dt) {
+    const mixed_consts mc = make_mixed_consts(p);
+    if (p.precision == HG_PRECISION_MIXED) {
+        if (Vo)
+            k_pool_wide_step<BIN, HAM, PAIRS, HG_PRECISION_MIXED>
+                <<<blocks, kWideBlock, 0, s>>>(p, mc, V, *Vo, map, pass, half, dt);
+        else
+            k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_MIXED><<<blocks, kWideBlock, 0, s>>>(p, mc, V, map, pass);
+    } else {
+        if (Vo)
+            k_pool_wide_step<BIN, HAM, PAIRS, HG_PRECISION_FP64>
+                <<<blocks, kWideBlock, 0, s>>>(p, mc, V, *Vo, map, pass, half, dt);
+        else
+            k_pool_wide_force<BIN, HAM, PAIRS, HG_PRECISION_FP64><<<blocks, kWideBlock, 0, s>>>(p, mc, V, map, pass);
+    }
     return hipGetLastError();
 }
 
 template <int PAIRS>
 hipError_t launch_wide_force_p(const hg_params &p, const pool_view &V, const wide_block *map, unsigned blocks,
-                               int pass, hipStream_t s) {
-#define HG_WCASE(B, H) \
-    if (p.binary_model == B && p.hamaker_model == H) return launch_wide_force_t<B, H, PAIRS>(p, V, map, blocks, pass, s);
+                               int pass, hipStream_t s, const pool_view *Vo = nullptr, double half = 0.0,
+                               double dt = 0.0) {
+#define HG_WCASE(B, H)                                                                                 \
+    if (p.binary_model == B && p.hamaker_model == H)                                                   \
+        return launch_wide_force_t<B, H, PAIRS>(p, V, map, blocks, pass, s, Vo, half, dt);
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_NONE)
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_FULL)
     HG_WCASE(HG_BINARY_NONE, HG_HAMAKER_LIMITED)
@@ -922,6 +1002,7 @@ struct hg_pool {
     size_t energy_cap = 0;
     hipStream_t stream = nullptr;
     double *x = nullptr, *v = nullptr, *th = nullptr, *w = nullptr, *a = nullptr, *al = nullptr;
+    double *xb = nullptr, *vb = nullptr, *wb = nullptr;  // second x, v, omega for the fused wide steps
     uint32_t cap_n = 0, *nb = nullptr, *ncnt = nullptr;
     uint32_t cap_s = 0, *pj[2] = {nullptr, nullptr}, *pcnt[2] = {nullptr, nullptr};
     uint32_t *bj = nullptr, *bcnt = nullptr;  // neck strengths (sinter_breaking)
@@ -966,6 +1047,7 @@ struct hg_pool {
     size_t tune_pos = 0;
     hipGraphExec_t wg_exec = nullptr;
     uint32_t wg_len = 0;
+    bool wg_fused = false;
     unsigned wg_blocks = 0;
     double wg_dt = 0.0;
     hipEvent_t wev[3] = {nullptr, nullptr, nullptr};
@@ -1199,7 +1281,7 @@ extern "C" {
 size_t hg_pool_bytes_per_particle(const hg_params *p, size_t max_facets) {
     if (!p) return 0;
     const size_t cap_n = pool_cap_n(*p), cap_s = pool_cap_s(*p);
-    size_t b = 6 * 32;                   // x, v, theta, omega, a, alpha
+    size_t b = 9 * 32;                   // x, v, theta, omega, a, alpha; second x, v, omega
     b += cap_n * 4 + 4;                  // neighbour slots + count
     b += 2 * (cap_s * (4 + 8 * kSpringRec) + 4);  // pair-state table, double buffered
     b += max_facets * 72;                // facet springs
@@ -1274,7 +1356,7 @@ hg_status hg_pool_create(const hg_params *params, size_t max_particles, size_t m
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream_w, hipStreamNonBlocking));
     PG_HIPC(hipStreamCreateWithFlags(&Q->stream_c, hipStreamNonBlocking));
     const size_t P = Q->P;
-    double **fields[6] = {&Q->x, &Q->v, &Q->th, &Q->w, &Q->a, &Q->al};
+    double **fields[9] = {&Q->x, &Q->v, &Q->th, &Q->w, &Q->a, &Q->al, &Q->xb, &Q->vb, &Q->wb};
     for (auto f : fields) PG_TRYC(palloc(Q, f, 4 * P));
     Q->cap_n = pool_cap_n(Q->p);
     Q->cap_s = pool_cap_s(Q->p);
@@ -1819,10 +1901,43 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
             HG_HIP(force(-1));
             Q->last_wide_launches++;
         }
-        // the interval's 2*len+1 launches: recorded once into a HIP graph
-        // per (interval length, dt, wide set) and replayed; launched one
-        // by one when graphs are off
+        // Without wide facets the steps are fused: the first kick/drift, then
+        // one launch per step doing its pass and the next step's kick/drift,
+        // x, v and omega alternating between the pool's buffers (A) and the
+        // second set (B), the last pass alone and a final kick that leaves
+        // the state in A: len + 2 launches instead of 2 len + 1.
+        bool wide_facets = false;
+        for (size_t k = 0; k < Q->order.size() - count; k++) wide_facets |= Q->meta[Q->order[k]].nf > 0;
+        const bool fused = !wide_facets;
+        pool_view VB = V;
+        VB.x = Q->xb;
+        VB.v = Q->vb;
+        VB.w = Q->wb;
+        auto enqueue_fused = [&](hipStream_t st_) -> hipError_t {
+            k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, 0, half, dt);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            for (uint32_t st = 0; st + 1 < len; st++) {
+                const pool_view &Vi = st % 2 ? VB : V, &Vo = st % 2 ? V : VB;
+                e = nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, Vi, Q->wide_map_d, Q->wide_blocks,
+                                                                     int(st), st_, &Vo, half, dt)
+                       : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, Vi, Q->wide_map_d, Q->wide_blocks, int(st), st_,
+                                                           &Vo, half, dt);
+                if (e != hipSuccess) return e;
+            }
+            const pool_view &Vl = (len - 1) % 2 ? VB : V;
+            e = nl ? launch_wide_force_p<HG_PAIRS_NEIGHBOR_LIST>(Q->p, Vl, Q->wide_map_d, Q->wide_blocks,
+                                                                 int(len - 1), st_)
+                   : launch_wide_force_p<HG_PAIRS_ALL>(Q->p, Vl, Q->wide_map_d, Q->wide_blocks, int(len - 1), st_);
+            if (e != hipSuccess) return e;
+            k_pool_wide_final_kick_io<<<Q->wide_blocks, kWideBlock, 0, st_>>>(Vl, V, Q->wide_map_d, half);
+            return hipGetLastError();
+        };
+        // the interval's launches: recorded once into a HIP graph per
+        // (interval length, dt, wide set, fused or not) and replayed;
+        // launched one by one when graphs are off
         auto enqueue = [&](hipStream_t st_) -> hipError_t {
+            if (fused) return enqueue_fused(st_);
             for (uint32_t st = 0; st < len; st++) {
                 k_pool_wide_kick_drift<<<Q->wide_blocks, kWideBlock, 0, st_>>>(V, Q->wide_map_d, st > 0 ? 1 : 0,
                                                                                 half, dt);
@@ -1837,7 +1952,8 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
             return hipGetLastError();
         };
         if (Q->wide_graph) {
-            if (!Q->wg_exec || Q->wg_len != len || Q->wg_dt != dt || Q->wg_blocks != Q->wide_blocks) {
+            if (!Q->wg_exec || Q->wg_len != len || Q->wg_dt != dt || Q->wg_blocks != Q->wide_blocks ||
+                Q->wg_fused != fused) {
                 if (Q->wg_exec) HG_HIP(hipGraphExecDestroy(Q->wg_exec));
                 Q->wg_exec = nullptr;
                 hipGraph_t graph = nullptr;
@@ -1850,6 +1966,7 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
                 (void)hipGraphDestroy(graph);
                 HG_HIP(ei);
                 Q->wg_len = len;
+                Q->wg_fused = fused;
                 Q->wg_dt = dt;
                 Q->wg_blocks = Q->wide_blocks;
             }
@@ -1857,7 +1974,7 @@ hg_status pool_begin(hg_pool *Q, double dt, uint64_t max_len, uint32_t rebuild_e
         } else {
             HG_HIP(enqueue(s2));
         }
-        Q->last_wide_launches += 2 * len + 1;
+        Q->last_wide_launches += fused ? len + 2 : 2 * len + 1;
         HG_HIP(hipEventRecord(Q->wev[1], s2));
     }
     if (count) {
