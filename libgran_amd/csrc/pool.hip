@@ -1738,24 +1738,47 @@ hg_status refresh_order(hg_pool *Q) {
     if (map.size() > Q->wide_map_cap) return fail(HG_ERR_INVALID, "hg_pool_step: wide map overflow");
     Q->narrow_count = uint32_t(Q->order.size() - first_narrow);
     Q->wide_blocks = unsigned(map.size());
-    // packs, largest first: a system of at least a workgroup's threads goes
-    // alone; a smaller one takes the smallest remaining systems beside it
-    // while their particles fit one thread each (neighbour-list mode)
+    // The queue, largest first: each system of at least a workgroup's
+    // threads alone, then (neighbour-list mode) the smaller ones packed best
+    // fit decreasing: each, in descending size, joins the fullest pack it
+    // fits (one particle per thread, at most kMaxPack members), or opens one.
     const bool pack = Q->packing && Q->p.pair_mode == HG_PAIRS_NEIGHBOR_LIST;
     const uint32_t tpb = uint32_t(Q->chunk_tpb);
     std::vector<uint32_t> slots, packs;
     slots.reserve(Q->narrow_count);
-    for (size_t lo = first_narrow, hi = Q->order.size(); lo < hi;) {
-        const uint32_t first = uint32_t(slots.size());
-        uint32_t s0 = Q->order[lo++], tot = Q->meta[s0].n, cnt = 1;
-        slots.push_back(s0);
-        while (pack && lo < hi && cnt < kMaxPack && tot + Q->meta[Q->order[hi - 1]].n <= tpb) {
-            const uint32_t s1 = Q->order[--hi];
-            slots.push_back(s1);
-            tot += Q->meta[s1].n;
-            cnt++;
+    size_t k = first_narrow;
+    for (; k < Q->order.size() && (!pack || Q->meta[Q->order[k]].n >= tpb); k++) {
+        packs.push_back(uint32_t(slots.size()) | (1u << 24));
+        slots.push_back(Q->order[k]);
+    }
+    if (k < Q->order.size()) {
+        std::vector<std::vector<uint32_t>> bins;
+        std::vector<uint32_t> fill;
+        std::vector<std::vector<uint32_t>> open(tpb + 1);  // bins with room for a member, by fill
+        for (; k < Q->order.size(); k++) {
+            const uint32_t s = Q->order[k], n = Q->meta[s].n;
+            uint32_t b = ~0u;
+            for (uint32_t f = tpb - n; f >= 1 && b == ~0u; f--)
+                if (!open[f].empty()) {
+                    b = open[f].back();
+                    open[f].pop_back();
+                }
+            if (b == ~0u) {
+                b = uint32_t(bins.size());
+                bins.emplace_back();
+                fill.push_back(0);
+            }
+            bins[b].push_back(s);
+            fill[b] += n;
+            if (bins[b].size() < kMaxPack) open[fill[b]].push_back(b);
         }
-        packs.push_back(first | (cnt << 24));
+        std::vector<uint32_t> idx(bins.size());
+        for (uint32_t b = 0; b < idx.size(); b++) idx[b] = b;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return fill[a] > fill[b]; });
+        for (uint32_t b : idx) {
+            packs.push_back(uint32_t(slots.size()) | (uint32_t(bins[b].size()) << 24));
+            slots.insert(slots.end(), bins[b].begin(), bins[b].end());
+        }
     }
     Q->pack_count = uint32_t(packs.size());
     if (Q->narrow_count) {
