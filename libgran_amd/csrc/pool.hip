@@ -211,6 +211,18 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *s_red) {
 struct no_after {
     __device__ void operator()(uint32_t, d3, d3, d3, d3, d3) const {}
 };
+// The FP64 pair path's bond constants held in vector registers for a pass
+// (as pinned() does for the mixed path): kept as scalars they are spilled and
+// re-read lane by lane inside the pair loop.
+__device__ __forceinline__ hg_params pinned64(const hg_params &P) {
+    hg_params p = P;
+    hg_sinter_params &s = p.sinter;
+    double *f[] = {&s.k, &s.gamma_n, &s.k_t, &s.gamma_t, &s.k_r, &s.gamma_r,
+                   &s.k_o, &s.gamma_o, &s.r_part, &s.mass, &s.inertia, &s.dt};
+#pragma unroll
+    for (double *q : f) pin_vgpr(*q);
+    return p;
+}
 template <int BIN, int HAM, int PAIRS, int PREC, int LDS, bool LIDS = false, bool FAC = true, class AFTER = no_after>
 __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed_consts &MC, const pool_view &V,
                                                  const pool_meta &M, uint32_t cur, const facet_dev *F,
@@ -266,8 +278,9 @@ __device__ __forceinline__ void pool_force_phase(const hg_params &P, const mixed
         auto vw_of = [&](uint32_t j) { return [&V, &M, j](d3 &v, d3 &w) { load_vw<LDS>(V, M.offset, j, v, w); }; };
         if constexpr (PREC == HG_PRECISION_FP64) {
             acc6 acc{zero3(), zero3()};
+            const hg_params pp = pinned64(P);
             if constexpr (PAIRS == HG_PAIRS_NEIGHBOR_LIST) {
-                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_vw<BIN, HAM>(P, j, xi, vi, wi, xj, vw, W, acc); });
+                partners([&](uint32_t j, d3 xj, auto &&vw) { pair_fold_vw<BIN, HAM>(pp, j, xi, vi, wi, xj, vw, W, acc); });
             } else {
                 for (uint32_t j = M.offset; j < end; j++) {
                     if (j == g) continue;
