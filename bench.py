@@ -523,6 +523,8 @@ def main():
     ap.add_argument("--only-batched", action="store_true", help="skip the comparison legs (profiling runs)")
     ap.add_argument("--chunk-shape", default=None, help="THREADS:WG_PER_CU chunk-kernel shape (default: tuned)")
     ap.add_argument("--only-single", action="store_true", help="only the N=20,000 FP64 per-step path (profiling)")
+    ap.add_argument("--only-fp64-pool", action="store_true",
+                    help="only the batch in an FP64 pool (profiling the FP64 chunk kernel)")
     ap.add_argument("--only-small", action="store_true",
                     help="only the reference's 19-particle lattice scenarios (test/hamaker.cpp, hamaker_neighbor.cpp) "
                          "as single systems: one launch per interval against per-step launches")
@@ -576,6 +578,22 @@ def main():
         # batch, at which the committed counter profiles were taken
         split = PROFILED_SPLIT
         chunk_shape = chunk_shape or PROFILED_SHAPE
+    if args.only_fp64_pool:
+        pool, el, prof, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_FP64, sizes, xs, vs, args.steps,
+                                                              args.warmup, split, chunk_shape, not args.no_pack)
+        pool.close()
+        c = max(prof["chunks"], 1)
+        us = prof["chunk_ms"] * 1e3 / c
+        spl = prof["chunk_steps"] / c
+        b = state_bytes_per_particle_step(kbar, cbar) * narrow * spl
+        print(json.dumps({"kernel": "k_pool_chunk<SINTER,LIMITED,NEIGHBOR_LIST,FP64>", "value": total * args.steps / el,
+                          "unit": "particle-steps/s", "avg_launch_us": round(us, 1), "launches": int(prof["chunks"]),
+                          "steps_per_launch": spl, "particles_per_launch": narrow, "alg_bytes_per_launch": int(b),
+                          "kbar": round(kbar, 3), "cbar": round(cbar, 3),
+                          "frac": round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                          "launch_split": f"{tuning['wide_min']}:{tuning['wide_cus']}",
+                          "chunk_shape": f"{tuning['chunk_threads']} x {tuning['chunk_wg_per_cu']}"}))
+        return
     pool, el_mx, prof_mx, kbar, cbar, tuning, narrow = run_pool(L, L.PRECISION_MIXED, sizes, xs, vs, args.steps,
                                                                  args.warmup, split, chunk_shape,
                                                                  not args.no_pack)
