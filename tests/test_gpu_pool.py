@@ -177,6 +177,32 @@ def test_pool_packed_small_systems_equal_alone(precision, shape):
             same(alone[k], mix.state(s), f"packing {packing}, n {sizes[k]}")
 
 
+def test_pool_many_packs_equal_alone():
+    """Thirty-two small systems packed best fit decreasing into several packs,
+    among them packs held to the eight-member cap (eight 50-particle systems
+    fill only 400 of 512 threads) beside systems too large to pack: each
+    result equals the system alone, FP64, and one packed system equals the
+    oracle."""
+    announce("many packs")
+    L = _L()
+    sizes = [50] * 12 + [64, 90, 120, 150, 200, 260, 333, 410, 470, 500, 511] + [520, 700] + [75] * 7
+    cases = [agg_case(n, 200 + k) for k, n in enumerate(sizes)]
+    alone = []
+    for x, v0 in cases:
+        P = L.Pool(nl_params(0), max_particles=1000, max_systems=1)
+        s = P.add(x, v0)
+        P.step(DT, 60, rebuild_every=50)
+        alone.append(P.state(s))
+    o = oracle_nl(*cases[0])
+    o.step(DT, 60, rebuild_every=50)
+    same(o.state(), alone[0], "oracle vs alone")
+    mix = L.Pool(nl_params(0), max_particles=int(sum(sizes)) + 64, max_systems=len(sizes))
+    slots = [mix.add(x, v0) for x, v0 in cases]
+    mix.step(DT, 60, rebuild_every=50)
+    for k, s in enumerate(slots):
+        same(alone[k], mix.state(s), f"n {sizes[k]} (system {k})")
+
+
 def test_pool_continuous_admission_and_retirement():
     """Systems join and leave between steps; each result equals its own
     alone-run with the same number of steps."""
